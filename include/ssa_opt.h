@@ -1,0 +1,197 @@
+/*
+ * ssa_opt.h -- drop-in C API of SSA-Opt's decomposition hot path, backed by
+ * an MI355X (gfx950) device path.
+ *
+ * Every declaration below replaces the function of the same name in the
+ * reference header MKL/ssa_opt.h (declarations at MKL/ssa_opt.h:318-430,
+ * definitions cited per entry) and keeps its C ABI: cdecl, LP64, int = 32
+ * bits, double data, caller-allocated outputs, 0 / -1 returns, getters that
+ * return a count, double-returning calls that return 0.0 on error.  The
+ * reference's Python ctypes wrapper (MKL/py/ssa_wrapper.py:125-291) binds
+ * these symbols unchanged; it treats the context as an opaque 1024-byte
+ * block, so sizeof(SSA_Opt) and sizeof(MSSA_Opt) stay below 1024.
+ *
+ * The context keeps the reference's public result fields (dimensions,
+ * flags, U, V, sigma, eigenvalues, n_components, total_variance) valid on
+ * the host after each decomposition; the device state hangs off `dev`.
+ *
+ * Differences from the reference, all deliberate:
+ *   - decompose_randomized performs q = 2 subspace (power) iterations by
+ *     default (the reference has none); ssa_amd_set_options changes q.
+ *   - the Gaussian test matrix comes from a counter-based Philox4x32-10
+ *     stream keyed by (seed, decomposition number) instead of MKL's MT2203;
+ *     it can be injected with ssa_amd_inject_omega for parity checks.
+ *   - fft_len is the device correlation length (a power of two >= the
+ *     segment length), not next_pow2(N + K - 1).
+ *   - ssa_opt_decompose / _decompose_block / _extend run the same device
+ *     subspace iteration with q derived from max_iter (documented below).
+ */
+#ifndef SSA_OPT_H
+#define SSA_OPT_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double ssa_real;
+
+/* MKL/ssa_opt.h:145-225 */
+typedef struct {
+    int N, L, K;
+    int fft_len, r2c_len, batch_size;
+    /* host views kept for C callers that check them (MKL/test/ssa_opt_test.c:179-181) */
+    ssa_real *fft_x;            /* first segment spectrum, r2c_len complex (mirror) */
+    ssa_real *ws_real, *ws_complex, *ws_real2, *ws_proj, *ws_batch_real, *ws_batch_complex;
+    /* results (host mirrors, refreshed by every decomposition) */
+    ssa_real *U;                /* L x n_components, column-major */
+    ssa_real *V;                /* K x n_components, column-major */
+    ssa_real *sigma;
+    ssa_real *eigenvalues;
+    int n_components;
+    ssa_real *inv_diag_count;   /* 1/min(t+1, L, K, N-t) */
+    ssa_real *U_fft, *V_fft;    /* unused (spectra stay on the device) */
+    bool fft_cached;
+    int prepared_kp, prepared_lwork;
+    ssa_real *decomp_Omega, *decomp_Y;   /* unused host views (device resident) */
+    bool prepared, initialized, decomposed;
+    ssa_real total_variance;    /* sum of the k computed eigenvalues (MKL/ssa_opt.h:1741-1746) */
+    /* device path */
+    void *dev;
+    int q;                      /* power iterations of decompose_randomized */
+    int precision;              /* 0: plain FP64, 1: mixed (FP32 bulk, FP64 small solves) */
+    unsigned long long seed;    /* Philox key of the test matrix */
+    unsigned long long calls;   /* decompositions performed (Philox counter word) */
+} SSA_Opt;
+
+/* MKL/ssa_opt.h:227-249 */
+typedef struct {
+    int n;
+    ssa_real *singular_values;
+    ssa_real *log_sv;
+    ssa_real *gaps;
+    ssa_real *cumulative_var;
+    ssa_real *second_diff;
+    int suggested_signal;
+    ssa_real gap_threshold;
+} SSA_ComponentStats;
+
+/* MKL/ssa_opt.h:251-268 */
+typedef struct {
+    ssa_real *R;
+    int L;
+    ssa_real verticality;
+    bool valid;
+} SSA_LRF;
+
+/* MKL/ssa_opt.h:271-315 */
+typedef struct {
+    int M, N, L, K;
+    int fft_len, r2c_len;
+    ssa_real *U;                /* (M L) x n_components */
+    ssa_real *V;                /* K x n_components */
+    ssa_real *sigma;
+    ssa_real *eigenvalues;
+    int n_components;
+    ssa_real *inv_diag_count;
+    bool initialized, decomposed;
+    ssa_real total_variance;
+    void *dev;
+    int q, precision;
+    unsigned long long seed, calls;
+} MSSA_Opt;
+
+/* ---- lifecycle (MKL/ssa_opt.h:1026-1558) ---- */
+int ssa_opt_init(SSA_Opt *ssa, const ssa_real *x, int N, int L);                 /* :1026-1284 */
+void ssa_opt_free(SSA_Opt *ssa);                                                  /* :1286-1350 */
+int ssa_opt_prepare(SSA_Opt *ssa, int max_k, int oversampling);                   /* :1414-1543 */
+void ssa_opt_free_prepared(SSA_Opt *ssa);                                         /* :1352-1379 */
+int ssa_opt_update_signal(SSA_Opt *ssa, const ssa_real *new_x);                   /* :1545-1558 */
+
+/* ---- decomposition ---- */
+int ssa_opt_decompose_randomized(SSA_Opt *ssa, int k, int oversampling);         /* :1685-1763 */
+/* sequential power iteration in the reference (:1560-1678); here the device
+ * subspace iteration with b = min(k + 8, L, K) and q = clamp(max_iter / 4, 2, 30) */
+int ssa_opt_decompose(SSA_Opt *ssa, int k, int max_iter);
+/* block power iteration in the reference (:1765-1933); here the device
+ * subspace iteration with b = min(max(k + 8, block_size), L, K), q as above */
+int ssa_opt_decompose_block(SSA_Opt *ssa, int k, int block_size, int max_iter);
+int ssa_opt_extend(SSA_Opt *ssa, int additional_k, int max_iter);               /* :1935-2056 */
+
+/* Hankel products y = H v (v: K, y: L) and z = H^T u (u: L, z: K); static
+ * in the reference (:871-903) and called by its tests (MKL/test/ssa_opt_test.c:1484, :1533) */
+void ssa_opt_hankel_matvec(SSA_Opt *ssa, const ssa_real *v, ssa_real *y);
+void ssa_opt_hankel_matvec_T(SSA_Opt *ssa, const ssa_real *u, ssa_real *y);
+
+/* ---- reconstruction and analysis ---- */
+int ssa_opt_reconstruct(const SSA_Opt *ssa, const int *group, int n_group, ssa_real *output);   /* :2325-2400 */
+int ssa_opt_cache_ffts(SSA_Opt *ssa);                                             /* :2110-2323 */
+void ssa_opt_free_cached_ffts(SSA_Opt *ssa);                                      /* :2058-2078 */
+int ssa_opt_wcorr_matrix(const SSA_Opt *ssa, ssa_real *W);                        /* :2407-2477 */
+int ssa_opt_wcorr_matrix_fast(const SSA_Opt *ssa, ssa_real *W);                   /* :2492-2548 */
+ssa_real ssa_opt_wcorr_pair(const SSA_Opt *ssa, int i, int j);                    /* :2551-2625 */
+int ssa_opt_component_stats(const SSA_Opt *ssa, SSA_ComponentStats *stats);      /* :2627-2676 */
+void ssa_opt_component_stats_free(SSA_ComponentStats *stats);                     /* :2678-2688 */
+int ssa_opt_find_periodic_pairs(const SSA_Opt *ssa, int *pairs, int max_pairs,
+                                ssa_real sv_tol, ssa_real wcorr_thresh);          /* :2690-2756 */
+int ssa_opt_get_trend(const SSA_Opt *ssa, ssa_real *output);                      /* :3086-3090 */
+int ssa_opt_get_noise(const SSA_Opt *ssa, int noise_start, ssa_real *output);     /* :3091-3110 */
+ssa_real ssa_opt_variance_explained(const SSA_Opt *ssa, int start, int end);      /* :3111-3121 */
+int ssa_opt_get_singular_values(const SSA_Opt *ssa, ssa_real *output, int max_n); /* :3123-3130 */
+int ssa_opt_get_eigenvalues(const SSA_Opt *ssa, ssa_real *output, int max_n);     /* :3132-3139 */
+ssa_real ssa_opt_get_total_variance(const SSA_Opt *ssa);                          /* :3141-3145 */
+
+/* ---- forecasting (MKL/ssa_opt.h:2758-2885) ---- */
+int ssa_opt_compute_lrf(const SSA_Opt *ssa, const int *group, int n_group, SSA_LRF *lrf);
+void ssa_opt_lrf_free(SSA_LRF *lrf);
+int ssa_opt_forecast_with_lrf(const SSA_LRF *lrf, const ssa_real *base_signal, int base_len, int n_forecast,
+                              ssa_real *output);
+int ssa_opt_forecast(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output);
+int ssa_opt_forecast_full(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output);
+
+/* ---- MSSA (MKL/ssa_opt.h:3881-4283) ---- */
+int mssa_opt_init(MSSA_Opt *mssa, const ssa_real *X, int M, int N, int L);
+int mssa_opt_decompose(MSSA_Opt *mssa, int k, int oversampling);
+int mssa_opt_reconstruct(const MSSA_Opt *mssa, int series_idx, const int *group, int n_group, ssa_real *output);
+int mssa_opt_reconstruct_all(const MSSA_Opt *mssa, const int *group, int n_group, ssa_real *output);
+int mssa_opt_series_contributions(const MSSA_Opt *mssa, ssa_real *contributions);
+ssa_real mssa_opt_variance_explained(const MSSA_Opt *mssa, int start, int end);
+void mssa_opt_free(MSSA_Opt *mssa);
+
+/* ---- extensions of the device path (no counterpart in the reference) ---- */
+/* q power iterations, precision (0 plain FP64, 1 mixed), Philox seed; call
+ * after init.  A precision change rebuilds the device state from the
+ * host copy of the signal. */
+int ssa_amd_set_options(SSA_Opt *ssa, int q, int precision, unsigned long long seed);
+int mssa_amd_set_options(MSSA_Opt *mssa, int q, int precision, unsigned long long seed);
+/* use this K x b column-major Gaussian matrix (b = min(k + p, L, K)) for the
+ * next decompose_randomized instead of drawing one */
+int ssa_amd_inject_omega(SSA_Opt *ssa, const ssa_real *Omega, int K, int b);
+/* reconstruct n_groups groups in one launch; groups concatenated in glist,
+ * sizes in gcount; output n_groups * N values */
+int ssa_amd_reconstruct_many(const SSA_Opt *ssa, const int *glist, const int *gcount, int n_groups,
+                             ssa_real *output);
+/* the device correlation plan of the context: out[0..5] = n, CL, CK, AL, SK, Jacobi sweeps */
+int ssa_amd_plan_info(const SSA_Opt *ssa, int *out6);
+
+/* batched path: many uneven series in one device engine */
+typedef struct ssa_amd_batch ssa_amd_batch;
+ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const int *k, int p, int q,
+                                    int precision, unsigned long long seed);
+/* signals concatenated as float32 (offsets[i] = first sample of series i) */
+int ssa_amd_batch_set_signals_f32(ssa_amd_batch *b, const float *cat, const long long *offsets);
+/* one SSA of every series: spectra, decomposition, reconstruction of all
+ * components; use_graph captures the launch sequence in a HIP graph */
+int ssa_amd_batch_run(ssa_amd_batch *b, int use_graph);
+int ssa_amd_batch_sync(ssa_amd_batch *b);
+int ssa_amd_batch_sigma(ssa_amd_batch *b, int i, ssa_real *sigma);       /* returns k_i */
+int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out);         /* N_i values */
+long long ssa_amd_batch_bytes(const ssa_amd_batch *b);
+void ssa_amd_batch_free(ssa_amd_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -112,23 +112,25 @@ void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint3
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-/* out[c*rows + r] ~ N(0,1), column-major rows x cols.  Element pair
- * (2j, 2j+1) of column c comes from counter (j, c, call_lo, call_hi) under
- * key (seed_lo, seed_hi); two 53-bit uniforms feed one Box-Muller draw. */
+/* out[c*rows + r] ~ N(0,1), column-major rows x cols.  Row r of the column
+ * pair (2j, 2j+1) comes from one Philox call with counter (r, j, call_lo,
+ * call_hi) under key (seed_lo, seed_hi): two 53-bit uniforms u1, u2 give the
+ * Box-Muller pair rad*cos(2 pi u2) -> column 2j, rad*sin(2 pi u2) -> 2j+1,
+ * rad = sqrt(-2 ln(1 - u1)).  The device path draws the same values. */
 void orc_philox_gaussian(uint64_t seed, uint64_t call, int rows, int cols, double *out)
 {
     uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
-    for (int c = 0; c < cols; c++) {
-        for (int j = 0; 2 * j < rows; j++) {
-            uint32_t ctr[4] = { (uint32_t)j, (uint32_t)c, (uint32_t)call, (uint32_t)(call >> 32) };
-            uint32_t r[4];
-            orc_philox4x32_10(ctr, key, r);
-            double u1 = (double)(((uint64_t)(r[0] >> 5) << 26) | (r[1] >> 6)) * 0x1p-53;
-            double u2 = (double)(((uint64_t)(r[2] >> 5) << 26) | (r[3] >> 6)) * 0x1p-53;
+    for (int j = 0; 2 * j < cols; j++) {
+        for (int r = 0; r < rows; r++) {
+            uint32_t ctr[4] = { (uint32_t)r, (uint32_t)j, (uint32_t)call, (uint32_t)(call >> 32) };
+            uint32_t o[4];
+            orc_philox4x32_10(ctr, key, o);
+            double u1 = (double)(((uint64_t)(o[0] >> 5) << 26) | (o[1] >> 6)) * 0x1p-53;
+            double u2 = (double)(((uint64_t)(o[2] >> 5) << 26) | (o[3] >> 6)) * 0x1p-53;
             double rad = sqrt(-2.0 * log(1.0 - u1));
             double ang = 6.283185307179586476925286766559 * u2;
-            out[(size_t)c * rows + 2 * j] = rad * cos(ang);
-            if (2 * j + 1 < rows) out[(size_t)c * rows + 2 * j + 1] = rad * sin(ang);
+            out[(size_t)(2 * j) * rows + r] = rad * cos(ang);
+            if (2 * j + 1 < cols) out[(size_t)(2 * j + 1) * rows + r] = rad * sin(ang);
         }
     }
 }

@@ -1,0 +1,114 @@
+/*
+ * ssa_device.h -- the thin C ABI between the host C code (ssa_opt.c,
+ * ssa_batch.c) and the HIP kernels (ssa_kernels.hip).  Plain C types only.
+ *
+ * A "series" is one SSA (M = 1) or MSSA (M > 1) problem resident in HBM.
+ * Its Hankel operator H (rows: M stacked windows of length L; columns: K) is
+ * applied chunk by chunk: rows in chunks of CL (AL chunks per channel),
+ * columns in chunks of CK (SK chunks), so that every piece is a correlation
+ * of length n >= CL + CK - 1 that fits one workgroup's LDS.  The spectra of
+ * the signal segments x_m[a CL + s CK : ... + CL + CK - 1] are cached in HBM.
+ */
+#ifndef SSA_DEVICE_H
+#define SSA_DEVICE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SSA_BMAX 80           /* max block width k + p */
+#define SSA_GRAM_TR 32        /* rows per LDS sub-tile of Gram / apply */
+#define SSA_NMAX_LOG 16       /* twiddle table length 2^16 */
+
+enum { SSA_PREC_F64 = 0, SSA_PREC_F32 = 1 };
+
+/* per-series descriptor (device resident, one per series of a batch) */
+typedef struct ssa_desc {
+    int N, L, K, M;           /* series length, window, K = N-L+1, channels */
+    int ML;                   /* M*L rows of the stacked trajectory */
+    int b, k;                 /* block width (k+p clamped), components */
+    int n, lgn;               /* correlation length (power of two) */
+    int CL, CK, AL, SK;       /* chunk sizes and counts (AL per channel) */
+    int tilesY, tilesZ;       /* Gram row tiles of the ML x b and K x b panels */
+    int status;               /* set non-zero by device code on failure */
+    int pad0;
+    /* element offsets into the bulk arena (T = double or float) */
+    long long off_x;          /* M*N signal values */
+    long long off_spec;       /* M*AL*SK spectra, n complex each (2n T) */
+    long long off_Y;          /* ML x b, ld ML */
+    long long off_Z;          /* K x b, ld K */
+    long long off_U;          /* ML x k, ld ML */
+    long long off_V;          /* K x k, ld K */
+    long long off_stage;      /* reconstruction staging */
+    long long off_out;        /* reconstruction outputs */
+    long long off_w;          /* N diagonal weights 1/min(t+1, L, K, N-t) */
+    /* double offsets into the small (FP64) arena */
+    long long off_part;       /* Gram partials: tiles x (b*b + b) */
+    long long off_small;      /* small-matrix slots, see SSA_SLOT_* */
+    unsigned long long seed;  /* Philox key of this series' test matrix */
+    unsigned long long call;  /* decomposition counter (Philox counter word) */
+    long long stage_cap;      /* staging capacity in T elements */
+    long long out_cap;        /* output capacity in T elements */
+} ssa_desc;
+
+/* FP64 small-matrix slots, each SSA_BMAX*SSA_BMAX doubles, relative to off_small */
+enum {
+    SSA_SLOT_R1 = 0, SSA_SLOT_R1INV, SSA_SLOT_RY, SSA_SLOT_RINVY, SSA_SLOT_RZ, SSA_SLOT_RINVZ,
+    SSA_SLOT_W, SSA_SLOT_WK, SSA_SLOT_WKS, SSA_SLOT_MISC, SSA_NSLOTS
+};
+/* SSA_SLOT_MISC layout (doubles): [0, BMAX) colsum of raw Y, [BMAX, 2BMAX) colsum of Q,
+ * [2BMAX, 3BMAX) sigma, [3BMAX, 4BMAX) lambda, 4BMAX: total variance,
+ * 4BMAX+1: Jacobi sweeps used, 4BMAX+2: dead columns seen */
+#define SSA_MISC_COLSUMY 0
+#define SSA_MISC_COLSUMQ (SSA_BMAX)
+#define SSA_MISC_SIGMA (2 * SSA_BMAX)
+#define SSA_MISC_LAMBDA (3 * SSA_BMAX)
+#define SSA_MISC_TOTVAR (4 * SSA_BMAX)
+#define SSA_MISC_SWEEPS (4 * SSA_BMAX + 1)
+#define SSA_MISC_DEAD (4 * SSA_BMAX + 2)
+
+typedef struct { int s, x, y, z; } ssa_item;
+
+/* panel selectors */
+enum { SSA_PANEL_Y = 0, SSA_PANEL_Z = 1, SSA_PANEL_U = 2, SSA_PANEL_V = 3 };
+
+/* ---- launchers (return 0 or a hipError_t value) ---- */
+int ssa_dev_twiddles(int prec, void **table_out, void *stream);
+int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items, int n_items,
+                    void *bulk, const void *tw, void *stream);
+/* dir 0: out = H * in (in: K-dim panel, out: L-dim panel); dir 1: out = H^T * in.
+ * gen_omega: the input is the Philox test matrix, generated on the fly. */
+int ssa_dev_matvec(int prec, int lgn, int dir, int gen_omega, int in_panel, int out_panel,
+                   const ssa_desc *D, const ssa_item *items, int n_items,
+                   void *bulk, const void *tw, void *stream);
+int ssa_dev_gram(int prec, int panel, int use_m, int bmax, const ssa_desc *D, const ssa_item *items, int n_items,
+                 void *bulk, double *small, void *stream);
+int ssa_dev_chol(int panel, int pass, const ssa_desc *D, const ssa_item *items, int n_items,
+                 double *small, void *stream);
+int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, const ssa_desc *D,
+                  const ssa_item *items, int n_items, void *bulk, double *small, void *stream);
+int ssa_dev_svd(const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream);
+/* reconstruction: items (s, gid, row chunk o, col chunk sk); groups: gdesc[gid] = {start, count},
+ * glist = component indices; scale_sigma 1: sum_g sigma_g u_g (*) v_g, 0: u (*) v */
+int ssa_dev_recon(int prec, int lgn, int scale_sigma, const ssa_desc *D, const ssa_item *items,
+                  int n_items, const int *gdesc, const int *glist, void *bulk, const double *small,
+                  const void *tw, void *stream);
+/* overlap-add of the staging pieces: items (s, gid, t tile); weight 0: w_t, 1: sqrt(w_t);
+ * channel index taken from gdesc[gid].z.  Output of gid goes to out + gid*N. */
+int ssa_dev_olap(int prec, int weight, const ssa_desc *D, const ssa_item *items, int n_items,
+                 const int *gdesc, void *bulk, void *stream);
+/* W-correlation: Gram of the sqrt(w)-weighted component panel (stored at off_out, N x k)
+ * then normalisation; W written row-major k x k to small + off_small + SLOT_W */
+int ssa_dev_wcorr(int prec, const ssa_desc *D, const ssa_item *tile_items, int n_tiles,
+                  const ssa_item *series_items, int n_series, void *bulk, double *small, void *stream);
+/* LRF + forecast: per series, group gdesc[0]; reads the reconstruction at off_out, writes
+ * h forecasts after it (off_out + N) */
+int ssa_dev_forecast(int prec, int h, const ssa_desc *D, const ssa_item *items, int n_items,
+                     const int *gdesc, const int *glist, void *bulk, double *small, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,878 @@
+/*
+ * ssa_engine.c -- host C engine of the device-resident SSA path.
+ *
+ * Sequence of one decomposition (every series of the engine in lockstep),
+ * restating MKL/ssa_opt.h:1685-1763 with q subspace iterations:
+ *
+ *   Y = H Omega                     k_matvec (Omega drawn on the fly or injected)
+ *   Q = orth(Y)                     CholeskyQR2: gram, chol, gram(Y R1^-1), chol, apply
+ *   repeat q: Z = orth(H^T Q); Q = orth(H Z)
+ *   B = H^T Q                       k_matvec
+ *   R_B = chol-QR2 factor of B      gram, chol, gram, chol (no Q_B needed)
+ *   R_B = Ur S W^T                  k_svd (one-sided Jacobi, sign fix)
+ *   U = Q W_k, V = B W_k S^-1       k_apply
+ *
+ * The bulk arena offsets of a descriptor are element offsets from the base
+ * pointer passed to the kernels (the arena of the signals); the panel arena
+ * is a separate allocation addressed through the same base.
+ */
+#include "ssa_engine.h"
+
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NLG 14                 /* lgn buckets 0..13 (6..13 used) */
+#define ROWS_PER_WG 512        /* must match GRAM_ROWS_PER_WG of the kernels */
+#define OLAP_TILE 1024
+
+typedef struct { long long off; int n; } span;
+
+struct ssa_engine {
+    int prec, count;
+    size_t esz;
+    ssa_desc *hd;
+    int *bmax, *kmax;
+    ssa_desc *dd;
+    void *arena_a;             /* signals + spectra (base pointer of all offsets) */
+    void *arena_b;             /* panels, staging, outputs */
+    long long a_elems, b_elems;
+    double *small;
+    long long small_elems;
+    void *tw;
+    hipStream_t st;
+    int prepared, max_groups, max_h;
+    /* work lists */
+    ssa_item *ditems;
+    long long ditems_cap;
+    ssa_item *hitems;
+    long long hitems_n, hitems_cap;
+    span spec[NLG], mvf[NLG], mva[NLG], tY, tZ, ser;
+    span rec_step[NLG], olap_step;
+    int lists_dirty, desc_dirty;
+    int dyn_base;              /* first item of the per-call region */
+    /* groups (int4 descriptors followed by component lists) */
+    int *dgroups;
+    long long dgroups_cap;
+    int step_groups_ready;
+    /* pinned host staging */
+    void *hpin;
+    size_t hpin_bytes;
+    /* graph of the batched step */
+    hipGraphExec_t gexec;
+    int graph_q;
+    int has_graph;
+};
+
+/* ------------------------------------------------------------------------ */
+/* planner                                                                   */
+/* ------------------------------------------------------------------------ */
+
+static int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+static double plan_cost(int n, int lgn, int M, int AL, int SK, int b, int k, int q)
+{
+    int nLc = M * AL;
+    double fwd = (SK == 1) ? 1.0 + nLc : (double)nLc * (SK + 1);
+    double adj = (nLc == 1) ? 1.0 + SK : (double)SK * (nLc + 1);
+    double pairs = ceil_div(b > 0 ? b : 2, 2);
+    double transforms = (q + 1) * pairs * (fwd + adj) + (double)AL * SK * (k + 1);
+    return transforms * (double)n * (lgn + 3);
+}
+
+int ssa_plan_series(int prec, int N, int L, int M, int b, int k, int q, int lgn_force, ssa_plan *out)
+{
+    (void)prec;
+    int K = N - L + 1;
+    ssa_plan best;
+    best.cost = -1;
+    for (int lgn = 6; lgn <= 13; lgn++) {
+        if (lgn_force >= 6 && lgn_force <= 13 && lgn != lgn_force) continue;
+        int n = 1 << lgn;
+        int cand[3][2];
+        int nc = 0;
+        if (L + 1 <= n) { cand[nc][0] = L; cand[nc][1] = n - L + 1; nc++; }
+        if (K + 1 <= n) { cand[nc][0] = n - K + 1; cand[nc][1] = K; nc++; }
+        cand[nc][0] = n / 2; cand[nc][1] = n / 2; nc++;
+        for (int c = 0; c < nc; c++) {
+            int CL = cand[c][0] < L ? cand[c][0] : L;
+            int CK = cand[c][1] < K ? cand[c][1] : K;
+            if (CL < 1 || CK < 1 || CL + CK - 1 > n) continue;
+            int AL = ceil_div(L, CL), SK = ceil_div(K, CK);
+            double cost = plan_cost(n, lgn, M, AL, SK, b, k, q);
+            if (best.cost < 0 || cost < best.cost) {
+                best.n = n; best.lgn = lgn; best.CL = CL; best.CK = CK; best.AL = AL; best.SK = SK;
+                best.cost = cost;
+            }
+        }
+    }
+    if (best.cost < 0) return -1;
+    *out = best;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* helpers                                                                   */
+/* ------------------------------------------------------------------------ */
+
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+    fprintf(stderr, "ssa_engine: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+    return -1; } } while (0)
+#define DEVCHK(x) do { int e_ = (x); if (e_ != 0) { \
+    fprintf(stderr, "ssa_engine: %s failed: %d (%s:%d)\n", #x, e_, __FILE__, __LINE__); return -1; } } while (0)
+
+static long long align64(long long v) { return (v + 63) & ~63LL; }
+
+static void *g_tw[2];          /* twiddle tables per precision (process lifetime, device 0) */
+
+static int pin_reserve(ssa_engine *e, size_t bytes)
+{
+    if (e->hpin_bytes >= bytes) return 0;
+    if (e->hpin) hipHostFree(e->hpin);
+    e->hpin = NULL;
+    size_t nb = bytes < (1 << 20) ? (1 << 20) : bytes;
+    HIPCHK(hipHostMalloc(&e->hpin, nb, hipHostMallocDefault));
+    e->hpin_bytes = nb;
+    return 0;
+}
+
+static int items_push(ssa_engine *e, int s, int x, int y, int z)
+{
+    if (e->hitems_n == e->hitems_cap) {
+        long long nc = e->hitems_cap ? 2 * e->hitems_cap : 4096;
+        ssa_item *p = (ssa_item *)realloc(e->hitems, sizeof(ssa_item) * nc);
+        if (!p) return -1;
+        e->hitems = p;
+        e->hitems_cap = nc;
+    }
+    ssa_item it = {s, x, y, z};
+    e->hitems[e->hitems_n++] = it;
+    return 0;
+}
+
+static int items_upload(ssa_engine *e, long long from)
+{
+    long long n = e->hitems_n;
+    if (n > e->ditems_cap) {
+        if (e->ditems) hipFree(e->ditems);
+        e->ditems = NULL;
+        long long cap = n + n / 2 + 1024;
+        HIPCHK(hipMalloc((void **)&e->ditems, sizeof(ssa_item) * cap));
+        e->ditems_cap = cap;
+        from = 0;
+    }
+    if (n > from) {
+        DEVCHK(pin_reserve(e, sizeof(ssa_item) * (n - from)));
+        memcpy(e->hpin, e->hitems + from, sizeof(ssa_item) * (n - from));
+        HIPCHK(hipMemcpyAsync(e->ditems + from, e->hpin, sizeof(ssa_item) * (n - from), hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipStreamSynchronize(e->st));
+    }
+    return 0;
+}
+
+static const ssa_item *it_ptr(const ssa_engine *e, span sp) { return e->ditems + sp.off; }
+
+/* matvec work items: (s, pair, o0, o1).  When a single input chunk feeds
+ * several output chunks and the engine has plenty of pairs, one item
+ * covers all output chunks so the input transform is reused. */
+static int build_matvec(ssa_engine *e, int dir, int lgn, span *sp)
+{
+    long long pairs_total = 0;
+    for (int s = 0; s < e->count; s++) pairs_total += (e->hd[s].b + 1) / 2;
+    sp->off = e->hitems_n;
+    for (int s = 0; s < e->count; s++) {
+        const ssa_desc *d = &e->hd[s];
+        if (d->lgn != lgn || d->b <= 0) continue;
+        int nLc = d->M * d->AL;
+        int n_in = dir == 0 ? d->SK : nLc, n_out = dir == 0 ? nLc : d->SK;
+        int reuse = (n_in == 1) && (pairs_total * n_out >= 2048);
+        for (int p = 0; p < (d->b + 1) / 2; p++) {
+            if (reuse) { if (items_push(e, s, p, 0, n_out)) return -1; }
+            else for (int o = 0; o < n_out; o++) if (items_push(e, s, p, o, o + 1)) return -1;
+        }
+    }
+    sp->n = (int)(e->hitems_n - sp->off);
+    return 0;
+}
+
+static int rebuild_lists(ssa_engine *e)
+{
+    e->hitems_n = 0;
+    for (int lg = 0; lg < NLG; lg++) {
+        e->spec[lg].off = e->hitems_n;
+        for (int s = 0; s < e->count; s++) {
+            const ssa_desc *d = &e->hd[s];
+            if (d->lgn != lg) continue;
+            for (int o = 0; o < d->M * d->AL; o++)
+                for (int sk = 0; sk < d->SK; sk++) if (items_push(e, s, o, sk, 0)) return -1;
+        }
+        e->spec[lg].n = (int)(e->hitems_n - e->spec[lg].off);
+    }
+    for (int lg = 0; lg < NLG; lg++) {
+        if (build_matvec(e, 0, lg, &e->mvf[lg])) return -1;
+        if (build_matvec(e, 1, lg, &e->mva[lg])) return -1;
+    }
+    e->tY.off = e->hitems_n;
+    for (int s = 0; s < e->count; s++)
+        for (int t = 0; t < e->hd[s].tilesY; t++) if (items_push(e, s, t, 0, 0)) return -1;
+    e->tY.n = (int)(e->hitems_n - e->tY.off);
+    e->tZ.off = e->hitems_n;
+    for (int s = 0; s < e->count; s++)
+        for (int t = 0; t < e->hd[s].tilesZ; t++) if (items_push(e, s, t, 0, 0)) return -1;
+    e->tZ.n = (int)(e->hitems_n - e->tZ.off);
+    e->ser.off = e->hitems_n;
+    for (int s = 0; s < e->count; s++) if (items_push(e, s, 0, 0, 0)) return -1;
+    e->ser.n = (int)(e->hitems_n - e->ser.off);
+    /* batched step: group gid = s covering components 0..k-1 */
+    for (int lg = 0; lg < NLG; lg++) {
+        e->rec_step[lg].off = e->hitems_n;
+        for (int s = 0; s < e->count; s++) {
+            const ssa_desc *d = &e->hd[s];
+            if (d->lgn != lg || d->k <= 0) continue;
+            for (int a = 0; a < d->AL; a++)
+                for (int sk = 0; sk < d->SK; sk++) if (items_push(e, s, 0, a, sk)) return -1;
+        }
+        e->rec_step[lg].n = (int)(e->hitems_n - e->rec_step[lg].off);
+    }
+    e->olap_step.off = e->hitems_n;
+    for (int s = 0; s < e->count; s++) {
+        const ssa_desc *d = &e->hd[s];
+        if (d->k <= 0) continue;
+        for (int t = 0; t < ceil_div(d->N, OLAP_TILE); t++) if (items_push(e, s, 0, t, 0)) return -1;
+    }
+    e->olap_step.n = (int)(e->hitems_n - e->olap_step.off);
+    e->dyn_base = (int)e->hitems_n;
+    DEVCHK(items_upload(e, 0));
+    e->lists_dirty = 0;
+    return 0;
+}
+
+static int sync_desc(ssa_engine *e)
+{
+    if (e->lists_dirty) DEVCHK(rebuild_lists(e));
+    if (e->desc_dirty) {
+        DEVCHK(pin_reserve(e, sizeof(ssa_desc) * e->count));
+        memcpy(e->hpin, e->hd, sizeof(ssa_desc) * e->count);
+        HIPCHK(hipMemcpyAsync(e->dd, e->hpin, sizeof(ssa_desc) * e->count, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipStreamSynchronize(e->st));
+        e->desc_dirty = 0;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* lifecycle                                                                 */
+/* ------------------------------------------------------------------------ */
+
+ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, const int *M, const int *bmax,
+                              const int *kmax, int q_hint, int *err)
+{
+    int dummy;
+    if (!err) err = &dummy;
+    *err = -1;
+    if (count < 1 || (prec != SSA_PREC_F64 && prec != SSA_PREC_F32)) return NULL;
+    ssa_engine *e = (ssa_engine *)calloc(1, sizeof(ssa_engine));
+    if (!e) return NULL;
+    e->prec = prec;
+    e->count = count;
+    e->esz = prec == SSA_PREC_F64 ? 8 : 4;
+    e->hd = (ssa_desc *)calloc(count, sizeof(ssa_desc));
+    e->bmax = (int *)calloc(count, sizeof(int));
+    e->kmax = (int *)calloc(count, sizeof(int));
+    if (!e->hd || !e->bmax || !e->kmax) goto fail;
+    if (hipStreamCreate(&e->st) != hipSuccess) goto fail;
+    if (!g_tw[prec]) {
+        if (ssa_dev_twiddles(prec, &g_tw[prec], e->st) != 0) goto fail;
+    }
+    e->tw = g_tw[prec];
+    long long off = 0;
+    for (int s = 0; s < count; s++) {
+        ssa_desc *d = &e->hd[s];
+        int m = M ? M[s] : 1;
+        if (N[s] < 4 || L[s] < 2 || L[s] > N[s] - 1 || m < 1) goto fail;
+        d->N = N[s]; d->L = L[s]; d->K = N[s] - L[s] + 1; d->M = m; d->ML = m * L[s];
+        e->bmax[s] = bmax ? bmax[s] : 0;
+        e->kmax[s] = kmax ? kmax[s] : 0;
+        if (e->bmax[s] > SSA_BMAX) goto fail;
+        ssa_plan pl;
+        int bb = e->bmax[s] > 0 ? e->bmax[s] : 60, kk = e->kmax[s] > 0 ? e->kmax[s] : 50;
+        if (ssa_plan_series(prec, d->N, d->L, m, bb, kk, q_hint, -1, &pl)) goto fail;
+        d->n = pl.n; d->lgn = pl.lgn; d->CL = pl.CL; d->CK = pl.CK; d->AL = pl.AL; d->SK = pl.SK;
+        d->tilesY = ceil_div(d->ML, ROWS_PER_WG);
+        d->tilesZ = ceil_div(d->K, ROWS_PER_WG);
+        d->seed = 42;
+        d->off_x = off; off = align64(off + (long long)m * d->N);
+        d->off_spec = off; off = align64(off + (long long)m * d->AL * d->SK * 2 * d->n);
+    }
+    e->a_elems = off;
+    if (hipMalloc(&e->arena_a, e->esz * (size_t)(off > 0 ? off : 64)) != hipSuccess) goto fail;
+    if (hipMalloc((void **)&e->dd, sizeof(ssa_desc) * count) != hipSuccess) goto fail;
+    e->lists_dirty = 1;
+    e->desc_dirty = 1;
+    *err = 0;
+    return e;
+fail:
+    ssa_engine_destroy(e);
+    return NULL;
+}
+
+void ssa_engine_destroy(ssa_engine *e)
+{
+    if (!e) return;
+    if (e->st) hipStreamSynchronize(e->st);
+    if (e->has_graph) hipGraphExecDestroy(e->gexec);
+    if (e->arena_a) hipFree(e->arena_a);
+    if (e->arena_b) hipFree(e->arena_b);
+    if (e->small) hipFree(e->small);
+    if (e->dd) hipFree(e->dd);
+    if (e->ditems) hipFree(e->ditems);
+    if (e->dgroups) hipFree(e->dgroups);
+    if (e->hpin) hipHostFree(e->hpin);
+    if (e->st) hipStreamDestroy(e->st);
+    free(e->hd); free(e->bmax); free(e->kmax); free(e->hitems);
+    free(e);
+}
+
+int ssa_engine_count(const ssa_engine *e) { return e->count; }
+int ssa_engine_precision(const ssa_engine *e) { return e->prec; }
+const ssa_desc *ssa_engine_desc(const ssa_engine *e, int s) { return &e->hd[s]; }
+void *ssa_engine_stream(ssa_engine *e) { return (void *)e->st; }
+
+long long ssa_engine_bytes(const ssa_engine *e)
+{
+    return (long long)e->esz * (e->a_elems + e->b_elems) + 8LL * e->small_elems;
+}
+
+int ssa_engine_sync(ssa_engine *e)
+{
+    HIPCHK(hipStreamSynchronize(e->st));
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* signals                                                                   */
+/* ------------------------------------------------------------------------ */
+
+int ssa_engine_upload_signal(ssa_engine *e, int s, const double *x)
+{
+    if (s < 0 || s >= e->count || !x) return -1;
+    const ssa_desc *d = &e->hd[s];
+    size_t n = (size_t)d->M * d->N;
+    DEVCHK(pin_reserve(e, n * e->esz));
+    if (e->prec == SSA_PREC_F64) memcpy(e->hpin, x, n * 8);
+    else for (size_t i = 0; i < n; i++) ((float *)e->hpin)[i] = (float)x[i];
+    HIPCHK(hipMemcpyAsync((char *)e->arena_a + e->esz * d->off_x, e->hpin, n * e->esz, hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    return 0;
+}
+
+int ssa_engine_upload_signals_f32(ssa_engine *e, const float *cat, const long long *offsets)
+{
+    for (int s = 0; s < e->count; s++) {
+        const ssa_desc *d = &e->hd[s];
+        size_t n = (size_t)d->M * d->N;
+        DEVCHK(pin_reserve(e, n * e->esz));
+        if (e->prec == SSA_PREC_F32) memcpy(e->hpin, cat + offsets[s], n * 4);
+        else for (size_t i = 0; i < n; i++) ((double *)e->hpin)[i] = cat[offsets[s] + i];
+        HIPCHK(hipMemcpyAsync((char *)e->arena_a + e->esz * d->off_x, e->hpin, n * e->esz, hipMemcpyHostToDevice,
+                              e->st));
+        HIPCHK(hipStreamSynchronize(e->st));
+    }
+    return 0;
+}
+
+static int launch_spectra(ssa_engine *e)
+{
+    for (int lg = 6; lg < NLG; lg++)
+        DEVCHK(ssa_dev_spectra(e->prec, lg, e->dd, it_ptr(e, e->spec[lg]), e->spec[lg].n, e->arena_a, e->tw, e->st));
+    return 0;
+}
+
+int ssa_engine_spectra(ssa_engine *e)
+{
+    DEVCHK(sync_desc(e));
+    DEVCHK(launch_spectra(e));
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* panels                                                                    */
+/* ------------------------------------------------------------------------ */
+
+int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
+{
+    if (max_groups < 1) max_groups = 1;
+    if (max_h < 0) max_h = 0;
+    if (e->prepared && max_groups <= e->max_groups && max_h <= e->max_h) return 0;
+    if (max_groups < e->max_groups) max_groups = e->max_groups;
+    if (max_h < e->max_h) max_h = e->max_h;
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (e->arena_b) { hipFree(e->arena_b); e->arena_b = NULL; }
+    if (e->small) { hipFree(e->small); e->small = NULL; }
+    long long off = 0, soff = 0;
+    for (int s = 0; s < e->count; s++) {
+        ssa_desc *d = &e->hd[s];
+        int bm = e->bmax[s], km = e->kmax[s];
+        if (bm < 2 || km < 1) return -1;
+        int gmax = max_groups > km ? max_groups : km;   /* W-correlation needs k pieces */
+        d->off_Y = off; off = align64(off + (long long)d->ML * bm);
+        d->off_Z = off; off = align64(off + (long long)d->K * bm);
+        d->off_U = off; off = align64(off + (long long)d->ML * km);
+        d->off_V = off; off = align64(off + (long long)d->K * km);
+        d->stage_cap = (long long)gmax * d->AL * d->SK * d->n;
+        d->off_stage = off; off = align64(off + d->stage_cap);
+        d->out_cap = (long long)gmax * d->N + max_h + 64;
+        d->off_out = off; off = align64(off + d->out_cap);
+        int tiles = d->tilesY > d->tilesZ ? d->tilesY : d->tilesZ;
+        int tout = ceil_div(d->N, ROWS_PER_WG);
+        if (tout > tiles) tiles = tout;
+        long long part = (long long)tiles * (bm * bm + bm);
+        long long need = 2LL * d->L + max_h + 256;
+        if (part < need) part = need;
+        d->off_part = soff; soff = align64(soff + part);
+        d->off_small = soff; soff = align64(soff + (long long)SSA_NSLOTS * SSA_BMAX * SSA_BMAX);
+    }
+    HIPCHK(hipMalloc(&e->arena_b, e->esz * (size_t)(off > 0 ? off : 64)));
+    HIPCHK(hipMalloc((void **)&e->small, 8 * (size_t)soff));
+    e->b_elems = off;
+    e->small_elems = soff;
+    /* rebase panel offsets on arena_a */
+    long long delta = ((char *)e->arena_b - (char *)e->arena_a);
+    if (delta % (long long)e->esz) return -1;
+    delta /= (long long)e->esz;
+    for (int s = 0; s < e->count; s++) {
+        ssa_desc *d = &e->hd[s];
+        d->off_Y += delta; d->off_Z += delta; d->off_U += delta; d->off_V += delta;
+        d->off_stage += delta; d->off_out += delta;
+        if (d->b == 0) { d->b = e->bmax[s]; d->k = e->kmax[s]; }
+    }
+    e->prepared = 1;
+    e->max_groups = max_groups;
+    e->max_h = max_h;
+    e->lists_dirty = 1;
+    e->desc_dirty = 1;
+    e->step_groups_ready = 0;
+    if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+    return 0;
+}
+
+int ssa_engine_set_caps(ssa_engine *e, int s, int bmax, int kmax)
+{
+    if (s < 0 || s >= e->count || bmax < 2 || bmax > SSA_BMAX || kmax < 1 || kmax > bmax) return -1;
+    if (e->bmax[s] != bmax || e->kmax[s] != kmax) {
+        e->bmax[s] = bmax;
+        e->kmax[s] = kmax;
+        e->hd[s].b = bmax;
+        e->hd[s].k = kmax;
+        e->prepared = 0;
+        e->max_groups = 0;
+        e->max_h = 0;
+        e->lists_dirty = 1;
+        e->desc_dirty = 1;
+    }
+    return 0;
+}
+
+int ssa_engine_read_spectrum(ssa_engine *e, int s, double *out, int ncomplex)
+{
+    if (s < 0 || s >= e->count || !out) return -1;
+    const ssa_desc *d = &e->hd[s];
+    if (ncomplex > d->n) ncomplex = d->n;
+    DEVCHK(pin_reserve(e, 2 * (size_t)ncomplex * e->esz));
+    HIPCHK(hipMemcpyAsync(e->hpin, (char *)e->arena_a + e->esz * d->off_spec, 2 * (size_t)ncomplex * e->esz,
+                          hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    /* stored as X / n; report the unscaled transform */
+    for (int i = 0; i < 2 * ncomplex; i++)
+        out[i] = (e->prec == SSA_PREC_F64 ? ((double *)e->hpin)[i] : ((float *)e->hpin)[i]) * d->n;
+    return 0;
+}
+
+int ssa_engine_set_bk(ssa_engine *e, int s, int b, int k)
+{
+    if (s < 0 || s >= e->count || b < 1 || k < 1 || b > e->bmax[s] || k > e->kmax[s] || k > b) return -1;
+    if (e->hd[s].b != b || e->hd[s].k != k) {
+        e->hd[s].b = b;
+        e->hd[s].k = k;
+        e->lists_dirty = 1;
+        e->desc_dirty = 1;
+        e->step_groups_ready = 0;
+        if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+    }
+    return 0;
+}
+
+int ssa_engine_set_rng(ssa_engine *e, int s, unsigned long long seed, unsigned long long call)
+{
+    if (s < 0 || s >= e->count) return -1;
+    e->hd[s].seed = seed;
+    e->hd[s].call = call;
+    e->desc_dirty = 1;
+    return 0;
+}
+
+int ssa_engine_inject_omega(ssa_engine *e, int s, const double *Omega)
+{
+    if (!e->prepared || s < 0 || s >= e->count || !Omega) return -1;
+    const ssa_desc *d = &e->hd[s];
+    size_t n = (size_t)d->K * d->b;
+    DEVCHK(pin_reserve(e, n * e->esz));
+    if (e->prec == SSA_PREC_F64) memcpy(e->hpin, Omega, n * 8);
+    else for (size_t i = 0; i < n; i++) ((float *)e->hpin)[i] = (float)Omega[i];
+    HIPCHK(hipMemcpyAsync((char *)e->arena_a + e->esz * d->off_Z, e->hpin, n * e->esz, hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* decomposition                                                             */
+/* ------------------------------------------------------------------------ */
+
+static int bmax_all(const ssa_engine *e)
+{
+    int m = 0;
+    for (int s = 0; s < e->count; s++) if (e->hd[s].b > m) m = e->hd[s].b;
+    return m;
+}
+
+static int matvec_all(ssa_engine *e, int dir, int gen, int in_panel, int out_panel)
+{
+    for (int lg = 6; lg < NLG; lg++) {
+        span sp = dir == 0 ? e->mvf[lg] : e->mva[lg];
+        DEVCHK(ssa_dev_matvec(e->prec, lg, dir, gen, in_panel, out_panel, e->dd, it_ptr(e, sp), sp.n, e->arena_a,
+                              e->tw, e->st));
+    }
+    return 0;
+}
+
+static int cholqr_factor(ssa_engine *e, int panel, int bm)
+{
+    span t = panel == SSA_PANEL_Y ? e->tY : e->tZ;
+    DEVCHK(ssa_dev_gram(e->prec, panel, 0, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
+    DEVCHK(ssa_dev_chol(panel, 1, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    DEVCHK(ssa_dev_gram(e->prec, panel, 1, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
+    DEVCHK(ssa_dev_chol(panel, 2, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    return 0;
+}
+
+static int orth(ssa_engine *e, int panel, int bm)
+{
+    span t = panel == SSA_PANEL_Y ? e->tY : e->tZ;
+    DEVCHK(cholqr_factor(e, panel, bm));
+    DEVCHK(ssa_dev_apply(e->prec, panel, panel, panel == SSA_PANEL_Y ? SSA_SLOT_RINVY : SSA_SLOT_RINVZ, 0, e->dd,
+                         it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
+    return 0;
+}
+
+static int launch_decompose(ssa_engine *e, int q, int injected)
+{
+    int bm = bmax_all(e);
+    DEVCHK(matvec_all(e, 0, !injected, SSA_PANEL_Z, SSA_PANEL_Y));
+    DEVCHK(orth(e, SSA_PANEL_Y, bm));
+    for (int it = 0; it < q; it++) {
+        DEVCHK(matvec_all(e, 1, 0, SSA_PANEL_Y, SSA_PANEL_Z));
+        DEVCHK(orth(e, SSA_PANEL_Z, bm));
+        DEVCHK(matvec_all(e, 0, 0, SSA_PANEL_Z, SSA_PANEL_Y));
+        DEVCHK(orth(e, SSA_PANEL_Y, bm));
+    }
+    DEVCHK(matvec_all(e, 1, 0, SSA_PANEL_Y, SSA_PANEL_Z));
+    DEVCHK(cholqr_factor(e, SSA_PANEL_Z, bm));
+    DEVCHK(ssa_dev_svd(e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Y, SSA_PANEL_U, SSA_SLOT_WK, 1, e->dd, it_ptr(e, e->tY), e->tY.n,
+                         e->arena_a, e->small, e->st));
+    DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Z, SSA_PANEL_V, SSA_SLOT_WKS, 1, e->dd, it_ptr(e, e->tZ), e->tZ.n,
+                         e->arena_a, e->small, e->st));
+    return 0;
+}
+
+int ssa_engine_decompose(ssa_engine *e, int q, int injected)
+{
+    if (!e->prepared || q < 0) return -1;
+    DEVCHK(sync_desc(e));
+    DEVCHK(launch_decompose(e, q, injected));
+    return 0;
+}
+
+int ssa_engine_results(ssa_engine *e, int s, double *sigma, double *lambda, double *totvar, double *U, double *V,
+                       double *sweeps)
+{
+    if (!e->prepared || s < 0 || s >= e->count) return -1;
+    const ssa_desc *d = &e->hd[s];
+    size_t misc_n = 4 * SSA_BMAX + 3;
+    size_t u_n = U ? (size_t)d->ML * d->k : 0, v_n = V ? (size_t)d->K * d->k : 0;
+    DEVCHK(pin_reserve(e, misc_n * 8 + (u_n + v_n) * e->esz));
+    double *misc = (double *)e->hpin;
+    char *pu = (char *)e->hpin + misc_n * 8;
+    char *pv = pu + u_n * e->esz;
+    HIPCHK(hipMemcpyAsync(misc, e->small + d->off_small + (long long)SSA_SLOT_MISC * SSA_BMAX * SSA_BMAX,
+                          misc_n * 8, hipMemcpyDeviceToHost, e->st));
+    if (u_n) HIPCHK(hipMemcpyAsync(pu, (char *)e->arena_a + e->esz * d->off_U, u_n * e->esz, hipMemcpyDeviceToHost, e->st));
+    if (v_n) HIPCHK(hipMemcpyAsync(pv, (char *)e->arena_a + e->esz * d->off_V, v_n * e->esz, hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (sigma) memcpy(sigma, misc + SSA_MISC_SIGMA, sizeof(double) * d->k);
+    if (lambda) memcpy(lambda, misc + SSA_MISC_LAMBDA, sizeof(double) * d->k);
+    if (totvar) *totvar = misc[SSA_MISC_TOTVAR];
+    if (sweeps) *sweeps = misc[SSA_MISC_SWEEPS];
+    if (u_n) {
+        if (e->prec == SSA_PREC_F64) memcpy(U, pu, u_n * 8);
+        else for (size_t i = 0; i < u_n; i++) U[i] = ((float *)pu)[i];
+    }
+    if (v_n) {
+        if (e->prec == SSA_PREC_F64) memcpy(V, pv, v_n * 8);
+        else for (size_t i = 0; i < v_n; i++) V[i] = ((float *)pv)[i];
+    }
+    return d->k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* reconstruction, W-correlation, forecast                                   */
+/* ------------------------------------------------------------------------ */
+
+static int groups_upload(ssa_engine *e, const int *gdesc4, int ng, const int *glist, int nl)
+{
+    long long need = 4LL * ng + nl + 16;
+    if (need > e->dgroups_cap) {
+        if (e->dgroups) hipFree(e->dgroups);
+        e->dgroups = NULL;
+        long long cap = need + 1024;
+        HIPCHK(hipMalloc((void **)&e->dgroups, sizeof(int) * cap));
+        e->dgroups_cap = cap;
+    }
+    DEVCHK(pin_reserve(e, sizeof(int) * need));
+    int *h = (int *)e->hpin;
+    memcpy(h, gdesc4, sizeof(int) * 4 * ng);
+    memcpy(h + 4 * ng, glist, sizeof(int) * nl);
+    HIPCHK(hipMemcpyAsync(e->dgroups, h, sizeof(int) * (4 * ng + nl), hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    return 0;
+}
+
+/* launch the reconstruction of ng groups of series s into its output slots */
+static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weight)
+{
+    const ssa_desc *d = &e->hd[s];
+    e->hitems_n = e->dyn_base;
+    span rec, ol;
+    rec.off = e->hitems_n;
+    for (int g = 0; g < ng; g++)
+        for (int a = 0; a < d->AL; a++)
+            for (int sk = 0; sk < d->SK; sk++) DEVCHK(items_push(e, s, g, a, sk));
+    rec.n = (int)(e->hitems_n - rec.off);
+    ol.off = e->hitems_n;
+    for (int g = 0; g < ng; g++)
+        for (int t = 0; t < ceil_div(d->N, OLAP_TILE); t++) DEVCHK(items_push(e, s, g, t, 0));
+    ol.n = (int)(e->hitems_n - ol.off);
+    DEVCHK(items_upload(e, e->dyn_base));
+    DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
+                         e->dgroups + 4 * ng, e->arena_a, e->small, e->tw, e->st));
+    DEVCHK(ssa_dev_olap(e->prec, weight, e->dd, it_ptr(e, ol), ol.n, e->dgroups, e->arena_a, e->st));
+    return 0;
+}
+
+static int fetch_out(ssa_engine *e, int s, long long count, double *out)
+{
+    const ssa_desc *d = &e->hd[s];
+    DEVCHK(pin_reserve(e, count * e->esz));
+    HIPCHK(hipMemcpyAsync(e->hpin, (char *)e->arena_a + e->esz * d->off_out, count * e->esz, hipMemcpyDeviceToHost,
+                          e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (e->prec == SSA_PREC_F64) memcpy(out, e->hpin, count * 8);
+    else for (long long i = 0; i < count; i++) out[i] = ((float *)e->hpin)[i];
+    return 0;
+}
+
+int ssa_engine_reconstruct(ssa_engine *e, int s, int ch, const int *glist, const int *gcount, int n_groups,
+                           int scale_sigma, int weight, double *out)
+{
+    if (!e->prepared || s < 0 || s >= e->count || n_groups < 1) return -1;
+    const ssa_desc *d = &e->hd[s];
+    if (ch < 0 || ch >= d->M) return -1;
+    int gmax = e->max_groups > d->k ? e->max_groups : e->kmax[s];
+    if (n_groups > gmax) return -1;
+    DEVCHK(sync_desc(e));
+    int *gd = (int *)malloc(sizeof(int) * 4 * n_groups);
+    if (!gd) return -1;
+    int start = 0;
+    for (int g = 0; g < n_groups; g++) {
+        gd[4 * g] = start; gd[4 * g + 1] = gcount[g]; gd[4 * g + 2] = ch; gd[4 * g + 3] = 0;
+        start += gcount[g];
+    }
+    int rc = groups_upload(e, gd, n_groups, glist, start);
+    free(gd);
+    DEVCHK(rc);
+    DEVCHK(launch_recon(e, s, n_groups, scale_sigma, weight));
+    if (out) DEVCHK(fetch_out(e, s, (long long)n_groups * d->N, out));
+    return 0;
+}
+
+int ssa_engine_wcorr(ssa_engine *e, int s, double *W)
+{
+    if (!e->prepared || s < 0 || s >= e->count || !W) return -1;
+    const ssa_desc *d = &e->hd[s];
+    int k = d->k;
+    int *gl = (int *)malloc(sizeof(int) * k), *gc = (int *)malloc(sizeof(int) * k);
+    if (!gl || !gc) { free(gl); free(gc); return -1; }
+    for (int i = 0; i < k; i++) { gl[i] = i; gc[i] = 1; }
+    int rc = ssa_engine_reconstruct(e, s, 0, gl, gc, k, 0, 1, NULL);
+    free(gl); free(gc);
+    DEVCHK(rc);
+    /* Gram tiles over the N x k weighted component panel */
+    e->hitems_n = e->dyn_base;
+    span t, one;
+    t.off = e->hitems_n;
+    for (int i = 0; i < ceil_div(d->N, ROWS_PER_WG); i++) DEVCHK(items_push(e, s, i, 0, 0));
+    t.n = (int)(e->hitems_n - t.off);
+    one.off = e->hitems_n;
+    DEVCHK(items_push(e, s, 0, 0, 0));
+    one.n = 1;
+    DEVCHK(items_upload(e, e->dyn_base));
+    DEVCHK(ssa_dev_wcorr(e->prec, e->dd, it_ptr(e, t), t.n, it_ptr(e, one), 1, e->arena_a, e->small, e->st));
+    DEVCHK(pin_reserve(e, sizeof(double) * k * k));
+    HIPCHK(hipMemcpyAsync(e->hpin, e->small + d->off_small + (long long)SSA_SLOT_W * SSA_BMAX * SSA_BMAX,
+                          sizeof(double) * k * k, hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    memcpy(W, e->hpin, sizeof(double) * k * k);
+    return 0;
+}
+
+int ssa_engine_forecast(ssa_engine *e, int s, const int *group, int n_group, int h, double *out, int full)
+{
+    if (!e->prepared || s < 0 || s >= e->count || n_group < 1 || h < 1 || h > e->max_h) return -1;
+    const ssa_desc *d = &e->hd[s];
+    if (n_group > SSA_BMAX) return -1;
+    for (int g = 0; g < n_group; g++) if (group[g] < 0 || group[g] >= d->k) return -1;  /* MKL/ssa_opt.h:2782-2787 */
+    int gc = n_group;
+    DEVCHK(ssa_engine_reconstruct(e, s, 0, group, &gc, 1, 1, 0, NULL));
+    e->hitems_n = e->dyn_base;
+    span one;
+    one.off = e->hitems_n;
+    DEVCHK(items_push(e, s, 0, 0, 0));
+    one.n = 1;
+    DEVCHK(items_upload(e, e->dyn_base));
+    DEVCHK(ssa_dev_forecast(e->prec, h, e->dd, it_ptr(e, one), 1, e->dgroups, e->dgroups + 4, e->arena_a, e->small,
+                            e->st));
+    /* status of the series: 2 = verticality >= 1 */
+    int status = 0;
+    HIPCHK(hipMemcpyAsync(&status, (char *)(e->dd + s) + offsetof(ssa_desc, status), sizeof(int),
+                          hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (status) {
+        int zero = 0;
+        HIPCHK(hipMemcpy((char *)(e->dd + s) + offsetof(ssa_desc, status), &zero, sizeof(int), hipMemcpyHostToDevice));
+        return -1;
+    }
+    double *tmp = (double *)malloc(sizeof(double) * ((size_t)d->N + h));
+    if (!tmp) return -1;
+    int rc = fetch_out(e, s, (long long)d->N + h, tmp);
+    if (rc == 0) {
+        if (full) memcpy(out, tmp, sizeof(double) * ((size_t)d->N + h));
+        else memcpy(out, tmp + d->N, sizeof(double) * h);
+    }
+    free(tmp);
+    return rc;
+}
+
+int ssa_engine_matvec(ssa_engine *e, int s, int dir, const double *in, double *out)
+{
+    if (!e->prepared || s < 0 || s >= e->count) return -1;
+    ssa_desc *d = &e->hd[s];
+    int b = d->b, k = d->k;
+    /* column 0 of Z (dir 0) or Y (dir 1) holds the input; column 0 of the other panel the output */
+    int lin = dir == 0 ? d->K : d->ML, lout = dir == 0 ? d->ML : d->K;
+    long long oin = dir == 0 ? d->off_Z : d->off_Y, oout = dir == 0 ? d->off_Y : d->off_Z;
+    DEVCHK(pin_reserve(e, (size_t)(lin > lout ? lin : lout) * e->esz));
+    if (e->prec == SSA_PREC_F64) memcpy(e->hpin, in, (size_t)lin * 8);
+    else for (int i = 0; i < lin; i++) ((float *)e->hpin)[i] = (float)in[i];
+    HIPCHK(hipMemcpyAsync((char *)e->arena_a + e->esz * oin, e->hpin, (size_t)lin * e->esz, hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    /* temporarily a block of width 1 */
+    d->b = 1; d->k = 1;
+    e->desc_dirty = 1;
+    e->lists_dirty = 1;
+    int rc = sync_desc(e);
+    if (rc == 0) {
+        e->hitems_n = e->dyn_base;
+        span sp;
+        sp.off = e->hitems_n;
+        int n_out = dir == 0 ? d->M * d->AL : d->SK;
+        for (int o = 0; o < n_out; o++) items_push(e, s, 0, o, o + 1);
+        sp.n = (int)(e->hitems_n - sp.off);
+        rc = items_upload(e, e->dyn_base);
+        if (rc == 0)
+            rc = ssa_dev_matvec(e->prec, d->lgn, dir, 0, dir == 0 ? SSA_PANEL_Z : SSA_PANEL_Y,
+                                dir == 0 ? SSA_PANEL_Y : SSA_PANEL_Z, e->dd, it_ptr(e, sp), sp.n, e->arena_a, e->tw,
+                                e->st);
+    }
+    if (rc == 0 && hipMemcpyAsync(e->hpin, (char *)e->arena_a + e->esz * oout, (size_t)lout * e->esz,
+                                  hipMemcpyDeviceToHost, e->st) != hipSuccess) rc = -1;
+    if (rc == 0 && hipStreamSynchronize(e->st) != hipSuccess) rc = -1;
+    if (rc == 0) {
+        if (e->prec == SSA_PREC_F64) memcpy(out, e->hpin, (size_t)lout * 8);
+        else for (int i = 0; i < lout; i++) out[i] = ((float *)e->hpin)[i];
+    }
+    d->b = b; d->k = k;
+    e->desc_dirty = 1;
+    e->lists_dirty = 1;
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* batched step (spectra + decomposition + reconstruction of all k comps)    */
+/* ------------------------------------------------------------------------ */
+
+static int prepare_step_groups(ssa_engine *e)
+{
+    if (e->step_groups_ready) return 0;
+    /* gid 0 of every series: components 0..k-1 (gdesc shared: glist = 0..BMAX-1) */
+    int gd[4] = {0, 0, 0, 0};
+    int gl[SSA_BMAX];
+    for (int i = 0; i < SSA_BMAX; i++) gl[i] = i;
+    /* per-series k differs: the kernels skip indices >= k, so count = BMAX covers all */
+    gd[1] = SSA_BMAX;
+    DEVCHK(groups_upload(e, gd, 1, gl, SSA_BMAX));
+    e->step_groups_ready = 1;
+    return 0;
+}
+
+static int launch_step(ssa_engine *e, int q)
+{
+    DEVCHK(launch_spectra(e));
+    DEVCHK(launch_decompose(e, q, 0));
+    for (int lg = 6; lg < NLG; lg++)
+        DEVCHK(ssa_dev_recon(e->prec, lg, 1, e->dd, it_ptr(e, e->rec_step[lg]), e->rec_step[lg].n, e->dgroups,
+                             e->dgroups + 4, e->arena_a, e->small, e->tw, e->st));
+    DEVCHK(ssa_dev_olap(e->prec, 0, e->dd, it_ptr(e, e->olap_step), e->olap_step.n, e->dgroups, e->arena_a, e->st));
+    return 0;
+}
+
+int ssa_engine_step(ssa_engine *e, int q, int use_graph)
+{
+    if (!e->prepared || q < 0) return -1;
+    DEVCHK(sync_desc(e));
+    DEVCHK(prepare_step_groups(e));
+    if (!use_graph) return launch_step(e, q);
+    if (!e->has_graph || e->graph_q != q) {
+        if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+        int rc = launch_step(e, q);
+        hipError_t ce = hipStreamEndCapture(e->st, &g);
+        if (rc != 0 || ce != hipSuccess) return -1;
+        HIPCHK(hipGraphInstantiate(&e->gexec, g, NULL, NULL, 0));
+        hipGraphDestroy(g);
+        e->has_graph = 1;
+        e->graph_q = q;
+    }
+    HIPCHK(hipGraphLaunch(e->gexec, e->st));
+    return 0;
+}
+
+int ssa_engine_step_outputs(ssa_engine *e, int s, double *out)
+{
+    if (!e->prepared || s < 0 || s >= e->count) return -1;
+    return fetch_out(e, s, e->hd[s].N, out);
+}

@@ -1,0 +1,92 @@
+/*
+ * ssa_engine.h -- host C engine that keeps a batch of SSA / MSSA series
+ * resident in HBM and drives the HIP kernels of ssa_kernels.hip.
+ *
+ * One engine = one HIP stream + one bulk arena (signals, segment spectra,
+ * panels, results; FP64 or FP32) + one FP64 arena (Gram partials and the
+ * b x b factors).  A single SSA_Opt context is an engine with one series;
+ * the batched and serving paths put many series in one engine.
+ */
+#ifndef SSA_ENGINE_H
+#define SSA_ENGINE_H
+
+#include "ssa_device.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ssa_engine ssa_engine;
+
+/* chunking chosen for a series: n, CL, CK (see ssa_device.h) */
+typedef struct { int n, lgn, CL, CK, AL, SK; double cost; } ssa_plan;
+
+/* planner: correlation length and chunking for (N, L, M) at precision prec,
+ * for block width b, k components and q power iterations.  When lgn_force
+ * is in [6, 13] only that length is considered. */
+int ssa_plan_series(int prec, int N, int L, int M, int b, int k, int q, int lgn_force, ssa_plan *out);
+
+/* create: count series with lengths N[i], windows L[i], channels M[i] (NULL -> 1) */
+ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, const int *M,
+                              const int *bmax, const int *kmax, int q_hint, int *err);
+void ssa_engine_destroy(ssa_engine *e);
+int ssa_engine_count(const ssa_engine *e);
+int ssa_engine_precision(const ssa_engine *e);
+const ssa_desc *ssa_engine_desc(const ssa_engine *e, int s);
+
+/* signals: x has M*N values (channel-major) */
+int ssa_engine_upload_signal(ssa_engine *e, int s, const double *x);
+/* all series at once from one float array, offsets[s] = first value of series s */
+int ssa_engine_upload_signals_f32(ssa_engine *e, const float *cat, const long long *offsets);
+int ssa_engine_spectra(ssa_engine *e);
+
+/* capacities (block width, components) of series s; takes effect at the next prepare */
+int ssa_engine_set_caps(ssa_engine *e, int s, int bmax, int kmax);
+/* first ncomplex values of the spectrum of segment (0, 0) of series s */
+int ssa_engine_read_spectrum(ssa_engine *e, int s, double *out, int ncomplex);
+
+/* prepare panels and staging: groups = max reconstruction groups per call,
+ * max_h = max forecast horizon; may be called again to enlarge */
+int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h);
+/* block width and components of the next decomposition of series s
+ * (b <= bmax, k <= kmax of create); seed and call number of its test matrix */
+int ssa_engine_set_bk(ssa_engine *e, int s, int b, int k);
+int ssa_engine_set_rng(ssa_engine *e, int s, unsigned long long seed, unsigned long long call);
+/* copy a K x b column-major test matrix into the Z panel; the next
+ * decomposition of the engine uses injected matrices for every series */
+int ssa_engine_inject_omega(ssa_engine *e, int s, const double *Omega);
+
+/* decomposition of every series (q power iterations) */
+int ssa_engine_decompose(ssa_engine *e, int q, int injected);
+/* results of series s (any pointer may be NULL); returns k */
+int ssa_engine_results(ssa_engine *e, int s, double *sigma, double *lambda, double *totvar,
+                       double *U, double *V, double *sweeps);
+
+/* reconstruction of n_groups groups of series s (channel ch for MSSA):
+ * glist holds the concatenated groups, gcount their sizes; out receives
+ * n_groups * N values (weight 0: diagonal averaging).  scale_sigma 1: sigma-weighted */
+int ssa_engine_reconstruct(ssa_engine *e, int s, int ch, const int *glist, const int *gcount, int n_groups,
+                           int scale_sigma, int weight, double *out);
+/* W-correlation matrix (k x k row-major) of series s */
+int ssa_engine_wcorr(ssa_engine *e, int s, double *W);
+/* reconstruction of group + LRF forecast of h steps.  out: N + h values
+ * (full) or the h forecasts; returns -1 when the verticality is >= 1 */
+int ssa_engine_forecast(ssa_engine *e, int s, const int *group, int n_group, int h, double *out, int full);
+
+/* single-vector Hankel products of series s (dir 0: y = H v, 1: z = H^T u) */
+int ssa_engine_matvec(ssa_engine *e, int s, int dir, const double *in, double *out);
+
+/* batched step of every series: spectra, decomposition, reconstruction of
+ * the group {0..k-1} into the series' output slot; captured into a HIP
+ * graph on first use when use_graph is set */
+int ssa_engine_step(ssa_engine *e, int q, int use_graph);
+int ssa_engine_step_outputs(ssa_engine *e, int s, double *out);   /* N values */
+int ssa_engine_sync(ssa_engine *e);
+void *ssa_engine_stream(ssa_engine *e);
+/* bytes of device memory held */
+long long ssa_engine_bytes(const ssa_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,227 @@
+// ssa_fft.hpp -- register-resident Stockham FFT for one workgroup (gfx950).
+//
+// A length-N complex transform is held as E = N/NT values per thread in the
+// "natural" layout: thread t holds positions t + e*NT, e < E.  Each pass is a
+// Stockham autosort step (Govindaraju et al., "High performance discrete
+// Fourier transforms on graphics processors", SC'08): butterfly j reads
+// positions j + r*N/R (which is exactly the natural layout), applies the
+// twiddles w^(r*(j mod Ns)), an in-register DFT of size R, and writes
+// (j/Ns)*Ns*R + (j mod Ns) + r*Ns.  Between passes the values are exchanged
+// through LDS (re and im planes, optionally one plane at a time so the
+// exchange needs half the LDS); the last pass lands in the natural layout
+// without an exchange, so the result stays in registers for the caller's
+// pointwise stage and the next transform.
+//
+// SIGN = -1 is the forward DFT sum_t z_t exp(-2 pi i k t/N), SIGN = +1 the
+// unscaled inverse.  Twiddles come from a table tw[m] = exp(+2 pi i m/2^16)
+// built on the host in long double.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ssa {
+
+template <typename T> struct C2 { T x, y; };
+
+template <typename T> __device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T> __device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by S*i
+template <typename T, int S> __device__ __forceinline__ C2<T> muli(C2<T> a)
+{
+    return S > 0 ? C2<T>{-a.y, a.x} : C2<T>{a.y, -a.x};
+}
+
+// exp(S * 2 pi i * m / 16) for m = 0..7, exact-to-rounding constants
+template <typename T, int S> __device__ __forceinline__ C2<T> w16(int m)
+{
+    const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, h = (T)0.70710678118654752440;
+    T c = 0, s = 0;
+    switch (m) {
+    case 0: c = 1; s = 0; break;
+    case 1: c = c1; s = s1; break;
+    case 2: c = h; s = h; break;
+    case 3: c = s1; s = c1; break;
+    case 4: c = 0; s = 1; break;
+    case 5: c = -s1; s = c1; break;
+    case 6: c = -h; s = h; break;
+    default: c = -c1; s = s1; break;
+    }
+    return {c, (T)S * s};
+}
+
+// in-register DFTs, natural order in and out
+template <typename T, int S> __device__ __forceinline__ void dft2(C2<T> *v)
+{
+    C2<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <typename T, int S> __device__ __forceinline__ void dft4(C2<T> &a0, C2<T> &a1, C2<T> &a2, C2<T> &a3)
+{
+    C2<T> s02 = cadd(a0, a2), d02 = csub(a0, a2), s13 = cadd(a1, a3), d13 = muli<T, S>(csub(a1, a3));
+    a0 = cadd(s02, s13);
+    a2 = csub(s02, s13);
+    a1 = cadd(d02, d13);
+    a3 = csub(d02, d13);
+}
+
+template <typename T, int S> __device__ __forceinline__ void dft8(C2<T> *v)
+{
+    C2<T> e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    C2<T> o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<T, S>(e0, e1, e2, e3);
+    dft4<T, S>(o0, o1, o2, o3);
+    const T h = (T)0.70710678118654752440;
+    C2<T> t1 = cmul(o1, C2<T>{h, (T)S * h});
+    C2<T> t2 = muli<T, S>(o2);
+    C2<T> t3 = cmul(o3, C2<T>{-h, (T)S * h});
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, t1); v[5] = csub(e1, t1);
+    v[2] = cadd(e2, t2); v[6] = csub(e2, t2);
+    v[3] = cadd(e3, t3); v[7] = csub(e3, t3);
+}
+
+template <typename T, int S> __device__ __forceinline__ void dft16(C2<T> *v)
+{
+    C2<T> e[8], o[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+    dft8<T, S>(e);
+    dft8<T, S>(o);
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        C2<T> t = (m == 0) ? o[0] : cmul(o[m], w16<T, S>(m));
+        v[m] = cadd(e[m], t);
+        v[m + 8] = csub(e[m], t);
+    }
+}
+
+template <typename T, int S, int R> __device__ __forceinline__ void dftR(C2<T> *v)
+{
+    if constexpr (R == 2) dft2<T, S>(v);
+    else if constexpr (R == 4) dft4<T, S>(v[0], v[1], v[2], v[3]);
+    else if constexpr (R == 8) dft8<T, S>(v);
+    else if constexpr (R == 16) dft16<T, S>(v);
+}
+
+// LDS index with one pad element every 16 (breaks the power-of-two strides
+// of the scatter writes across banks)
+__device__ __forceinline__ int lpad(int p) { return p + (p >> 4); }
+
+// E = 8 values per thread (radix-8 passes), so a transform plus an
+// accumulator plus the butterfly temporaries fit the 128-VGPR budget of
+// 4 waves per SIMD; 1024 threads cover N = 8192.
+template <typename T, int LOGN> struct FftGeom {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int E = N >= 16 ? 8 : N / 2;
+    static constexpr int NT = N / E;
+    static constexpr int LOGE = (E == 16) ? 4 : (E == 8) ? 3 : (E == 4) ? 2 : 1;
+    static constexpr int FIRST = (LOGN % LOGE) ? (LOGN % LOGE) : LOGE;   // radix log of pass 0
+    static constexpr int LDS = N + (N >> 4) + 1;                         // elements per plane
+    // exchange one plane at a time when both planes would keep a second
+    // workgroup off the CU (160 KB LDS)
+    static constexpr bool SPLIT = 2 * LDS * sizeof(T) > 80000;
+};
+
+template <typename T, int LOGN> using FftRegs = C2<T>[FftGeom<T, LOGN>::E];
+
+template <typename T, int LOGN, int SIGN, bool SPLIT, int NSLOG, int RLOG>
+__device__ __forceinline__ void fft_pass(FftRegs<T, LOGN> &c, T *lre, T *lim, const C2<T> *tw, int tid)
+{
+    using G = FftGeom<T, LOGN>;
+    constexpr int N = G::N, NT = G::NT, E = G::E;
+    constexpr int R = 1 << RLOG, NS = 1 << NSLOG, BPT = E / R;
+#pragma unroll
+    for (int m = 0; m < BPT; m++) {
+        const int j = tid + m * NT;
+        C2<T> v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = c[m + r * BPT];
+        if constexpr (NS > 1) {
+            // w^r for r < R from the table entries w^1, w^2, w^4, w^8 and at most
+            // two products (a few ulp), keeping few twiddles live at once
+            const int kk = j & (NS - 1);
+            const int step = kk << (SSA_NMAX_LOG_DEV - NSLOG - RLOG);
+            C2<T> wp[5];
+#pragma unroll
+            for (int b = 0; (1 << b) < R; b++) {
+                wp[b] = tw[(1 << b) * step];
+                if constexpr (SIGN < 0) wp[b].y = -wp[b].y;
+            }
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                C2<T> w = C2<T>{1, 0};
+                bool first = true;
+#pragma unroll
+                for (int b = 0; (1 << b) < R; b++)
+                    if (r & (1 << b)) {
+                        w = first ? wp[b] : cmul(w, wp[b]);
+                        first = false;
+                    }
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dftR<T, SIGN, R>(v);
+#pragma unroll
+        for (int r = 0; r < R; r++) c[m + r * BPT] = v[r];
+    }
+    if constexpr (NSLOG + RLOG < LOGN) {
+        // scatter to Stockham destinations, gather natural layout
+        constexpr int PLANES = SPLIT ? 2 : 1;
+#pragma unroll
+        for (int ph = 0; ph < PLANES; ph++) {
+#pragma unroll
+            for (int m = 0; m < BPT; m++) {
+                const int j = tid + m * NT;
+                const int base = ((j >> NSLOG) << (NSLOG + RLOG)) + (j & (NS - 1));
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int p = lpad(base + r * NS);
+                    if (SPLIT) {
+                        if (ph == 0) lre[p] = c[m + r * BPT].x;
+                        else lre[p] = c[m + r * BPT].y;
+                    } else {
+                        lre[p] = c[m + r * BPT].x;
+                        lim[p] = c[m + r * BPT].y;
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int p = lpad(tid + e * NT);
+                if (SPLIT) {
+                    if (ph == 0) c[e].x = lre[p];
+                    else c[e].y = lre[p];
+                } else {
+                    c[e].x = lre[p];
+                    c[e].y = lim[p];
+                }
+            }
+            __syncthreads();
+        }
+        constexpr int NEXT = G::LOGE;
+        fft_pass<T, LOGN, SIGN, SPLIT, NSLOG + RLOG, NEXT>(c, lre, lim, tw, tid);
+    }
+}
+
+// Full transform of the natural-layout registers c (in place).
+template <typename T, int LOGN, int SIGN, bool SPLIT>
+__device__ __forceinline__ void fft(FftRegs<T, LOGN> &c, T *lre, T *lim, const C2<T> *tw)
+{
+    // The twiddles and LDS addresses depend only on the thread index, so the
+    // compiler would hoist those of every pass out of the callers' loops and
+    // keep them all live (register spills).  Opaque copies of the thread index
+    // and the table pointer make each transform recompute its own.
+    int tid = threadIdx.x;
+    const C2<T> *t = tw;
+    asm volatile("" : "+v"(tid));
+    asm volatile("" : "+s"(t));
+    fft_pass<T, LOGN, SIGN, SPLIT, 0, FftGeom<T, LOGN>::FIRST>(c, lre, lim, t, tid);
+}
+
+}  // namespace ssa

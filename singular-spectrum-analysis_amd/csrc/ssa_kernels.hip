@@ -1,0 +1,969 @@
+// ssa_kernels.hip -- HIP kernels of the SSA decomposition hot path (gfx950).
+//
+// Every kernel works on a batch of series described by ssa_desc records and
+// a work list of ssa_item; a single SSA_Opt context is a batch of one.
+//
+//   k_spectra   cached spectra of the signal segments (init / update_signal,
+//               MKL/ssa_opt.h:1255-1257, :1545-1558)
+//   k_matvec    fused block Hankel matvec H*X / H^T*X: two columns per complex
+//               transform, spectrum multiply, inverse, extract
+//               (MKL/ssa_opt.h:910-1004); optionally draws the Gaussian test
+//               matrix on the fly (MKL/ssa_opt.h:1719)
+//   k_gram      partial Gram P^T P (optionally of P*M) over row tiles: the
+//               CholeskyQR2 replacement of dgeqrf/dorgqr (MKL/ssa_opt.h:1723-1725)
+//   k_chol      FP64 Cholesky of the summed Gram, triangular inverse, products
+//   k_apply     panel * small matrix (Q = Y R^-1, U = Q W, V = B W S^-1;
+//               MKL/ssa_opt.h:1736-1738)
+//   k_svd       FP64 one-sided Jacobi SVD of the b x b R factor of B = H^T Q,
+//               sign convention sum(U[:,i]) >= 0 (MKL/ssa_opt.h:1731, :1749-1759)
+//   k_recon     frequency-domain diagonal averaging (MKL/ssa_opt.h:2325-2400)
+//   k_olap      overlap-add of the chunked convolutions and 1/count weights
+//   k_wnorm     W-correlation normalisation (MKL/ssa_opt.h:2407-2477)
+//   k_forecast  LRF coefficients and recurrence (MKL/ssa_opt.h:2766-2885)
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+
+#define SSA_NMAX_LOG_DEV 16
+#include "ssa_device.h"
+#include "ssa_fft.hpp"
+
+using namespace ssa;
+
+namespace {
+
+constexpr int GRAM_ROWS_PER_WG = 512;   // rows of a panel handled by one Gram / apply workgroup
+constexpr int SUB = SSA_GRAM_TR;        // LDS sub-tile rows
+constexpr int LDP = SSA_BMAX + 1;       // padded LDS row length of small matrices
+
+__device__ __forceinline__ double *slot(double *small, const ssa_desc &d, int s)
+{
+    return small + d.off_small + (long long)s * SSA_BMAX * SSA_BMAX;
+}
+
+template <typename T> struct Panel {
+    T *p;
+    int rows, ld, cols;
+};
+
+template <typename T> __device__ __forceinline__ Panel<T> panel_of(T *bulk, const ssa_desc &d, int which)
+{
+    switch (which) {
+    case SSA_PANEL_Y: return {bulk + d.off_Y, d.ML, d.ML, d.b};
+    case SSA_PANEL_Z: return {bulk + d.off_Z, d.K, d.K, d.b};
+    case SSA_PANEL_U: return {bulk + d.off_U, d.ML, d.ML, d.k};
+    case SSA_PANEL_V: return {bulk + d.off_V, d.K, d.K, d.k};
+    default: return {bulk + d.off_out, d.N, d.N, d.k};   // weighted component panel (W-corr)
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 + Box-Muller: identical definition to the oracle's
+// orc_philox_gaussian (row r, column pair j -> counter (r, j, call_lo, call_hi))
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ __forceinline__ void gauss_pair(uint64_t seed, uint64_t call, int row, int pair, double &g0, double &g1)
+{
+    uint32_t c[4] = {(uint32_t)row, (uint32_t)pair, (uint32_t)call, (uint32_t)(call >> 32)};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double u1 = (double)(((uint64_t)(c[0] >> 5) << 26) | (c[1] >> 6)) * 0x1p-53;
+    double u2 = (double)(((uint64_t)(c[2] >> 5) << 26) | (c[3] >> 6)) * 0x1p-53;
+    double rad = sqrt(-2.0 * log(1.0 - u1));
+    double ang = 6.283185307179586476925286766559 * u2;
+    g0 = rad * cos(ang);
+    g1 = rad * sin(ang);
+}
+
+template <typename T> __device__ __forceinline__ void block_sum2(double &a, double &b, double *red)
+{
+    // 256-thread sum of two values; result broadcast to every thread
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b += __shfl_xor(b, o);
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) { red[2 * w] = a; red[2 * w + 1] = b; }
+    __syncthreads();
+    a = 0; b = 0;
+    const int nw = blockDim.x >> 6;
+    for (int i = 0; i < nw; i++) { a += red[2 * i]; b += red[2 * i + 1]; }
+}
+
+// ---------------------------------------------------------------------------
+// spectra of the signal segments: X[o][sk] = FFT(x_m[a CL + sk CK : +CL+CK-1]) / n
+// ---------------------------------------------------------------------------
+template <typename T, int LOGN>
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_spectra(const ssa_desc *__restrict__ D,
+                                                                const ssa_item *__restrict__ items,
+                                                                T *__restrict__ bulk, const C2<T> *__restrict__ tw)
+{
+    using G = FftGeom<T, LOGN>;
+    constexpr bool SPLIT = G::SPLIT;
+    __shared__ T lre[G::LDS];
+    __shared__ T lim[SPLIT ? 1 : G::LDS];
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int o = it.x, sk = it.y;
+    const int m = o / d.AL, a = o - m * d.AL;
+    const int start = a * d.CL + sk * d.CK;
+    int len = d.CL + d.CK - 1;
+    if (start + len > d.N) len = d.N - start;
+    const T *x = bulk + d.off_x + (long long)m * d.N + start;
+    C2<T> c[G::E];
+#pragma unroll
+    for (int e = 0; e < G::E; e++) {
+        const int p = threadIdx.x + e * G::NT;
+        c[e].x = (p < len) ? x[p] : (T)0;
+        c[e].y = 0;
+    }
+    fft<T, LOGN, -1, SPLIT>(c, lre, lim, tw);
+    const T sc = (T)1 / (T)G::N;
+    C2<T> *X = reinterpret_cast<C2<T> *>(bulk + d.off_spec) + ((long long)o * d.SK + sk) * G::N;
+#pragma unroll
+    for (int e = 0; e < G::E; e++) {
+        const int p = threadIdx.x + e * G::NT;
+        X[p] = C2<T>{c[e].x * sc, c[e].y * sc};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused block matvec.  Item (s, pair, o0, o1): columns 2*pair, 2*pair+1 and
+// output chunks [o0, o1).  dir 0: out rows are the stacked L dimension
+// (chunk o = m*AL + a), input chunks run over K; dir 1: the reverse.
+// out[o][i] = sum_p sum_j seg_(o,p)[i + j] in_p[j]  via  T+( X_(o,p) . T+(in_p) ).
+// ---------------------------------------------------------------------------
+template <typename T, int LOGN, int GEN>
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_matvec(const ssa_desc *__restrict__ D,
+                                                                     const ssa_item *__restrict__ items, T *bulk,
+                                                                     const C2<T> *__restrict__ tw, int dir,
+                                                                     int in_panel, int out_panel)
+{
+    using G = FftGeom<T, LOGN>;
+    constexpr int E = G::E, NT = G::NT;
+    constexpr bool SPLIT = G::SPLIT;
+    __shared__ T lre[G::LDS];
+    __shared__ T lim[SPLIT ? 1 : G::LDS];
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int c0 = 2 * it.x, c1 = c0 + 1;
+    const bool two = c1 < d.b;
+    const Panel<T> pin = panel_of<T>(bulk, d, in_panel);
+    const Panel<T> pout = panel_of<T>(bulk, d, out_panel);
+    const int n_in = dir == 0 ? d.SK : d.M * d.AL;
+    const C2<T> *X = reinterpret_cast<const C2<T> *>(bulk + d.off_spec);
+
+    C2<T> z[E], acc[E];
+    for (int o = it.y; o < it.z; o++) {
+#pragma unroll
+        for (int e = 0; e < E; e++) acc[e] = C2<T>{0, 0};
+        for (int p = 0; p < n_in; p++) {
+            if (n_in > 1 || o == it.y) {
+                // input chunk p: rows [row0, row0 + len) of the K or stacked L dimension
+                int row0, len;
+                if (dir == 0) {
+                    row0 = p * d.CK;
+                    len = min(d.CK, d.K - row0);
+                } else {
+                    const int m = p / d.AL, a = p - m * d.AL;
+                    row0 = m * d.L + a * d.CL;
+                    len = min(d.CL, d.L - a * d.CL);
+                }
+                if constexpr (GEN) {
+#pragma unroll
+                    for (int e = 0; e < E; e++) {
+                        const int q = threadIdx.x + e * NT;
+                        double g0 = 0, g1 = 0;
+                        if (q < len) gauss_pair(d.seed, d.call, row0 + q, it.x, g0, g1);
+                        z[e].x = (T)g0;
+                        z[e].y = two ? (T)g1 : (T)0;
+                    }
+                } else {
+                    const T *a0 = pin.p + (long long)c0 * pin.ld + row0;
+                    const T *a1 = pin.p + (long long)(two ? c1 : c0) * pin.ld + row0;
+#pragma unroll
+                    for (int e = 0; e < E; e++) {
+                        const int q = threadIdx.x + e * NT;
+                        z[e].x = (q < len) ? a0[q] : (T)0;
+                        z[e].y = (two && q < len) ? a1[q] : (T)0;
+                    }
+                }
+                fft<T, LOGN, 1, SPLIT>(z, lre, lim, tw);
+            }
+            // spectra are indexed [L chunk][K chunk]
+            const int lo = dir == 0 ? o : p, ko = dir == 0 ? p : o;
+            const C2<T> *Xo = X + ((long long)lo * d.SK + ko) * G::N;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const C2<T> xv = Xo[threadIdx.x + e * NT];
+                acc[e].x += xv.x * z[e].x - xv.y * z[e].y;
+                acc[e].y += xv.x * z[e].y + xv.y * z[e].x;
+            }
+        }
+        fft<T, LOGN, 1, SPLIT>(acc, lre, lim, tw);
+        int row0, len;
+        if (dir == 0) {
+            const int m = o / d.AL, a = o - m * d.AL;
+            row0 = m * d.L + a * d.CL;
+            len = min(d.CL, d.L - a * d.CL);
+        } else {
+            row0 = o * d.CK;
+            len = min(d.CK, d.K - row0);
+        }
+        T *b0 = pout.p + (long long)c0 * pout.ld + row0;
+        T *b1 = pout.p + (long long)c1 * pout.ld + row0;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int q = threadIdx.x + e * NT;
+            if (q < len) {
+                b0[q] = acc[e].x;
+                if (two) b1[q] = acc[e].y;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// partial Gram: item (s, tile) covers rows [tile*512, +512) of the panel.
+// G_part = T^T T with T = P (use_m = 0) or T = P * M (M = R1^-1 slot), FP64
+// accumulation; use_m = 0 also emits the column sums of P.
+// ---------------------------------------------------------------------------
+template <typename T, int BB>
+__global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items,
+                                              const T *__restrict__ bulk, double *small, int which, int use_m)
+{
+    extern __shared__ double sm[];
+    double *A = sm;                    // SUB x LDP
+    double *Tm = sm + SUB * LDP;       // SUB x LDP (transformed rows)
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const Panel<T> P = panel_of<T>(const_cast<T *>(bulk), d, which);
+    const int b = P.cols;
+    const int r_begin = it.x * GRAM_ROWS_PER_WG;
+    const int r_end = min(P.rows, r_begin + GRAM_ROWS_PER_WG);
+    const double *M = slot(small, d, SSA_SLOT_R1INV);
+    const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;   // 16 x 16 grid of BB x BB blocks
+    double acc[BB][BB];
+#pragma unroll
+    for (int i = 0; i < BB; i++)
+#pragma unroll
+        for (int j = 0; j < BB; j++) acc[i][j] = 0;
+    double csum = 0;
+    for (int r0 = r_begin; r0 < r_end; r0 += SUB) {
+        const int nr = min(SUB, r_end - r0);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < SUB * b; idx += 256) {
+            const int c = idx / SUB, r = idx - c * SUB;
+            A[r * LDP + c] = (r < nr) ? (double)P.p[(long long)c * P.ld + r0 + r] : 0.0;
+        }
+        __syncthreads();
+        const double *S = A;
+        if (use_m) {
+            // Tm[r][j] = sum_{i<=j} A[r][i] M[i][j]   (M upper triangular, col-major b x b)
+            for (int idx = threadIdx.x; idx < SUB * b; idx += 256) {
+                const int j = idx / SUB, r = idx - j * SUB;
+                double s = 0;
+                const double *mj = M + (long long)j * SSA_BMAX;
+                for (int i = 0; i <= j; i++) s += A[r * LDP + i] * mj[i];
+                Tm[r * LDP + j] = s;
+            }
+            __syncthreads();
+            S = Tm;
+        } else if (threadIdx.x < b) {
+            for (int r = 0; r < nr; r++) csum += A[r * LDP + threadIdx.x];
+        }
+        for (int r = 0; r < SUB; r++) {
+            double ai[BB], aj[BB];
+#pragma unroll
+            for (int u = 0; u < BB; u++) {
+                const int ci = ti * BB + u, cj = tj * BB + u;
+                ai[u] = (ci < b) ? S[r * LDP + ci] : 0.0;
+                aj[u] = (cj < b) ? S[r * LDP + cj] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < BB; u++)
+#pragma unroll
+                for (int v = 0; v < BB; v++) acc[u][v] += ai[u] * aj[v];
+        }
+    }
+    double *out = small + d.off_part + (long long)it.x * (b * b + b);
+#pragma unroll
+    for (int u = 0; u < BB; u++)
+#pragma unroll
+        for (int v = 0; v < BB; v++) {
+            const int ci = ti * BB + u, cj = tj * BB + v;
+            if (ci < b && cj < b) out[ci * b + cj] = acc[u][v];
+        }
+    if (!use_m && threadIdx.x < b) out[b * b + threadIdx.x] = csum;
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky of the summed Gram (one workgroup per series).
+// pass 1: R1 = chol(G), R1^-1 -> slots; panel Y also stores its column sums.
+// pass 2: R2 = chol(G2); R = R2 R1, R^-1 = R1^-1 R2^-1 -> RY/RINVY or RZ/RINVZ;
+//         panel Y stores colsum(Q) = colsum(Y) R^-1.
+// A pivot below tol * (original diagonal) marks a numerically dependent
+// column: its row of R and column of R^-1 are zero (the column of Q is zero).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_chol(const ssa_desc *D, const ssa_item *items, double *small, int which,
+                                              int pass, double tol)
+{
+    extern __shared__ double sm[];
+    double *A = sm;                    // b x LDP, row-major (upper part used)
+    double *X = sm + SSA_BMAX * LDP;   // b x LDP
+    __shared__ double diag0[SSA_BMAX];
+    __shared__ int dead[SSA_BMAX];
+    __shared__ double piv;
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int b = d.b;
+    const int tiles = which == SSA_PANEL_Y ? d.tilesY : d.tilesZ;
+    const double *part = small + d.off_part;
+    const int stride = b * b + b;
+    // sum partials (fixed order -> deterministic)
+    for (int idx = threadIdx.x; idx < b * b; idx += 256) {
+        double s = 0;
+        for (int t = 0; t < tiles; t++) s += part[(long long)t * stride + idx];
+        const int i = idx / b, j = idx - i * b;
+        A[i * LDP + j] = s;
+    }
+    if (pass == 1 && which == SSA_PANEL_Y && threadIdx.x < b) {
+        double s = 0;
+        for (int t = 0; t < tiles; t++) s += part[(long long)t * stride + b * b + threadIdx.x];
+        slot(small, d, SSA_SLOT_MISC)[SSA_MISC_COLSUMY + threadIdx.x] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < b) diag0[threadIdx.x] = A[threadIdx.x * LDP + threadIdx.x];
+    __syncthreads();
+    for (int j = 0; j < b; j++) {
+        if (threadIdx.x == 0) {
+            const double p = A[j * LDP + j];
+            const int dd = !(p > tol * diag0[j]) || !(diag0[j] > 0.0);
+            dead[j] = dd;
+            piv = dd ? 0.0 : sqrt(p);
+        }
+        __syncthreads();
+        const double r = piv;
+        for (int l = j + threadIdx.x; l < b; l += 256)
+            A[j * LDP + l] = (r > 0.0) ? ((l == j) ? r : A[j * LDP + l] / r) : 0.0;
+        __syncthreads();
+        const int m = b - j - 1;
+        for (int idx = threadIdx.x; idx < m * m; idx += 256) {
+            const int i = j + 1 + idx / m, l = j + 1 + idx % m;
+            if (l >= i) A[i * LDP + l] -= A[j * LDP + i] * A[j * LDP + l];
+        }
+        __syncthreads();
+    }
+    // zero the strict lower triangle
+    for (int idx = threadIdx.x; idx < b * b; idx += 256) {
+        const int i = idx / b, j = idx - i * b;
+        if (j < i) A[i * LDP + j] = 0.0;
+    }
+    __syncthreads();
+    // X = R^-1 (upper), column by column
+    if (threadIdx.x < b) {
+        const int j = threadIdx.x;
+        for (int i = b - 1; i >= 0; i--) X[i * LDP + j] = 0.0;
+        if (!dead[j]) {
+            for (int i = j; i >= 0; i--) {
+                if (dead[i]) { X[i * LDP + j] = 0.0; continue; }
+                double s = (i == j) ? 1.0 : 0.0;
+                for (int l = i + 1; l <= j; l++) s -= A[i * LDP + l] * X[l * LDP + j];
+                X[i * LDP + j] = s / A[i * LDP + i];
+            }
+        }
+    }
+    __syncthreads();
+    double *misc = slot(small, d, SSA_SLOT_MISC);
+    if (pass == 1) {
+        double *R1 = slot(small, d, SSA_SLOT_R1), *R1i = slot(small, d, SSA_SLOT_R1INV);
+        for (int idx = threadIdx.x; idx < b * b; idx += 256) {
+            const int j = idx / b, i = idx - j * b;   // column-major store
+            R1[(long long)j * SSA_BMAX + i] = A[i * LDP + j];
+            R1i[(long long)j * SSA_BMAX + i] = X[i * LDP + j];
+        }
+        if (threadIdx.x == 0) {
+            int nd = 0;
+            for (int j = 0; j < b; j++) nd += dead[j];
+            misc[SSA_MISC_DEAD] = nd;
+        }
+        return;
+    }
+    const double *R1 = slot(small, d, SSA_SLOT_R1), *R1i = slot(small, d, SSA_SLOT_R1INV);
+    double *Ro = slot(small, d, which == SSA_PANEL_Y ? SSA_SLOT_RY : SSA_SLOT_RZ);
+    double *Rio = slot(small, d, which == SSA_PANEL_Y ? SSA_SLOT_RINVY : SSA_SLOT_RINVZ);
+    // R = R2 R1 (upper x upper), Rinv = R1inv X
+    for (int idx = threadIdx.x; idx < b * b; idx += 256) {
+        const int j = idx / b, i = idx - j * b;
+        double s = 0, t = 0;
+        if (i <= j) {
+            for (int l = i; l <= j; l++) {
+                s += A[i * LDP + l] * R1[(long long)j * SSA_BMAX + l];
+                t += R1i[(long long)l * SSA_BMAX + i] * X[l * LDP + j];
+            }
+        }
+        Ro[(long long)j * SSA_BMAX + i] = s;
+        Rio[(long long)j * SSA_BMAX + i] = t;
+    }
+    __syncthreads();
+    if (which == SSA_PANEL_Y && threadIdx.x < b) {
+        const int j = threadIdx.x;
+        double s = 0;
+        for (int i = 0; i <= j; i++) s += misc[SSA_MISC_COLSUMY + i] * Rio[(long long)j * SSA_BMAX + i];
+        misc[SSA_MISC_COLSUMQ + j] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// out (rows x ncols) = in (rows x b) * M (b x ncols, col-major, ld BMAX); in place allowed.
+// item (s, tile of 512 rows).  ncols_k: 0 -> ncols = b, 1 -> ncols = k.
+// scale_slot >= 0: M is scaled column-wise (unused; reserved).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply(const ssa_desc *D, const ssa_item *items, T *bulk, const double *small,
+                                               int src, int dst, int mslot, int ncols_k)
+{
+    extern __shared__ double sm[];
+    double *A = sm;   // SUB x LDP
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const Panel<T> Pi = panel_of<T>(bulk, d, src);
+    const Panel<T> Po = panel_of<T>(bulk, d, dst);
+    const int b = d.b, nc = ncols_k ? d.k : d.b;
+    const double *M = small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX;
+    const int r_begin = it.x * GRAM_ROWS_PER_WG;
+    const int r_end = min(Pi.rows, r_begin + GRAM_ROWS_PER_WG);
+    const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
+    for (int r0 = r_begin; r0 < r_end; r0 += SUB) {
+        const int nr = min(SUB, r_end - r0);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < SUB * b; idx += 256) {
+            const int c = idx / SUB, rr = idx - c * SUB;
+            A[rr * LDP + c] = (rr < nr) ? (double)Pi.p[(long long)c * Pi.ld + r0 + rr] : 0.0;
+        }
+        __syncthreads();
+        if (r < nr) {
+            for (int j = cg; j < nc; j += 4) {
+                const double *mj = M + (long long)j * SSA_BMAX;
+                double s = 0;
+                for (int i = 0; i < b; i++) s += A[r * LDP + i] * mj[i];
+                Po.p[(long long)j * Po.ld + r0 + r] = (T)s;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one-sided Jacobi SVD of R (b x b, FP64) with a round-robin parallel order:
+// R W = U S, W accumulated.  Sorted descending; signs chosen so that
+// colsum(Q) . W_i >= 0, i.e. sum(U[:,i]) >= 0 for U = Q W.
+// Outputs: WK = W[:, :k] (signed), WKS = WK S^-1, sigma, lambda, total variance.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *items, double *small)
+{
+    extern __shared__ double sm[];
+    double *A = sm;                        // column j at A + j*LDP (rows 0..b-1)
+    double *W = sm + (SSA_BMAX + 1) * LDP; // same layout
+    __shared__ int perm[SSA_BMAX + 1];
+    __shared__ double S[SSA_BMAX + 1];
+    __shared__ int rotated;
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int b = d.b, bp = (b + 1) & ~1;
+    const double *R = slot(small, d, SSA_SLOT_RZ);
+    for (int idx = threadIdx.x; idx < bp * b; idx += 256) {
+        const int j = idx / b, i = idx - j * b;
+        A[j * LDP + i] = (j < b) ? R[(long long)j * SSA_BMAX + i] : 0.0;
+        W[j * LDP + i] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nwave = blockDim.x >> 6;
+    int sweeps = 0;
+    for (; sweeps < 40; sweeps++) {
+        if (threadIdx.x == 0) rotated = 0;
+        __syncthreads();
+        for (int round = 0; round < bp - 1; round++) {
+            for (int pi = wave; pi < bp / 2; pi += nwave) {
+                int p, q;
+                if (pi == 0) { p = round; q = bp - 1; }
+                else {
+                    p = (round + pi) % (bp - 1);
+                    q = (round - pi + (bp - 1)) % (bp - 1);
+                }
+                if (p > q) { int t = p; p = q; q = t; }
+                double al = 0, be = 0, ga = 0;
+                for (int i = lane; i < b; i += 64) {
+                    const double x = A[p * LDP + i], y = A[q * LDP + i];
+                    al += x * x; be += y * y; ga += x * y;
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    al += __shfl_xor(al, o);
+                    be += __shfl_xor(be, o);
+                    ga += __shfl_xor(ga, o);
+                }
+                if (ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
+                    const double zeta = (be - al) / (2.0 * ga);
+                    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                    for (int i = lane; i < b; i += 64) {
+                        const double x = A[p * LDP + i], y = A[q * LDP + i];
+                        A[p * LDP + i] = c * x - s * y;
+                        A[q * LDP + i] = s * x + c * y;
+                        const double wx = W[p * LDP + i], wy = W[q * LDP + i];
+                        W[p * LDP + i] = c * wx - s * wy;
+                        W[q * LDP + i] = s * wx + c * wy;
+                    }
+                    if (lane == 0) rotated = 1;
+                }
+            }
+            __syncthreads();
+        }
+        const int any = rotated;
+        __syncthreads();
+        if (!any) break;
+    }
+    // singular values = column norms
+    for (int j = wave; j < b; j += nwave) {
+        double s = 0;
+        for (int i = lane; i < b; i += 64) s += A[j * LDP + i] * A[j * LDP + i];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (lane == 0) S[j] = sqrt(s);
+    }
+    __syncthreads();
+    if (threadIdx.x < b) {
+        const int i = threadIdx.x;
+        int rank = 0;
+        for (int j = 0; j < b; j++) rank += (S[j] > S[i]) || (S[j] == S[i] && j < i);
+        perm[rank] = i;
+    }
+    __syncthreads();
+    double *misc = slot(small, d, SSA_SLOT_MISC);
+    double *WK = slot(small, d, SSA_SLOT_WK), *WKS = slot(small, d, SSA_SLOT_WKS);
+    const double smax = S[perm[0]];
+    for (int j = wave; j < d.k; j += nwave) {
+        const int src = perm[j];
+        double dot = 0;
+        for (int i = lane; i < b; i += 64) dot += misc[SSA_MISC_COLSUMQ + i] * W[src * LDP + i];
+        for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
+        const double sg = dot < 0.0 ? -1.0 : 1.0;
+        const double sv = S[src];
+        const double inv = (sv > 1e-300 && sv > 1e-15 * smax) ? 1.0 / sv : 0.0;
+        for (int i = lane; i < b; i += 64) {
+            const double w = sg * W[src * LDP + i];
+            WK[(long long)j * SSA_BMAX + i] = w;
+            WKS[(long long)j * SSA_BMAX + i] = w * inv;
+        }
+        if (lane == 0) {
+            misc[SSA_MISC_SIGMA + j] = sv;
+            misc[SSA_MISC_LAMBDA + j] = sv * sv;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tv = 0;
+        for (int j = 0; j < d.k; j++) tv += S[perm[j]] * S[perm[j]];
+        misc[SSA_MISC_TOTVAR] = tv;
+        misc[SSA_MISC_SWEEPS] = sweeps;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction pieces.  Item (s, gid, a, sk): the convolution of the row
+// chunk a of channel m = gdesc[gid].z (U rows) with the column chunk sk
+// (V rows), summed over the group's components:
+//   piece = sum_g (s_g u_g,a) (*) v_g,sk = Im( T+( sum_g T-(z_g)^2 ) ) / (2n),
+//   z_g = sqrt(s_g) (u_g,a + i v_g,sk)
+// (the imaginary part of the self-convolution of a + ib is 2 a (*) b).
+// ---------------------------------------------------------------------------
+template <typename T, int LOGN>
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_recon(const ssa_desc *__restrict__ D,
+                                                              const ssa_item *__restrict__ items, T *bulk,
+                                                              const double *__restrict__ small,
+                                                              const int4 *__restrict__ gdesc,
+                                                              const int *__restrict__ glist,
+                                                              const C2<T> *__restrict__ tw, int scale_sigma)
+{
+    using G = FftGeom<T, LOGN>;
+    constexpr int E = G::E, NT = G::NT;
+    constexpr bool SPLIT = G::SPLIT;
+    __shared__ T lre[G::LDS];
+    __shared__ T lim[SPLIT ? 1 : G::LDS];
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int4 gd = gdesc[it.x];
+    const int a = it.y, sk = it.z, m = gd.z;
+    const int urow0 = m * d.L + a * d.CL, ulen = min(d.CL, d.L - a * d.CL);
+    const int vrow0 = sk * d.CK, vlen = min(d.CK, d.K - sk * d.CK);
+    const double *sig = small + d.off_small + (long long)SSA_SLOT_MISC * SSA_BMAX * SSA_BMAX + SSA_MISC_SIGMA;
+    C2<T> acc[E], z[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) acc[e] = C2<T>{0, 0};
+    for (int g = 0; g < gd.y; g++) {
+        const int idx = glist[gd.x + g];
+        if (idx < 0 || idx >= d.k) continue;
+        const T sc = (T)(scale_sigma ? sqrt(fabs(sig[idx])) : 1.0);
+        const T *u = bulk + d.off_U + (long long)idx * d.ML + urow0;
+        const T *v = bulk + d.off_V + (long long)idx * d.K + vrow0;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int q = threadIdx.x + e * NT;
+            z[e].x = (q < ulen) ? sc * u[q] : (T)0;
+            z[e].y = (q < vlen) ? sc * v[q] : (T)0;
+        }
+        fft<T, LOGN, -1, SPLIT>(z, lre, lim, tw);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            acc[e].x += z[e].x * z[e].x - z[e].y * z[e].y;
+            acc[e].y += (T)2 * z[e].x * z[e].y;
+        }
+    }
+    fft<T, LOGN, 1, SPLIT>(acc, lre, lim, tw);
+    const int plen = ulen + vlen - 1;
+    T *st = bulk + d.off_stage + (((long long)it.x * d.AL + a) * d.SK + sk) * G::N;
+    const T sc = (T)0.5 / (T)G::N;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int q = threadIdx.x + e * NT;
+        if (q < plen) st[q] = acc[e].y * sc;
+    }
+}
+
+// overlap-add of the pieces of group gid and diagonal-averaging weights.
+// item (s, gid, t tile of 1024); weight 0: 1/c_t, 1: sqrt(1/c_t)
+template <typename T>
+__global__ __launch_bounds__(256) void k_olap(const ssa_desc *D, const ssa_item *items, const int4 *gdesc, T *bulk,
+                                              int weight)
+{
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int n = d.n;
+    for (int t = it.y * 1024 + threadIdx.x; t < min(d.N, (it.y + 1) * 1024); t += 256) {
+        double s = 0;
+        for (int a = 0; a < d.AL; a++) {
+            const int ulen = min(d.CL, d.L - a * d.CL);
+            for (int sk = 0; sk < d.SK; sk++) {
+                const int off = a * d.CL + sk * d.CK;
+                const int vlen = min(d.CK, d.K - sk * d.CK);
+                const int q = t - off;
+                if (q >= 0 && q < ulen + vlen - 1)
+                    s += (double)bulk[d.off_stage + (((long long)it.x * d.AL + a) * d.SK + sk) * n + q];
+            }
+        }
+        const int c = min(min(t + 1, d.L), min(d.K, d.N - t));
+        const double w = 1.0 / (double)c;
+        bulk[d.off_out + (long long)it.x * d.N + t] = (T)(s * (weight ? sqrt(w) : w));
+    }
+}
+
+// W-correlation normalisation: W_ij = G_ij / sqrt(G_ii G_jj), rows whose
+// sigma * ||h||_w <= 1e-12 are zero (MKL/ssa_opt.h:2466-2471); row-major k x k.
+__global__ __launch_bounds__(256) void k_wnorm(const ssa_desc *D, const ssa_item *items, double *small)
+{
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int k = d.k, tiles = (d.N + GRAM_ROWS_PER_WG - 1) / GRAM_ROWS_PER_WG;
+    const double *part = small + d.off_part;
+    const double *sig = slot(small, d, SSA_SLOT_MISC) + SSA_MISC_SIGMA;
+    double *W = slot(small, d, SSA_SLOT_W);
+    __shared__ double dg[SSA_BMAX];
+    const int stride = k * k + k;
+    if (threadIdx.x < k) {
+        double s = 0;
+        for (int t = 0; t < tiles; t++) s += part[(long long)t * stride + threadIdx.x * k + threadIdx.x];
+        dg[threadIdx.x] = s;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < k * k; idx += 256) {
+        const int i = idx / k, j = idx - i * k;
+        double s = 0;
+        for (int t = 0; t < tiles; t++) s += part[(long long)t * stride + idx];
+        const double ni = sqrt(dg[i]), nj = sqrt(dg[j]);
+        const bool zi = !(fabs(sig[i]) * ni > 1e-12), zj = !(fabs(sig[j]) * nj > 1e-12);
+        W[idx] = (zi || zj) ? 0.0 : s / (ni * nj);
+    }
+}
+
+// LRF (verticality, coefficients) and the h-step recurrence on the group's
+// reconstruction (at off_out, gid 0); forecasts written after it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_forecast(ssa_desc *D, const ssa_item *items, const int4 *gdesc,
+                                                  const int *glist, T *bulk, double *small, int h)
+{
+    __shared__ double red[32];
+    __shared__ double pis[SSA_BMAX];
+    __shared__ int ids[SSA_BMAX];
+    const ssa_item it = items[blockIdx.x];
+    ssa_desc *dp = D + it.s;
+    const ssa_desc d = *dp;
+    const int4 gd = gdesc[0];
+    const int L = d.L, n = gd.y;
+    const T *U = bulk + d.off_U;
+    if (threadIdx.x == 0) {
+        for (int g = 0; g < n && g < SSA_BMAX; g++) ids[g] = glist[gd.x + g];
+    }
+    __syncthreads();
+    if (threadIdx.x < n) pis[threadIdx.x] = (double)U[(long long)ids[threadIdx.x] * d.ML + L - 1];
+    __syncthreads();
+    double nu = 0;
+    for (int g = 0; g < n; g++) nu += pis[g] * pis[g];
+    if (!(nu < 1.0 - 1e-10)) {
+        if (threadIdx.x == 0) dp->status = 2;
+        return;
+    }
+    const double sc = 1.0 / (1.0 - nu);
+    // R and the rolling window live in the staging area (FP64 stored as T pairs is avoided:
+    // use the small arena's partial region, sized for >= L + h doubles)
+    double *R = small + d.off_part;
+    double *buf = R + L;
+    const T *rec = bulk + d.off_out;
+    for (int j = threadIdx.x; j < L - 1; j += 256) {
+        double s = 0;
+        for (int g = 0; g < n; g++) s += pis[g] * (double)U[(long long)ids[g] * d.ML + j];
+        R[j] = s * sc;
+        buf[j] = (double)rec[d.N - (L - 1) + j];
+    }
+    __syncthreads();
+    T *out = bulk + d.off_out + d.N;
+    for (int t = 0; t < h; t++) {
+        double s = 0, dummy = 0;
+        for (int j = threadIdx.x; j < L - 1; j += 256) s += R[j] * buf[t + j];
+        block_sum2<T>(s, dummy, red);
+        if (threadIdx.x == 0) {
+            buf[L - 1 + t] = s;
+            out[t] = (T)s;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+extern "C" {
+
+int ssa_dev_twiddles(int prec, void **table_out, void *stream)
+{
+    const int n = 1 << SSA_NMAX_LOG;
+    const size_t esz = prec == SSA_PREC_F64 ? 16 : 8;
+    void *host = malloc(n * esz);
+    if (!host) return -1;
+    for (int m = 0; m < n; m++) {
+        // exp(+2 pi i m / n) in long double, octant-symmetric evaluation
+        long double a = 6.283185307179586476925286766559005768L * (long double)m / (long double)n;
+        long double c = cosl(a), s = sinl(a);
+        if (prec == SSA_PREC_F64) {
+            ((double *)host)[2 * m] = (double)c;
+            ((double *)host)[2 * m + 1] = (double)s;
+        } else {
+            ((float *)host)[2 * m] = (float)c;
+            ((float *)host)[2 * m + 1] = (float)s;
+        }
+    }
+    void *dev = nullptr;
+    hipError_t e = hipMalloc(&dev, n * esz);
+    if (e == hipSuccess) e = hipMemcpyAsync(dev, host, n * esz, hipMemcpyHostToDevice, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    free(host);
+    if (e != hipSuccess) return (int)e;
+    *table_out = dev;
+    return 0;
+}
+
+#define SSA_LGN_CASES(X) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13)
+
+static bool lgn_ok(int prec, int lgn) { (void)prec; return lgn >= 6 && lgn <= 13; }
+
+int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk,
+                    const void *tw, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (!lgn_ok(prec, lgn)) return -2;
+    hipStream_t st = (hipStream_t)stream;
+#define CASE(L)                                                                                               \
+    case L:                                                                                                   \
+        if (prec == SSA_PREC_F64)                                                                             \
+            hipLaunchKernelGGL((k_spectra<double, L>), dim3(n_items),                         \
+                               dim3(FftGeom<double, L>::NT), 0, st, D, items, (double *)bulk, \
+                               (const C2<double> *)tw);                                                       \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_spectra<float, L>), dim3(n_items), dim3(FftGeom<float, L>::NT), 0, st, D,   \
+                               items, (float *)bulk, (const C2<float> *)tw);                                  \
+        break;
+    switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
+#undef CASE
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_matvec(int prec, int lgn, int dir, int gen_omega, int in_panel, int out_panel, const ssa_desc *D,
+                   const ssa_item *items, int n_items, void *bulk, const void *tw, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (!lgn_ok(prec, lgn)) return -2;
+    hipStream_t st = (hipStream_t)stream;
+#define LAUNCH(T, L, GEN)                                                                                     \
+    hipLaunchKernelGGL((k_matvec<T, L, GEN>), dim3(n_items), dim3(FftGeom<T, L>::NT), 0, st, D, items,        \
+                       (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel)
+#define CASE(L)                                                                                               \
+    case L:                                                                                                   \
+        if (prec == SSA_PREC_F64) {                                                                           \
+            if (gen_omega) LAUNCH(double, L, 1);                                              \
+            else LAUNCH(double, L, 0);                                                        \
+        } else {                                                                                              \
+            if (gen_omega) LAUNCH(float, L, 1);                                                               \
+            else LAUNCH(float, L, 0);                                                                         \
+        }                                                                                                     \
+        break;
+    switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
+#undef CASE
+#undef LAUNCH
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_gram(int prec, int panel, int use_m, int bmax, const ssa_desc *D, const ssa_item *items, int n_items,
+                 void *bulk, double *small, void *stream)
+{
+    if (n_items <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t lds = 2 * SUB * LDP * sizeof(double);
+    // 16 x 16 threads of BB x BB blocks: BB = 4 covers b <= 64, BB = 8 covers b <= 128
+    if (prec == SSA_PREC_F64) {
+        if (bmax <= 64)
+            hipLaunchKernelGGL((k_gram<double, 4>), dim3(n_items), dim3(256), lds, st, D, items,
+                               (const double *)bulk, small, panel, use_m);
+        else
+            hipLaunchKernelGGL((k_gram<double, 8>), dim3(n_items), dim3(256), lds, st, D, items,
+                               (const double *)bulk, small, panel, use_m);
+    } else {
+        if (bmax <= 64)
+            hipLaunchKernelGGL((k_gram<float, 4>), dim3(n_items), dim3(256), lds, st, D, items,
+                               (const float *)bulk, small, panel, use_m);
+        else
+            hipLaunchKernelGGL((k_gram<float, 8>), dim3(n_items), dim3(256), lds, st, D, items,
+                               (const float *)bulk, small, panel, use_m);
+    }
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_chol(int panel, int pass, const ssa_desc *D, const ssa_item *items, int n_items, double *small,
+                 void *stream)
+{
+    if (n_items <= 0) return 0;
+    const size_t lds = 2 * SSA_BMAX * LDP * sizeof(double);
+    static int attr = 0;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_chol, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return (int)e;
+        attr = 1;
+    }
+    hipLaunchKernelGGL(k_chol, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items, small, panel, pass,
+                       1e-13);
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, const ssa_desc *D, const ssa_item *items,
+                  int n_items, void *bulk, double *small, void *stream)
+{
+    if (n_items <= 0) return 0;
+    const size_t lds = SUB * LDP * sizeof(double);
+    if (prec == SSA_PREC_F64)
+        hipLaunchKernelGGL(k_apply<double>, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items,
+                           (double *)bulk, (const double *)small, src, dst, slot, ncols_k);
+    else
+        hipLaunchKernelGGL(k_apply<float>, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items,
+                           (float *)bulk, (const double *)small, src, dst, slot, ncols_k);
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_svd(const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream)
+{
+    if (n_items <= 0) return 0;
+    const size_t lds = 2 * (SSA_BMAX + 1) * LDP * sizeof(double);
+    static int attr = 0;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return (int)e;
+        attr = 1;
+    }
+    hipLaunchKernelGGL(k_svd, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items, small);
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_recon(int prec, int lgn, int scale_sigma, const ssa_desc *D, const ssa_item *items, int n_items,
+                  const int *gdesc, const int *glist, void *bulk, const double *small, const void *tw, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (!lgn_ok(prec, lgn)) return -2;
+    hipStream_t st = (hipStream_t)stream;
+#define CASE(L)                                                                                               \
+    case L:                                                                                                   \
+        if (prec == SSA_PREC_F64)                                                                             \
+            hipLaunchKernelGGL((k_recon<double, L>), dim3(n_items),                           \
+                               dim3(FftGeom<double, L>::NT), 0, st, D, items, (double *)bulk, \
+                               small, (const int4 *)gdesc, glist, (const C2<double> *)tw, scale_sigma);       \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_recon<float, L>), dim3(n_items), dim3(FftGeom<float, L>::NT), 0, st, D,     \
+                               items, (float *)bulk, small, (const int4 *)gdesc, glist,                       \
+                               (const C2<float> *)tw, scale_sigma);                                           \
+        break;
+    switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
+#undef CASE
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_olap(int prec, int weight, const ssa_desc *D, const ssa_item *items, int n_items, const int *gdesc,
+                 void *bulk, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (prec == SSA_PREC_F64)
+        hipLaunchKernelGGL(k_olap<double>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
+                           (const int4 *)gdesc, (double *)bulk, weight);
+    else
+        hipLaunchKernelGGL(k_olap<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
+                           (const int4 *)gdesc, (float *)bulk, weight);
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_wcorr(int prec, const ssa_desc *D, const ssa_item *tile_items, int n_tiles,
+                  const ssa_item *series_items, int n_series, void *bulk, double *small, void *stream)
+{
+    int e = ssa_dev_gram(prec, 4, 0, SSA_BMAX, D, tile_items, n_tiles, bulk, small, stream);
+    if (e) return e;
+    if (n_series <= 0) return 0;
+    hipLaunchKernelGGL(k_wnorm, dim3(n_series), dim3(256), 0, (hipStream_t)stream, D, series_items, small);
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_forecast(int prec, int h, const ssa_desc *D, const ssa_item *items, int n_items, const int *gdesc,
+                     const int *glist, void *bulk, double *small, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (prec == SSA_PREC_F64)
+        hipLaunchKernelGGL(k_forecast<double>, dim3(n_items), dim3(256), 0, (hipStream_t)stream,
+                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (double *)bulk, small, h);
+    else
+        hipLaunchKernelGGL(k_forecast<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream,
+                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (float *)bulk, small, h);
+    return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,821 @@
+/*
+ * ssa_opt.c -- the reference's C API (include/ssa_opt.h) on the device engine.
+ *
+ * Every entry point keeps the argument meaning, validation and return
+ * convention of the reference function it replaces (cited per function);
+ * the arithmetic runs in the HIP kernels through ssa_engine.  Host code here
+ * only validates, mirrors results for C callers and does the O(k) / O(L)
+ * bookkeeping the reference also does on the host (getters, statistics,
+ * LRF coefficients handed back to the caller).
+ */
+#include "../../include/ssa_opt.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ssa_engine.h"
+
+#define SSA_MAX_H 65536        /* forecast horizon kept in the output slot */
+
+typedef struct {
+    ssa_engine *eng;
+    double *x;                 /* host copy of the signal(s) */
+    double *hostblk;           /* host views: fft_x and ws_* */
+    int injected;              /* width b of an Omega injected for the next decomposition (0: none) */
+    int pending_q;             /* q override of the next decomposition (-1: ctx q) */
+} ssa_ctx;
+
+static int imin(int a, int b) { return a < b ? a : b; }
+static int imax(int a, int b) { return a > b ? a : b; }
+
+static int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return (v && *v) ? atoi(v) : dflt;
+}
+
+static int env_precision(void)
+{
+    const char *v = getenv("SSA_AMD_PRECISION");
+    if (!v) return 0;
+    return (!strcmp(v, "mixed") || !strcmp(v, "1") || !strcmp(v, "fp32")) ? 1 : 0;
+}
+
+static void weights(double *w, int N, int L, int K)
+{
+    for (int t = 0; t < N; t++) {
+        int c = imin(imin(t + 1, L), imin(K, N - t));
+        w[t] = c > 0 ? 1.0 / c : 0.0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* SSA lifecycle                                                             */
+/* ------------------------------------------------------------------------ */
+
+static int build_engine(SSA_Opt *ssa, ssa_ctx *c)
+{
+    int err = 0;
+    int N = ssa->N, L = ssa->L, M = 1;
+    c->eng = ssa_engine_create(ssa->precision, 1, &N, &L, &M, NULL, NULL, ssa->q, &err);
+    if (!c->eng) return -1;
+    if (ssa_engine_upload_signal(c->eng, 0, c->x) || ssa_engine_spectra(c->eng)) return -1;
+    const ssa_desc *d = ssa_engine_desc(c->eng, 0);
+    ssa->fft_len = d->n;
+    ssa->r2c_len = d->n / 2 + 1;
+    ssa->batch_size = 2;
+    if (ssa_engine_read_spectrum(c->eng, 0, ssa->fft_x, ssa->r2c_len)) return -1;
+    return 0;
+}
+
+/* MKL/ssa_opt.h:1026-1284 */
+int ssa_opt_init(SSA_Opt *ssa, const ssa_real *x, int N, int L)
+{
+    if (!ssa || !x || N < 4 || L < 2 || L > N - 1) return -1;
+    memset(ssa, 0, sizeof(SSA_Opt));
+    ssa->N = N;
+    ssa->L = L;
+    ssa->K = N - L + 1;
+    ssa->q = env_int("SSA_AMD_Q", 2);
+    if (ssa->q < 0) ssa->q = 0;
+    ssa->precision = env_precision();
+    ssa->seed = 42;
+    ssa_ctx *c = (ssa_ctx *)calloc(1, sizeof(ssa_ctx));
+    if (!c) return -1;
+    ssa->dev = c;
+    c->pending_q = -1;
+    c->x = (double *)malloc(sizeof(double) * N);
+    ssa->inv_diag_count = (double *)malloc(sizeof(double) * N);
+    /* host views: fft_x (up to 4097 complex) + 6 small scratch views */
+    c->hostblk = (double *)calloc(2 * 4097 + 6 * 8, sizeof(double));
+    if (!c->x || !ssa->inv_diag_count || !c->hostblk) { ssa_opt_free(ssa); return -1; }
+    memcpy(c->x, x, sizeof(double) * N);
+    weights(ssa->inv_diag_count, N, L, ssa->K);
+    ssa->fft_x = c->hostblk;
+    double *w = c->hostblk + 2 * 4097;
+    ssa->ws_real = w; ssa->ws_complex = w + 8; ssa->ws_real2 = w + 16;
+    ssa->ws_proj = w + 24; ssa->ws_batch_real = w + 32; ssa->ws_batch_complex = w + 40;
+    if (build_engine(ssa, c)) { ssa_opt_free(ssa); return -1; }
+    ssa->initialized = true;
+    return 0;
+}
+
+static void free_results(SSA_Opt *ssa)
+{
+    free(ssa->U); free(ssa->V); free(ssa->sigma); free(ssa->eigenvalues);
+    ssa->U = ssa->V = ssa->sigma = ssa->eigenvalues = NULL;
+}
+
+/* MKL/ssa_opt.h:1286-1350: safe on a zeroed or failed context */
+void ssa_opt_free(SSA_Opt *ssa)
+{
+    if (!ssa) return;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    if (c) {
+        if (c->eng) ssa_engine_destroy(c->eng);
+        free(c->x);
+        free(c->hostblk);
+        free(c);
+    }
+    free_results(ssa);
+    free(ssa->inv_diag_count);
+    memset(ssa, 0, sizeof(SSA_Opt));
+}
+
+/* MKL/ssa_opt.h:1352-1379 */
+void ssa_opt_free_prepared(SSA_Opt *ssa)
+{
+    if (!ssa) return;
+    ssa->prepared_kp = 0;
+    ssa->prepared_lwork = 0;
+    ssa->prepared = false;
+}
+
+static int prepare_caps(SSA_Opt *ssa, int kp, int k)
+{
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    if (kp > SSA_BMAX || kp < 1) return -1;
+    int b = imax(kp, 2);
+    if (ssa_engine_set_caps(c->eng, 0, b, imin(k, b))) return -1;
+    if (ssa_engine_prepare(c->eng, imin(k, b), SSA_MAX_H)) return -1;
+    free_results(ssa);
+    ssa->U = (double *)calloc((size_t)ssa->L * k, sizeof(double));
+    ssa->V = (double *)calloc((size_t)ssa->K * k, sizeof(double));
+    ssa->sigma = (double *)calloc(k, sizeof(double));
+    ssa->eigenvalues = (double *)calloc(k, sizeof(double));
+    if (!ssa->U || !ssa->V || !ssa->sigma || !ssa->eigenvalues) return -1;
+    ssa->n_components = k;
+    ssa->prepared_kp = kp;
+    ssa->prepared = true;
+    ssa->decomposed = false;
+    return 0;
+}
+
+/* MKL/ssa_opt.h:1414-1543: kp = min(max_k + p, L, K), p <= 0 -> 8 */
+int ssa_opt_prepare(SSA_Opt *ssa, int max_k, int oversampling)
+{
+    if (!ssa || !ssa->initialized || max_k < 1) return -1;
+    ssa_opt_free_prepared(ssa);
+    int p = oversampling <= 0 ? 8 : oversampling;
+    int kp = imin(max_k + p, imin(ssa->L, ssa->K));
+    int k = imin(max_k, kp);
+    return prepare_caps(ssa, kp, k);
+}
+
+/* MKL/ssa_opt.h:1545-1558 */
+int ssa_opt_update_signal(SSA_Opt *ssa, const ssa_real *new_x)
+{
+    if (!ssa || !ssa->initialized || !new_x) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    memcpy(c->x, new_x, sizeof(double) * ssa->N);
+    ssa->decomposed = false;
+    ssa->fft_cached = false;
+    if (ssa_engine_upload_signal(c->eng, 0, c->x) || ssa_engine_spectra(c->eng)) return -1;
+    if (ssa_engine_read_spectrum(c->eng, 0, ssa->fft_x, ssa->r2c_len)) return -1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* decomposition                                                             */
+/* ------------------------------------------------------------------------ */
+
+static int run_decomposition(SSA_Opt *ssa, int kp, int k, int q)
+{
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    if (c->injected && c->injected != kp) { c->injected = 0; return -1; }
+    if (ssa_engine_set_bk(c->eng, 0, imax(kp, 2), k)) return -1;
+    if (ssa_engine_set_rng(c->eng, 0, ssa->seed, ssa->calls)) return -1;
+    int rc = ssa_engine_decompose(c->eng, q, c->injected);
+    c->injected = 0;
+    ssa->calls++;
+    if (rc) return -1;
+    double tv = 0;
+    if (ssa_engine_results(c->eng, 0, ssa->sigma, ssa->eigenvalues, &tv, ssa->U, ssa->V, NULL) != k) return -1;
+    ssa->total_variance = tv;
+    ssa->n_components = k;
+    ssa->decomposed = true;
+    return 0;
+}
+
+/* MKL/ssa_opt.h:1685-1763; requires prepare, kp <= prepared_kp */
+int ssa_opt_decompose_randomized(SSA_Opt *ssa, int k, int oversampling)
+{
+    if (!ssa || !ssa->initialized || k < 1) return -1;
+    ssa_opt_free_cached_ffts(ssa);
+    int p = oversampling <= 0 ? 8 : oversampling;
+    int kp = imin(k + p, imin(ssa->L, ssa->K));
+    k = imin(k, kp);
+    if (!ssa->prepared || kp > ssa->prepared_kp) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    int q = c->pending_q >= 0 ? c->pending_q : ssa->q;
+    c->pending_q = -1;
+    return run_decomposition(ssa, kp, k, q);
+}
+
+static int subspace_decompose(SSA_Opt *ssa, int k, int kp, int max_iter)
+{
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    if (!ssa->prepared || kp > ssa->prepared_kp || k > ssa->n_components) {
+        if (prepare_caps(ssa, kp, k)) return -1;
+    }
+    int q = max_iter / 4;
+    if (q < 2) q = 2;
+    if (q > 30) q = 30;
+    c->pending_q = q;
+    int rc = run_decomposition(ssa, kp, k, q);
+    c->pending_q = -1;
+    return rc;
+}
+
+/* MKL/ssa_opt.h:1560-1678 (argument checks); device subspace iteration */
+int ssa_opt_decompose(SSA_Opt *ssa, int k, int max_iter)
+{
+    if (!ssa || !ssa->initialized || k < 1) return -1;
+    ssa_opt_free_cached_ffts(ssa);
+    int r = imin(ssa->L, ssa->K);
+    k = imin(k, r);
+    int kp = imin(k + 8, r);
+    if (kp > SSA_BMAX) return -1;
+    return subspace_decompose(ssa, k, imax(kp, imin(2, r)), max_iter);
+}
+
+/* MKL/ssa_opt.h:1765-1933 (argument checks); device subspace iteration */
+int ssa_opt_decompose_block(SSA_Opt *ssa, int k, int block_size, int max_iter)
+{
+    if (!ssa || !ssa->initialized || k < 1 || block_size < 1) return -1;
+    ssa_opt_free_cached_ffts(ssa);
+    int r = imin(ssa->L, ssa->K);
+    k = imin(k, r);
+    int kp = imin(imax(k + 8, block_size), r);
+    if (kp > SSA_BMAX) kp = imin(imax(k + 8, k), imin(SSA_BMAX, r));
+    if (kp > SSA_BMAX) return -1;
+    return subspace_decompose(ssa, k, imax(kp, imin(2, r)), max_iter);
+}
+
+/* MKL/ssa_opt.h:1935-2056: grow to n_components + additional_k */
+int ssa_opt_extend(SSA_Opt *ssa, int additional_k, int max_iter)
+{
+    if (!ssa || !ssa->decomposed || additional_k < 1) return -1;
+    return ssa_opt_decompose(ssa, ssa->n_components + additional_k, max_iter);
+}
+
+static int ensure_prepared_minimal(SSA_Opt *ssa)
+{
+    if (ssa->prepared) return 0;
+    return prepare_caps(ssa, 2, 1);
+}
+
+void ssa_opt_hankel_matvec(SSA_Opt *ssa, const ssa_real *v, ssa_real *y)
+{
+    if (!ssa || !ssa->initialized || !v || !y) return;
+    if (ensure_prepared_minimal(ssa)) return;
+    ssa_engine_matvec(((ssa_ctx *)ssa->dev)->eng, 0, 0, v, y);
+}
+
+void ssa_opt_hankel_matvec_T(SSA_Opt *ssa, const ssa_real *u, ssa_real *y)
+{
+    if (!ssa || !ssa->initialized || !u || !y) return;
+    if (ensure_prepared_minimal(ssa)) return;
+    ssa_engine_matvec(((ssa_ctx *)ssa->dev)->eng, 0, 1, u, y);
+}
+
+/* ------------------------------------------------------------------------ */
+/* reconstruction and analysis                                               */
+/* ------------------------------------------------------------------------ */
+
+/* MKL/ssa_opt.h:2325-2400: out-of-range indices skipped; n_group < 1 fails */
+int ssa_opt_reconstruct(const SSA_Opt *ssa, const int *group, int n_group, ssa_real *output)
+{
+    if (!ssa || !ssa->decomposed || !group || !output || n_group < 1) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    return ssa_engine_reconstruct(c->eng, 0, 0, group, &n_group, 1, 1, 0, output);
+}
+
+int ssa_amd_reconstruct_many(const SSA_Opt *ssa, const int *glist, const int *gcount, int n_groups,
+                             ssa_real *output)
+{
+    if (!ssa || !ssa->decomposed || !glist || !gcount || !output || n_groups < 1) return -1;
+    for (int g = 0; g < n_groups; g++) if (gcount[g] < 0) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    return ssa_engine_reconstruct(c->eng, 0, 0, glist, gcount, n_groups, 1, 0, output);
+}
+
+/* MKL/ssa_opt.h:2110-2323.  The device reconstruction reads U and V from
+ * HBM and forms the component spectra on the fly, so there is no separate
+ * cache to build; the call keeps the reference's contract (requires a
+ * decomposition, sets the flag that update_signal / decompose clear). */
+int ssa_opt_cache_ffts(SSA_Opt *ssa)
+{
+    if (!ssa || !ssa->decomposed) return -1;
+    ssa->fft_cached = true;
+    return 0;
+}
+
+void ssa_opt_free_cached_ffts(SSA_Opt *ssa)
+{
+    if (ssa) ssa->fft_cached = false;
+}
+
+/* MKL/ssa_opt.h:2407-2477 */
+int ssa_opt_wcorr_matrix(const SSA_Opt *ssa, ssa_real *W)
+{
+    if (!ssa || !ssa->decomposed || !W) return -1;
+    return ssa_engine_wcorr(((ssa_ctx *)ssa->dev)->eng, 0, W);
+}
+
+/* MKL/ssa_opt.h:2492-2548 (same result as the slow path) */
+int ssa_opt_wcorr_matrix_fast(const SSA_Opt *ssa, ssa_real *W) { return ssa_opt_wcorr_matrix(ssa, W); }
+
+/* MKL/ssa_opt.h:2551-2625 */
+ssa_real ssa_opt_wcorr_pair(const SSA_Opt *ssa, int i, int j)
+{
+    if (!ssa || !ssa->decomposed || i < 0 || i >= ssa->n_components || j < 0 || j >= ssa->n_components) return 0.0;
+    int k = ssa->n_components;
+    double *W = (double *)malloc(sizeof(double) * k * k);
+    if (!W) return 0.0;
+    double r = 0.0;
+    if (ssa_opt_wcorr_matrix(ssa, W) == 0) r = W[(size_t)i * k + j];
+    free(W);
+    return r;
+}
+
+/* MKL/ssa_opt.h:2627-2676 */
+int ssa_opt_component_stats(const SSA_Opt *ssa, SSA_ComponentStats *stats)
+{
+    if (!ssa || !ssa->decomposed || !stats) return -1;
+    int n = ssa->n_components;
+    if (n < 2) return -1;
+    memset(stats, 0, sizeof(SSA_ComponentStats));
+    stats->n = n;
+    stats->singular_values = (double *)malloc(sizeof(double) * n);
+    stats->log_sv = (double *)malloc(sizeof(double) * n);
+    stats->gaps = (double *)malloc(sizeof(double) * (n - 1));
+    stats->cumulative_var = (double *)malloc(sizeof(double) * n);
+    stats->second_diff = (double *)malloc(sizeof(double) * n);
+    if (!stats->singular_values || !stats->log_sv || !stats->gaps || !stats->cumulative_var || !stats->second_diff) {
+        ssa_opt_component_stats_free(stats);
+        return -1;
+    }
+    for (int i = 0; i < n; i++) {
+        stats->singular_values[i] = ssa->sigma[i];
+        stats->log_sv[i] = log(ssa->sigma[i] + 1e-300);
+    }
+    double max_gap = 0.0;
+    int max_idx = 0;
+    for (int i = 0; i < n - 1; i++) {
+        double g = ssa->sigma[i] / (ssa->sigma[i + 1] + 1e-300);
+        stats->gaps[i] = g;
+        if (g > max_gap) { max_gap = g; max_idx = i; }
+    }
+    double cs = 0.0;
+    for (int i = 0; i < n; i++) {
+        cs += ssa->eigenvalues[i];
+        stats->cumulative_var[i] = cs / ssa->total_variance;
+    }
+    stats->second_diff[0] = 0.0;
+    stats->second_diff[n - 1] = 0.0;
+    for (int i = 1; i < n - 1; i++)
+        stats->second_diff[i] = stats->log_sv[i - 1] - 2.0 * stats->log_sv[i] + stats->log_sv[i + 1];
+    stats->suggested_signal = max_idx + 1;
+    stats->gap_threshold = max_gap;
+    return 0;
+}
+
+void ssa_opt_component_stats_free(SSA_ComponentStats *stats)
+{
+    if (!stats) return;
+    free(stats->singular_values); free(stats->log_sv); free(stats->gaps);
+    free(stats->cumulative_var); free(stats->second_diff);
+    memset(stats, 0, sizeof(SSA_ComponentStats));
+}
+
+/* MKL/ssa_opt.h:2690-2756: greedy pairing by singular-value ratio and |W| */
+int ssa_opt_find_periodic_pairs(const SSA_Opt *ssa, int *pairs, int max_pairs, ssa_real sv_tol,
+                                ssa_real wcorr_thresh)
+{
+    if (!ssa || !ssa->decomposed || !pairs || max_pairs < 1) return 0;
+    if (sv_tol <= 0) sv_tol = 0.1;
+    if (wcorr_thresh <= 0) wcorr_thresh = 0.5;
+    int n = ssa->n_components, np = 0;
+    double *W = (double *)malloc(sizeof(double) * n * n);
+    char *used = (char *)calloc(n, 1);
+    if (!W || !used || ssa_opt_wcorr_matrix(ssa, W)) { free(W); free(used); return 0; }
+    for (int i = 0; i < n - 1 && np < max_pairs; i++) {
+        if (used[i]) continue;
+        for (int j = i + 1; j < n && np < max_pairs; j++) {
+            if (used[j]) continue;
+            double ratio = ssa->sigma[j] / (ssa->sigma[i] + 1e-300);
+            if (fabs(1.0 - ratio) > sv_tol) continue;
+            if (fabs(W[(size_t)i * n + j]) < wcorr_thresh) continue;
+            pairs[2 * np] = i;
+            pairs[2 * np + 1] = j;
+            used[i] = used[j] = 1;
+            np++;
+            break;
+        }
+    }
+    free(W);
+    free(used);
+    return np;
+}
+
+/* MKL/ssa_opt.h:3086-3090 */
+int ssa_opt_get_trend(const SSA_Opt *ssa, ssa_real *output)
+{
+    int g[1] = {0};
+    return ssa_opt_reconstruct(ssa, g, 1, output);
+}
+
+/* MKL/ssa_opt.h:3091-3110 */
+int ssa_opt_get_noise(const SSA_Opt *ssa, int noise_start, ssa_real *output)
+{
+    if (!ssa || !ssa->decomposed || noise_start < 0) return -1;
+    int nn = ssa->n_components - noise_start;
+    if (nn <= 0) {
+        memset(output, 0, sizeof(double) * ssa->N);
+        return 0;
+    }
+    int *g = (int *)malloc(sizeof(int) * nn);
+    if (!g) return -1;
+    for (int i = 0; i < nn; i++) g[i] = noise_start + i;
+    int rc = ssa_opt_reconstruct(ssa, g, nn, output);
+    free(g);
+    return rc;
+}
+
+/* MKL/ssa_opt.h:3111-3121 */
+ssa_real ssa_opt_variance_explained(const SSA_Opt *ssa, int start, int end)
+{
+    if (!ssa || !ssa->decomposed || start < 0 || ssa->total_variance <= 0) return 0.0;
+    if (end < 0 || end >= ssa->n_components) end = ssa->n_components - 1;
+    double s = 0;
+    for (int i = start; i <= end; i++) s += ssa->eigenvalues[i];
+    return s / ssa->total_variance;
+}
+
+int ssa_opt_get_singular_values(const SSA_Opt *ssa, ssa_real *output, int max_n)   /* :3123-3130 */
+{
+    if (!ssa || !ssa->decomposed || !output) return -1;
+    int n = imin(max_n, ssa->n_components);
+    if (n > 0) memcpy(output, ssa->sigma, sizeof(double) * n);
+    return n;
+}
+
+int ssa_opt_get_eigenvalues(const SSA_Opt *ssa, ssa_real *output, int max_n)       /* :3132-3139 */
+{
+    if (!ssa || !ssa->decomposed || !output) return -1;
+    int n = imin(max_n, ssa->n_components);
+    if (n > 0) memcpy(output, ssa->eigenvalues, sizeof(double) * n);
+    return n;
+}
+
+ssa_real ssa_opt_get_total_variance(const SSA_Opt *ssa)                            /* :3141-3145 */
+{
+    if (!ssa || !ssa->decomposed) return 0.0;
+    return ssa->total_variance;
+}
+
+/* ------------------------------------------------------------------------ */
+/* forecasting                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* MKL/ssa_opt.h:2766-2816: coefficients handed back to the caller, from the
+ * host mirror of U */
+int ssa_opt_compute_lrf(const SSA_Opt *ssa, const int *group, int n_group, SSA_LRF *lrf)
+{
+    if (!ssa || !ssa->decomposed || !group || !lrf || n_group < 1) return -1;
+    int L = ssa->L;
+    memset(lrf, 0, sizeof(SSA_LRF));
+    lrf->L = L;
+    double nu = 0.0;
+    for (int g = 0; g < n_group; g++) {
+        int idx = group[g];
+        if (idx < 0 || idx >= ssa->n_components) return -1;
+        double pi = ssa->U[(size_t)idx * L + L - 1];
+        nu += pi * pi;
+    }
+    lrf->verticality = nu;
+    if (nu >= 1.0 - 1e-10) return -1;
+    lrf->R = (double *)malloc(sizeof(double) * (L > 1 ? L - 1 : 1));
+    if (!lrf->R) return -1;
+    double sc = 1.0 / (1.0 - nu);
+    for (int j = 0; j < L - 1; j++) {
+        double s = 0.0;
+        for (int g = 0; g < n_group; g++) {
+            int idx = group[g];
+            s += ssa->U[(size_t)idx * L + L - 1] * ssa->U[(size_t)idx * L + j];
+        }
+        lrf->R[j] = s * sc;
+    }
+    lrf->valid = true;
+    return 0;
+}
+
+void ssa_opt_lrf_free(SSA_LRF *lrf)   /* MKL/ssa_opt.h:2758-2764 */
+{
+    if (!lrf) return;
+    free(lrf->R);
+    memset(lrf, 0, sizeof(SSA_LRF));
+}
+
+/* MKL/ssa_opt.h:2818-2841 */
+int ssa_opt_forecast_with_lrf(const SSA_LRF *lrf, const ssa_real *base_signal, int base_len, int n_forecast,
+                              ssa_real *output)
+{
+    if (!lrf || !lrf->valid || !lrf->R || !base_signal || !output) return -1;
+    int L = lrf->L, w = L - 1;
+    if (base_len < w || n_forecast < 1) return -1;
+    double *buf = (double *)malloc(sizeof(double) * ((size_t)w + n_forecast));
+    if (!buf) return -1;
+    for (int i = 0; i < w; i++) buf[i] = base_signal[base_len - w + i];
+    for (int h = 0; h < n_forecast; h++) {
+        double f = 0.0;
+        for (int j = 0; j < w; j++) f += lrf->R[j] * buf[h + j];
+        buf[w + h] = f;
+        output[h] = f;
+    }
+    free(buf);
+    return 0;
+}
+
+/* MKL/ssa_opt.h:2843-2867: device-resident reconstruction, LRF and recurrence */
+int ssa_opt_forecast(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output)
+{
+    if (!ssa || !ssa->decomposed || !group || !output || n_group < 1 || n_forecast < 1) return -1;
+    if (n_forecast > SSA_MAX_H) return -1;
+    return ssa_engine_forecast(((ssa_ctx *)ssa->dev)->eng, 0, group, n_group, n_forecast, output, 0);
+}
+
+/* MKL/ssa_opt.h:2869-2885 */
+int ssa_opt_forecast_full(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output)
+{
+    if (!ssa || !ssa->decomposed || !group || !output || n_group < 1 || n_forecast < 1) return -1;
+    if (n_forecast > SSA_MAX_H) return -1;
+    return ssa_engine_forecast(((ssa_ctx *)ssa->dev)->eng, 0, group, n_group, n_forecast, output, 1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* extensions                                                                */
+/* ------------------------------------------------------------------------ */
+
+int ssa_amd_set_options(SSA_Opt *ssa, int q, int precision, unsigned long long seed)
+{
+    if (!ssa || !ssa->initialized || q < 0 || (precision != 0 && precision != 1)) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    ssa->q = q;
+    ssa->seed = seed;
+    ssa->calls = 0;
+    if (precision != ssa->precision) {
+        ssa_engine_destroy(c->eng);
+        c->eng = NULL;
+        ssa->precision = precision;
+        free_results(ssa);
+        ssa->prepared = false;
+        ssa->prepared_kp = 0;
+        ssa->decomposed = false;
+        ssa->n_components = 0;
+        if (build_engine(ssa, c)) return -1;
+    }
+    return 0;
+}
+
+int ssa_amd_inject_omega(SSA_Opt *ssa, const ssa_real *Omega, int K, int b)
+{
+    if (!ssa || !ssa->prepared || !Omega || K != ssa->K || b < 1 || b > ssa->prepared_kp) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    /* the engine's current width must be b for the copy */
+    if (ssa_engine_set_bk(c->eng, 0, imax(b, 2), imin(ssa->n_components, imax(b, 2)))) return -1;
+    if (b < 2) return -1;
+    if (ssa_engine_inject_omega(c->eng, 0, Omega)) return -1;
+    c->injected = b;
+    return 0;
+}
+
+int ssa_amd_plan_info(const SSA_Opt *ssa, int *out6)
+{
+    if (!ssa || !ssa->initialized || !out6) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    const ssa_desc *d = ssa_engine_desc(c->eng, 0);
+    out6[0] = d->n; out6[1] = d->CL; out6[2] = d->CK; out6[3] = d->AL; out6[4] = d->SK;
+    double sw = 0;
+    out6[5] = 0;
+    if (ssa->decomposed && ssa_engine_results(c->eng, 0, NULL, NULL, NULL, NULL, NULL, &sw) >= 0) out6[5] = (int)sw;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* MSSA (MKL/ssa_opt.h:3881-4283)                                            */
+/* ------------------------------------------------------------------------ */
+
+static int mssa_build(MSSA_Opt *m, ssa_ctx *c)
+{
+    int err = 0;
+    c->eng = ssa_engine_create(m->precision, 1, &m->N, &m->L, &m->M, NULL, NULL, m->q, &err);
+    if (!c->eng) return -1;
+    if (ssa_engine_upload_signal(c->eng, 0, c->x) || ssa_engine_spectra(c->eng)) return -1;
+    const ssa_desc *d = ssa_engine_desc(c->eng, 0);
+    m->fft_len = d->n;
+    m->r2c_len = d->n / 2 + 1;
+    return 0;
+}
+
+/* MKL/ssa_opt.h:3907-4057 */
+int mssa_opt_init(MSSA_Opt *mssa, const ssa_real *X, int M, int N, int L)
+{
+    if (!mssa || !X || M < 1 || N < 4 || L < 2 || L > N - 1) return -1;
+    memset(mssa, 0, sizeof(MSSA_Opt));
+    mssa->M = M; mssa->N = N; mssa->L = L; mssa->K = N - L + 1;
+    mssa->q = env_int("SSA_AMD_Q", 2);
+    if (mssa->q < 0) mssa->q = 0;
+    mssa->precision = env_precision();
+    mssa->seed = 42;
+    ssa_ctx *c = (ssa_ctx *)calloc(1, sizeof(ssa_ctx));
+    if (!c) return -1;
+    mssa->dev = c;
+    c->x = (double *)malloc(sizeof(double) * (size_t)M * N);
+    mssa->inv_diag_count = (double *)malloc(sizeof(double) * N);
+    if (!c->x || !mssa->inv_diag_count) { mssa_opt_free(mssa); return -1; }
+    memcpy(c->x, X, sizeof(double) * (size_t)M * N);
+    weights(mssa->inv_diag_count, N, L, mssa->K);
+    if (mssa_build(mssa, c)) { mssa_opt_free(mssa); return -1; }
+    mssa->initialized = true;
+    return 0;
+}
+
+void mssa_opt_free(MSSA_Opt *mssa)   /* MKL/ssa_opt.h:3881-3905 */
+{
+    if (!mssa) return;
+    ssa_ctx *c = (ssa_ctx *)mssa->dev;
+    if (c) {
+        if (c->eng) ssa_engine_destroy(c->eng);
+        free(c->x);
+        free(c);
+    }
+    free(mssa->U); free(mssa->V); free(mssa->sigma); free(mssa->eigenvalues); free(mssa->inv_diag_count);
+    memset(mssa, 0, sizeof(MSSA_Opt));
+}
+
+int mssa_amd_set_options(MSSA_Opt *mssa, int q, int precision, unsigned long long seed)
+{
+    if (!mssa || !mssa->initialized || q < 0 || (precision != 0 && precision != 1)) return -1;
+    ssa_ctx *c = (ssa_ctx *)mssa->dev;
+    mssa->q = q;
+    mssa->seed = seed;
+    mssa->calls = 0;
+    if (precision != mssa->precision) {
+        ssa_engine_destroy(c->eng);
+        c->eng = NULL;
+        mssa->precision = precision;
+        mssa->decomposed = false;
+        if (mssa_build(mssa, c)) return -1;
+    }
+    return 0;
+}
+
+/* MKL/ssa_opt.h:4116-4210: kp = min(k + p, M L, K), p <= 0 -> 8 */
+int mssa_opt_decompose(MSSA_Opt *mssa, int k, int oversampling)
+{
+    if (!mssa || !mssa->initialized || k < 1) return -1;
+    ssa_ctx *c = (ssa_ctx *)mssa->dev;
+    int ML = mssa->M * mssa->L;
+    int p = oversampling <= 0 ? 8 : oversampling;
+    int kp = imin(k + p, imin(ML, mssa->K));
+    k = imin(k, kp);
+    int b = imax(kp, 2);
+    if (b > SSA_BMAX) return -1;
+    if (ssa_engine_set_caps(c->eng, 0, b, k) || ssa_engine_prepare(c->eng, 1, 0)) return -1;
+    if (ssa_engine_set_bk(c->eng, 0, b, k) || ssa_engine_set_rng(c->eng, 0, mssa->seed, mssa->calls)) return -1;
+    int rc = ssa_engine_decompose(c->eng, mssa->q, 0);
+    mssa->calls++;
+    if (rc) return -1;
+    free(mssa->U); free(mssa->V); free(mssa->sigma); free(mssa->eigenvalues);
+    mssa->U = (double *)malloc(sizeof(double) * (size_t)ML * k);
+    mssa->V = (double *)malloc(sizeof(double) * (size_t)mssa->K * k);
+    mssa->sigma = (double *)malloc(sizeof(double) * k);
+    mssa->eigenvalues = (double *)malloc(sizeof(double) * k);
+    if (!mssa->U || !mssa->V || !mssa->sigma || !mssa->eigenvalues) return -1;
+    double tv = 0;
+    if (ssa_engine_results(c->eng, 0, mssa->sigma, mssa->eigenvalues, &tv, mssa->U, mssa->V, NULL) != k) return -1;
+    mssa->total_variance = tv;
+    mssa->n_components = k;
+    mssa->decomposed = true;
+    return 0;
+}
+
+/* MKL/ssa_opt.h:4212-4242 */
+int mssa_opt_reconstruct(const MSSA_Opt *mssa, int series_idx, const int *group, int n_group, ssa_real *output)
+{
+    if (!mssa || !mssa->decomposed || !group || !output || n_group < 1 || series_idx < 0 || series_idx >= mssa->M)
+        return -1;
+    ssa_ctx *c = (ssa_ctx *)mssa->dev;
+    return ssa_engine_reconstruct(c->eng, 0, series_idx, group, &n_group, 1, 1, 0, output);
+}
+
+/* MKL/ssa_opt.h:4244-4254 */
+int mssa_opt_reconstruct_all(const MSSA_Opt *mssa, const int *group, int n_group, ssa_real *output)
+{
+    if (!mssa || !mssa->decomposed || !group || !output || n_group < 1) return -1;
+    for (int m = 0; m < mssa->M; m++)
+        if (mssa_opt_reconstruct(mssa, m, group, n_group, output + (size_t)m * mssa->N) != 0) return -1;
+    return 0;
+}
+
+/* MKL/ssa_opt.h:4256-4270 */
+int mssa_opt_series_contributions(const MSSA_Opt *mssa, ssa_real *contributions)
+{
+    if (!mssa || !mssa->decomposed || !contributions) return -1;
+    int ML = mssa->M * mssa->L, k = mssa->n_components;
+    for (int i = 0; i < k; i++)
+        for (int m = 0; m < mssa->M; m++) {
+            const double *u = mssa->U + (size_t)i * ML + (size_t)m * mssa->L;
+            double s = 0;
+            for (int t = 0; t < mssa->L; t++) s += u[t] * u[t];
+            contributions[(size_t)m * k + i] = s;
+        }
+    return 0;
+}
+
+/* MKL/ssa_opt.h:4272-4283 */
+ssa_real mssa_opt_variance_explained(const MSSA_Opt *mssa, int start, int end)
+{
+    if (!mssa || !mssa->decomposed || start < 0 || mssa->total_variance <= 0) return 0.0;
+    if (end < 0 || end >= mssa->n_components) end = mssa->n_components - 1;
+    double s = 0;
+    for (int i = start; i <= end; i++) s += mssa->eigenvalues[i];
+    return s / mssa->total_variance;
+}
+
+/* ------------------------------------------------------------------------ */
+/* batched path                                                              */
+/* ------------------------------------------------------------------------ */
+
+struct ssa_amd_batch {
+    ssa_engine *eng;
+    int count, q;
+};
+
+ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const int *k, int p, int q,
+                                    int precision, unsigned long long seed)
+{
+    if (count < 1 || !N || !L || !k || q < 0 || (precision != 0 && precision != 1)) return NULL;
+    if (p <= 0) p = 8;
+    int *bm = (int *)malloc(sizeof(int) * count), *km = (int *)malloc(sizeof(int) * count);
+    if (!bm || !km) { free(bm); free(km); return NULL; }
+    for (int i = 0; i < count; i++) {
+        if (N[i] < 4 || L[i] < 2 || L[i] > N[i] - 1 || k[i] < 1) { free(bm); free(km); return NULL; }
+        int r = imin(L[i], N[i] - L[i] + 1);
+        int kp = imin(k[i] + p, r);
+        bm[i] = imax(kp, 2);
+        km[i] = imin(k[i], kp);
+        if (bm[i] > SSA_BMAX) { free(bm); free(km); return NULL; }
+    }
+    ssa_amd_batch *b = (ssa_amd_batch *)calloc(1, sizeof(ssa_amd_batch));
+    int err = 0;
+    if (b) b->eng = ssa_engine_create(precision, count, N, L, NULL, bm, km, q, &err);
+    free(bm);
+    free(km);
+    if (!b || !b->eng) { free(b); return NULL; }
+    b->count = count;
+    b->q = q;
+    if (ssa_engine_prepare(b->eng, 1, 0)) { ssa_amd_batch_free(b); return NULL; }
+    for (int i = 0; i < count; i++)
+        ssa_engine_set_rng(b->eng, i, seed ^ (0x9E3779B97F4A7C15ULL * (unsigned long long)(i + 1)), 0);
+    return b;
+}
+
+int ssa_amd_batch_set_signals_f32(ssa_amd_batch *b, const float *cat, const long long *offsets)
+{
+    if (!b || !cat || !offsets) return -1;
+    return ssa_engine_upload_signals_f32(b->eng, cat, offsets);
+}
+
+int ssa_amd_batch_run(ssa_amd_batch *b, int use_graph)
+{
+    if (!b) return -1;
+    return ssa_engine_step(b->eng, b->q, use_graph);
+}
+
+int ssa_amd_batch_sync(ssa_amd_batch *b) { return b ? ssa_engine_sync(b->eng) : -1; }
+
+int ssa_amd_batch_sigma(ssa_amd_batch *b, int i, ssa_real *sigma)
+{
+    if (!b || i < 0 || i >= b->count || !sigma) return -1;
+    return ssa_engine_results(b->eng, i, sigma, NULL, NULL, NULL, NULL, NULL);
+}
+
+int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out)
+{
+    if (!b || i < 0 || i >= b->count || !out) return -1;
+    return ssa_engine_step_outputs(b->eng, i, out);
+}
+
+long long ssa_amd_batch_bytes(const ssa_amd_batch *b) { return b ? ssa_engine_bytes(b->eng) : 0; }
+
+void ssa_amd_batch_free(ssa_amd_batch *b)
+{
+    if (!b) return;
+    ssa_engine_destroy(b->eng);
+    free(b);
+}
