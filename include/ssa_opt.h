@@ -180,6 +180,14 @@ int ssa_amd_plan_info(const SSA_Opt *ssa, int *out6);
 typedef struct ssa_amd_batch ssa_amd_batch;
 ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const int *k, int p, int q,
                                     int precision, unsigned long long seed);
+/* as above, with the group {0..g[i]-1} reconstructed by each step (g NULL: all
+ * k) and, when h > 0, an h-step LRF forecast of that group appended */
+ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, const int *k, const int *g, int h,
+                                       int p, int q, int precision, unsigned long long seed);
+/* 1 when the batch's queued work has finished, 0 if still running, -1 on error */
+int ssa_amd_batch_query(ssa_amd_batch *b);
+/* device status of series i after a step (0 ok, 2 forecast verticality >= 1) */
+int ssa_amd_batch_status(ssa_amd_batch *b, int i);
 /* signals concatenated as float32 (offsets[i] = first sample of series i) */
 int ssa_amd_batch_set_signals_f32(ssa_amd_batch *b, const float *cat, const long long *offsets);
 /* one SSA of every series: spectra, decomposition, reconstruction of all
@@ -187,8 +195,15 @@ int ssa_amd_batch_set_signals_f32(ssa_amd_batch *b, const float *cat, const long
 int ssa_amd_batch_run(ssa_amd_batch *b, int use_graph);
 int ssa_amd_batch_sync(ssa_amd_batch *b);
 int ssa_amd_batch_sigma(ssa_amd_batch *b, int i, ssa_real *sigma);       /* returns k_i */
-int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out);         /* N_i values */
+int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out);         /* N_i (+ h) values */
 long long ssa_amd_batch_bytes(const ssa_amd_batch *b);
+/* one eager step with HIP events around every kernel launch: per launch the
+ * phase (0 spectra, 1 matvec H, 2 matvec H^T, 3 Gram, 4 Cholesky, 5 apply,
+ * 6 small SVD, 7 U/V, 8 reconstruction, 9 overlap-add), the FFT length log2
+ * (or panel id) and the milliseconds; returns the number of launches */
+int ssa_amd_batch_profile(ssa_amd_batch *b, int max_rec, int *phase, int *lgn, float *ms);
+/* plan of series i: n, CL, CK, AL, SK, b, k, log2 n */
+int ssa_amd_batch_plan(const ssa_amd_batch *b, int i, int *out8);
 void ssa_amd_batch_free(ssa_amd_batch *b);
 
 #ifdef __cplusplus

@@ -33,7 +33,7 @@ typedef struct ssa_desc {
     int CL, CK, AL, SK;       /* chunk sizes and counts (AL per channel) */
     int tilesY, tilesZ;       /* Gram row tiles of the ML x b and K x b panels */
     int status;               /* set non-zero by device code on failure */
-    int pad0;
+    int gsize;                /* batched step: reconstruct / forecast the group {0..gsize-1} */
     /* element offsets into the bulk arena (T = double or float) */
     long long off_x;          /* M*N signal values */
     long long off_spec;       /* M*AL*SK spectra, n complex each (2n T) */
@@ -92,7 +92,7 @@ int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, const ssa_d
 int ssa_dev_svd(const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream);
 /* reconstruction: items (s, gid, row chunk o, col chunk sk); groups: gdesc[gid] = {start, count},
  * glist = component indices; scale_sigma 1: sum_g sigma_g u_g (*) v_g, 0: u (*) v */
-int ssa_dev_recon(int prec, int lgn, int scale_sigma, const ssa_desc *D, const ssa_item *items,
+int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, const ssa_desc *D, const ssa_item *items,
                   int n_items, const int *gdesc, const int *glist, void *bulk, const double *small,
                   const void *tw, void *stream);
 /* overlap-add of the staging pieces: items (s, gid, t tile); weight 0: w_t, 1: sqrt(w_t);
@@ -105,7 +105,7 @@ int ssa_dev_wcorr(int prec, const ssa_desc *D, const ssa_item *tile_items, int n
                   const ssa_item *series_items, int n_series, void *bulk, double *small, void *stream);
 /* LRF + forecast: per series, group gdesc[0]; reads the reconstruction at off_out, writes
  * h forecasts after it (off_out + N) */
-int ssa_dev_forecast(int prec, int h, const ssa_desc *D, const ssa_item *items, int n_items,
+int ssa_dev_forecast(int prec, int h, int limit_gsize, const ssa_desc *D, const ssa_item *items, int n_items,
                      const int *gdesc, const int *glist, void *bulk, double *small, void *stream);
 
 #ifdef __cplusplus

@@ -66,7 +66,31 @@ struct ssa_engine {
     hipGraphExec_t gexec;
     int graph_q;
     int has_graph;
+    /* per-launch profiler (HIP events on the engine stream) */
+    int prof_on, prof_n, prof_cap;
+    int step_h;                /* forecast horizon of the batched step (0: none) */
+    hipEvent_t *prof_ev;
+    int *prof_phase, *prof_lgn;
 };
+
+/* profiler phases */
+enum { PH_SPECTRA = 0, PH_MV_FWD, PH_MV_ADJ, PH_GRAM, PH_CHOL, PH_APPLY, PH_SVD, PH_UV, PH_RECON, PH_OLAP, PH_FORECAST,
+       PH_N };
+
+static void prof_begin(struct ssa_engine *e, int phase, int lgn)
+{
+    if (!e->prof_on || e->prof_n >= e->prof_cap) return;
+    e->prof_phase[e->prof_n] = phase;
+    e->prof_lgn[e->prof_n] = lgn;
+    hipEventRecord(e->prof_ev[2 * e->prof_n], e->st);
+}
+
+static void prof_end(struct ssa_engine *e)
+{
+    if (!e->prof_on || e->prof_n >= e->prof_cap) return;
+    hipEventRecord(e->prof_ev[2 * e->prof_n + 1], e->st);
+    e->prof_n++;
+}
 
 /* ------------------------------------------------------------------------ */
 /* planner                                                                   */
@@ -332,6 +356,8 @@ void ssa_engine_destroy(ssa_engine *e)
     if (e->ditems) hipFree(e->ditems);
     if (e->dgroups) hipFree(e->dgroups);
     if (e->hpin) hipHostFree(e->hpin);
+    for (int i = 0; i < 2 * e->prof_cap; i++) hipEventDestroy(e->prof_ev[i]);
+    free(e->prof_ev); free(e->prof_phase); free(e->prof_lgn);
     if (e->st) hipStreamDestroy(e->st);
     free(e->hd); free(e->bmax); free(e->kmax); free(e->hitems);
     free(e);
@@ -387,8 +413,12 @@ int ssa_engine_upload_signals_f32(ssa_engine *e, const float *cat, const long lo
 
 static int launch_spectra(ssa_engine *e)
 {
-    for (int lg = 6; lg < NLG; lg++)
+    for (int lg = 6; lg < NLG; lg++) {
+        if (!e->spec[lg].n) continue;
+        prof_begin(e, PH_SPECTRA, lg);
         DEVCHK(ssa_dev_spectra(e->prec, lg, e->dd, it_ptr(e, e->spec[lg]), e->spec[lg].n, e->arena_a, e->tw, e->st));
+        prof_end(e);
+    }
     return 0;
 }
 
@@ -449,6 +479,7 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
         d->off_Y += delta; d->off_Z += delta; d->off_U += delta; d->off_V += delta;
         d->off_stage += delta; d->off_out += delta;
         if (d->b == 0) { d->b = e->bmax[s]; d->k = e->kmax[s]; }
+        if (d->gsize <= 0 || d->gsize > d->k) d->gsize = d->k;
     }
     e->prepared = 1;
     e->max_groups = max_groups;
@@ -468,6 +499,7 @@ int ssa_engine_set_caps(ssa_engine *e, int s, int bmax, int kmax)
         e->kmax[s] = kmax;
         e->hd[s].b = bmax;
         e->hd[s].k = kmax;
+        e->hd[s].gsize = kmax;
         e->prepared = 0;
         e->max_groups = 0;
         e->max_h = 0;
@@ -498,12 +530,49 @@ int ssa_engine_set_bk(ssa_engine *e, int s, int b, int k)
     if (e->hd[s].b != b || e->hd[s].k != k) {
         e->hd[s].b = b;
         e->hd[s].k = k;
+        e->hd[s].gsize = k;
         e->lists_dirty = 1;
         e->desc_dirty = 1;
         e->step_groups_ready = 0;
         if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
     }
     return 0;
+}
+
+int ssa_engine_set_step(ssa_engine *e, int s, int gsize)
+{
+    if (s < 0 || s >= e->count || gsize < 1 || gsize > e->hd[s].k) return -1;
+    if (e->hd[s].gsize != gsize) {
+        e->hd[s].gsize = gsize;
+        e->desc_dirty = 1;
+    }
+    return 0;
+}
+
+int ssa_engine_set_step_h(ssa_engine *e, int h)
+{
+    if (h < 0 || (h > 0 && (!e->prepared || h > e->max_h))) return -1;
+    if (h != e->step_h) {
+        e->step_h = h;
+        if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+    }
+    return 0;
+}
+
+int ssa_engine_query(ssa_engine *e)
+{
+    hipError_t r = hipStreamQuery(e->st);
+    if (r == hipSuccess) return 1;
+    if (r == hipErrorNotReady) return 0;
+    return -1;
+}
+
+int ssa_engine_status(ssa_engine *e, int s)
+{
+    if (s < 0 || s >= e->count) return -1;
+    int st = 0;
+    HIPCHK(hipMemcpy(&st, (char *)(e->dd + s) + offsetof(ssa_desc, status), sizeof(int), hipMemcpyDeviceToHost));
+    return st;
 }
 
 int ssa_engine_set_rng(ssa_engine *e, int s, unsigned long long seed, unsigned long long call)
@@ -543,8 +612,11 @@ static int matvec_all(ssa_engine *e, int dir, int gen, int in_panel, int out_pan
 {
     for (int lg = 6; lg < NLG; lg++) {
         span sp = dir == 0 ? e->mvf[lg] : e->mva[lg];
+        if (!sp.n) continue;
+        prof_begin(e, dir == 0 ? PH_MV_FWD : PH_MV_ADJ, lg);
         DEVCHK(ssa_dev_matvec(e->prec, lg, dir, gen, in_panel, out_panel, e->dd, it_ptr(e, sp), sp.n, e->arena_a,
                               e->tw, e->st));
+        prof_end(e);
     }
     return 0;
 }
@@ -552,10 +624,18 @@ static int matvec_all(ssa_engine *e, int dir, int gen, int in_panel, int out_pan
 static int cholqr_factor(ssa_engine *e, int panel, int bm)
 {
     span t = panel == SSA_PANEL_Y ? e->tY : e->tZ;
+    prof_begin(e, PH_GRAM, panel);
     DEVCHK(ssa_dev_gram(e->prec, panel, 0, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
+    prof_end(e);
+    prof_begin(e, PH_CHOL, panel);
     DEVCHK(ssa_dev_chol(panel, 1, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    prof_end(e);
+    prof_begin(e, PH_GRAM, panel);
     DEVCHK(ssa_dev_gram(e->prec, panel, 1, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
+    prof_end(e);
+    prof_begin(e, PH_CHOL, panel);
     DEVCHK(ssa_dev_chol(panel, 2, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    prof_end(e);
     return 0;
 }
 
@@ -563,8 +643,10 @@ static int orth(ssa_engine *e, int panel, int bm)
 {
     span t = panel == SSA_PANEL_Y ? e->tY : e->tZ;
     DEVCHK(cholqr_factor(e, panel, bm));
+    prof_begin(e, PH_APPLY, panel);
     DEVCHK(ssa_dev_apply(e->prec, panel, panel, panel == SSA_PANEL_Y ? SSA_SLOT_RINVY : SSA_SLOT_RINVZ, 0, e->dd,
                          it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
+    prof_end(e);
     return 0;
 }
 
@@ -581,11 +663,15 @@ static int launch_decompose(ssa_engine *e, int q, int injected)
     }
     DEVCHK(matvec_all(e, 1, 0, SSA_PANEL_Y, SSA_PANEL_Z));
     DEVCHK(cholqr_factor(e, SSA_PANEL_Z, bm));
+    prof_begin(e, PH_SVD, 0);
     DEVCHK(ssa_dev_svd(e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    prof_end(e);
+    prof_begin(e, PH_UV, 0);
     DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Y, SSA_PANEL_U, SSA_SLOT_WK, 1, e->dd, it_ptr(e, e->tY), e->tY.n,
                          e->arena_a, e->small, e->st));
     DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Z, SSA_PANEL_V, SSA_SLOT_WKS, 1, e->dd, it_ptr(e, e->tZ), e->tZ.n,
                          e->arena_a, e->small, e->st));
+    prof_end(e);
     return 0;
 }
 
@@ -667,7 +753,7 @@ static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weigh
         for (int t = 0; t < ceil_div(d->N, OLAP_TILE); t++) DEVCHK(items_push(e, s, g, t, 0));
     ol.n = (int)(e->hitems_n - ol.off);
     DEVCHK(items_upload(e, e->dyn_base));
-    DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
+    DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, 0, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
                          e->dgroups + 4 * ng, e->arena_a, e->small, e->tw, e->st));
     DEVCHK(ssa_dev_olap(e->prec, weight, e->dd, it_ptr(e, ol), ol.n, e->dgroups, e->arena_a, e->st));
     return 0;
@@ -691,22 +777,28 @@ int ssa_engine_reconstruct(ssa_engine *e, int s, int ch, const int *glist, const
     if (!e->prepared || s < 0 || s >= e->count || n_groups < 1) return -1;
     const ssa_desc *d = &e->hd[s];
     if (ch < 0 || ch >= d->M) return -1;
-    int gmax = e->max_groups > d->k ? e->max_groups : e->kmax[s];
-    if (n_groups > gmax) return -1;
+    int gmax = e->max_groups > e->kmax[s] ? e->max_groups : e->kmax[s];
+    /* the staging area holds gmax groups: without an output buffer the groups
+     * must fit (the caller reads them on the device), otherwise go in chunks */
+    if (!out && n_groups > gmax) return -1;
     DEVCHK(sync_desc(e));
-    int *gd = (int *)malloc(sizeof(int) * 4 * n_groups);
+    int *gd = (int *)malloc(sizeof(int) * 4 * (size_t)gmax);
     if (!gd) return -1;
-    int start = 0;
-    for (int g = 0; g < n_groups; g++) {
-        gd[4 * g] = start; gd[4 * g + 1] = gcount[g]; gd[4 * g + 2] = ch; gd[4 * g + 3] = 0;
-        start += gcount[g];
+    int rc = 0, first = 0;
+    for (int g0 = 0; rc == 0 && g0 < n_groups; g0 += gmax) {
+        int ng = n_groups - g0 < gmax ? n_groups - g0 : gmax;
+        int start = 0;
+        for (int g = 0; g < ng; g++) {
+            gd[4 * g] = start; gd[4 * g + 1] = gcount[g0 + g]; gd[4 * g + 2] = ch; gd[4 * g + 3] = 0;
+            start += gcount[g0 + g];
+        }
+        rc = groups_upload(e, gd, ng, glist + first, start);
+        if (rc == 0) rc = launch_recon(e, s, ng, scale_sigma, weight);
+        if (rc == 0 && out) rc = fetch_out(e, s, (long long)ng * d->N, out + (size_t)g0 * d->N);
+        first += start;
     }
-    int rc = groups_upload(e, gd, n_groups, glist, start);
     free(gd);
-    DEVCHK(rc);
-    DEVCHK(launch_recon(e, s, n_groups, scale_sigma, weight));
-    if (out) DEVCHK(fetch_out(e, s, (long long)n_groups * d->N, out));
-    return 0;
+    return rc;
 }
 
 int ssa_engine_wcorr(ssa_engine *e, int s, double *W)
@@ -753,8 +845,8 @@ int ssa_engine_forecast(ssa_engine *e, int s, const int *group, int n_group, int
     DEVCHK(items_push(e, s, 0, 0, 0));
     one.n = 1;
     DEVCHK(items_upload(e, e->dyn_base));
-    DEVCHK(ssa_dev_forecast(e->prec, h, e->dd, it_ptr(e, one), 1, e->dgroups, e->dgroups + 4, e->arena_a, e->small,
-                            e->st));
+    DEVCHK(ssa_dev_forecast(e->prec, h, 0, e->dd, it_ptr(e, one), 1, e->dgroups, e->dgroups + 4, e->arena_a,
+                            e->small, e->st));
     /* status of the series: 2 = verticality >= 1 */
     int status = 0;
     HIPCHK(hipMemcpyAsync(&status, (char *)(e->dd + s) + offsetof(ssa_desc, status), sizeof(int),
@@ -842,10 +934,22 @@ static int launch_step(ssa_engine *e, int q)
 {
     DEVCHK(launch_spectra(e));
     DEVCHK(launch_decompose(e, q, 0));
-    for (int lg = 6; lg < NLG; lg++)
-        DEVCHK(ssa_dev_recon(e->prec, lg, 1, e->dd, it_ptr(e, e->rec_step[lg]), e->rec_step[lg].n, e->dgroups,
+    for (int lg = 6; lg < NLG; lg++) {
+        if (!e->rec_step[lg].n) continue;
+        prof_begin(e, PH_RECON, lg);
+        DEVCHK(ssa_dev_recon(e->prec, lg, 1, 1, e->dd, it_ptr(e, e->rec_step[lg]), e->rec_step[lg].n, e->dgroups,
                              e->dgroups + 4, e->arena_a, e->small, e->tw, e->st));
+        prof_end(e);
+    }
+    prof_begin(e, PH_OLAP, 0);
     DEVCHK(ssa_dev_olap(e->prec, 0, e->dd, it_ptr(e, e->olap_step), e->olap_step.n, e->dgroups, e->arena_a, e->st));
+    prof_end(e);
+    if (e->step_h > 0) {
+        prof_begin(e, PH_FORECAST, 0);
+        DEVCHK(ssa_dev_forecast(e->prec, e->step_h, 1, e->dd, it_ptr(e, e->ser), e->ser.n, e->dgroups, e->dgroups + 4,
+                                e->arena_a, e->small, e->st));
+        prof_end(e);
+    }
     return 0;
 }
 
@@ -871,8 +975,37 @@ int ssa_engine_step(ssa_engine *e, int q, int use_graph)
     return 0;
 }
 
+int ssa_engine_profile_step(ssa_engine *e, int q, int max_rec, int *phase, int *lgn, float *ms)
+{
+    if (!e->prepared || q < 0 || max_rec < 1) return -1;
+    DEVCHK(sync_desc(e));
+    DEVCHK(prepare_step_groups(e));
+    if (e->prof_cap < max_rec) {
+        for (int i = 0; i < 2 * e->prof_cap; i++) hipEventDestroy(e->prof_ev[i]);
+        free(e->prof_ev); free(e->prof_phase); free(e->prof_lgn);
+        e->prof_ev = (hipEvent_t *)calloc(2 * (size_t)max_rec, sizeof(hipEvent_t));
+        e->prof_phase = (int *)calloc(max_rec, sizeof(int));
+        e->prof_lgn = (int *)calloc(max_rec, sizeof(int));
+        if (!e->prof_ev || !e->prof_phase || !e->prof_lgn) { e->prof_cap = 0; return -1; }
+        for (int i = 0; i < 2 * max_rec; i++) HIPCHK(hipEventCreate(&e->prof_ev[i]));
+        e->prof_cap = max_rec;
+    }
+    e->prof_on = 1;
+    e->prof_n = 0;
+    int rc = launch_step(e, q);
+    e->prof_on = 0;
+    HIPCHK(hipStreamSynchronize(e->st));
+    if (rc) return -1;
+    for (int i = 0; i < e->prof_n; i++) {
+        phase[i] = e->prof_phase[i];
+        lgn[i] = e->prof_lgn[i];
+        HIPCHK(hipEventElapsedTime(&ms[i], e->prof_ev[2 * i], e->prof_ev[2 * i + 1]));
+    }
+    return e->prof_n;
+}
+
 int ssa_engine_step_outputs(ssa_engine *e, int s, double *out)
 {
     if (!e->prepared || s < 0 || s >= e->count) return -1;
-    return fetch_out(e, s, e->hd[s].N, out);
+    return fetch_out(e, s, (long long)e->hd[s].N + e->step_h, out);
 }

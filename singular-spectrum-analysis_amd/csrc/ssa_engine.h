@@ -80,7 +80,21 @@ int ssa_engine_matvec(ssa_engine *e, int s, int dir, const double *in, double *o
  * the group {0..k-1} into the series' output slot; captured into a HIP
  * graph on first use when use_graph is set */
 int ssa_engine_step(ssa_engine *e, int q, int use_graph);
-int ssa_engine_step_outputs(ssa_engine *e, int s, double *out);   /* N values */
+int ssa_engine_step_outputs(ssa_engine *e, int s, double *out);   /* N (+ step h) values */
+/* group {0..gsize-1} reconstructed (and forecast) by the batched step for series s */
+int ssa_engine_set_step(ssa_engine *e, int s, int gsize);
+/* forecast horizon of the batched step (0: none; <= max_h of prepare) */
+int ssa_engine_set_step_h(ssa_engine *e, int h);
+/* 1 when all work queued on the engine stream has finished, 0 if not, -1 on error */
+int ssa_engine_query(ssa_engine *e);
+/* device status word of series s (2: forecast verticality >= 1) */
+int ssa_engine_status(ssa_engine *e, int s);
+/* one eager step with a HIP event pair around every kernel launch; returns
+ * the number of records (phase id, FFT-length log2 or panel, milliseconds).
+ * Phases: 0 spectra, 1 matvec H, 2 matvec H^T, 3 Gram, 4 Cholesky,
+ * 5 apply R^-1, 6 small SVD, 7 U/V products, 8 reconstruction, 9 overlap-add,
+ * 10 forecast */
+int ssa_engine_profile_step(ssa_engine *e, int q, int max_rec, int *phase, int *lgn, float *ms);
 int ssa_engine_sync(ssa_engine *e);
 void *ssa_engine_stream(ssa_engine *e);
 /* bytes of device memory held */

@@ -599,7 +599,8 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_recon(const ssa_d
                                                               const double *__restrict__ small,
                                                               const int4 *__restrict__ gdesc,
                                                               const int *__restrict__ glist,
-                                                              const C2<T> *__restrict__ tw, int scale_sigma)
+                                                              const C2<T> *__restrict__ tw, int scale_sigma,
+                                                              int limit_gsize)
 {
     using G = FftGeom<T, LOGN>;
     constexpr int E = G::E, NT = G::NT;
@@ -618,7 +619,7 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_recon(const ssa_d
     for (int e = 0; e < E; e++) acc[e] = C2<T>{0, 0};
     for (int g = 0; g < gd.y; g++) {
         const int idx = glist[gd.x + g];
-        if (idx < 0 || idx >= d.k) continue;
+        if (idx < 0 || idx >= d.k || (limit_gsize && idx >= d.gsize)) continue;
         const T sc = (T)(scale_sigma ? sqrt(fabs(sig[idx])) : 1.0);
         const T *u = bulk + d.off_U + (long long)idx * d.ML + urow0;
         const T *v = bulk + d.off_V + (long long)idx * d.K + vrow0;
@@ -705,7 +706,7 @@ __global__ __launch_bounds__(256) void k_wnorm(const ssa_desc *D, const ssa_item
 // reconstruction (at off_out, gid 0); forecasts written after it.
 template <typename T>
 __global__ __launch_bounds__(256) void k_forecast(ssa_desc *D, const ssa_item *items, const int4 *gdesc,
-                                                  const int *glist, T *bulk, double *small, int h)
+                                                  const int *glist, T *bulk, double *small, int h, int limit_gsize)
 {
     __shared__ double red[32];
     __shared__ double pis[SSA_BMAX];
@@ -714,7 +715,9 @@ __global__ __launch_bounds__(256) void k_forecast(ssa_desc *D, const ssa_item *i
     ssa_desc *dp = D + it.s;
     const ssa_desc d = *dp;
     const int4 gd = gdesc[0];
-    const int L = d.L, n = gd.y;
+    const int L = d.L;
+    int n = gd.y < SSA_BMAX ? gd.y : SSA_BMAX;
+    if (limit_gsize && n > d.gsize) n = d.gsize;
     const T *U = bulk + d.off_U;
     if (threadIdx.x == 0) {
         for (int g = 0; g < n && g < SSA_BMAX; g++) ids[g] = glist[gd.x + g];
@@ -908,8 +911,9 @@ int ssa_dev_svd(const ssa_desc *D, const ssa_item *items, int n_items, double *s
     return (int)hipGetLastError();
 }
 
-int ssa_dev_recon(int prec, int lgn, int scale_sigma, const ssa_desc *D, const ssa_item *items, int n_items,
-                  const int *gdesc, const int *glist, void *bulk, const double *small, const void *tw, void *stream)
+int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, const ssa_desc *D, const ssa_item *items,
+                  int n_items, const int *gdesc, const int *glist, void *bulk, const double *small, const void *tw,
+                  void *stream)
 {
     if (n_items <= 0) return 0;
     if (!lgn_ok(prec, lgn)) return -2;
@@ -919,11 +923,11 @@ int ssa_dev_recon(int prec, int lgn, int scale_sigma, const ssa_desc *D, const s
         if (prec == SSA_PREC_F64)                                                                             \
             hipLaunchKernelGGL((k_recon<double, L>), dim3(n_items),                           \
                                dim3(FftGeom<double, L>::NT), 0, st, D, items, (double *)bulk, \
-                               small, (const int4 *)gdesc, glist, (const C2<double> *)tw, scale_sigma);       \
+                               small, (const int4 *)gdesc, glist, (const C2<double> *)tw, scale_sigma, limit_gsize);       \
         else                                                                                                  \
             hipLaunchKernelGGL((k_recon<float, L>), dim3(n_items), dim3(FftGeom<float, L>::NT), 0, st, D,     \
                                items, (float *)bulk, small, (const int4 *)gdesc, glist,                       \
-                               (const C2<float> *)tw, scale_sigma);                                           \
+                               (const C2<float> *)tw, scale_sigma, limit_gsize);                              \
         break;
     switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
 #undef CASE
@@ -953,16 +957,16 @@ int ssa_dev_wcorr(int prec, const ssa_desc *D, const ssa_item *tile_items, int n
     return (int)hipGetLastError();
 }
 
-int ssa_dev_forecast(int prec, int h, const ssa_desc *D, const ssa_item *items, int n_items, const int *gdesc,
-                     const int *glist, void *bulk, double *small, void *stream)
+int ssa_dev_forecast(int prec, int h, int limit_gsize, const ssa_desc *D, const ssa_item *items, int n_items,
+                     const int *gdesc, const int *glist, void *bulk, double *small, void *stream)
 {
     if (n_items <= 0) return 0;
     if (prec == SSA_PREC_F64)
         hipLaunchKernelGGL(k_forecast<double>, dim3(n_items), dim3(256), 0, (hipStream_t)stream,
-                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (double *)bulk, small, h);
+                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (double *)bulk, small, h, limit_gsize);
     else
         hipLaunchKernelGGL(k_forecast<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream,
-                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (float *)bulk, small, h);
+                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (float *)bulk, small, h, limit_gsize);
     return (int)hipGetLastError();
 }
 

@@ -756,10 +756,10 @@ struct ssa_amd_batch {
     int count, q;
 };
 
-ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const int *k, int p, int q,
-                                    int precision, unsigned long long seed)
+ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, const int *k, const int *g, int h,
+                                       int p, int q, int precision, unsigned long long seed)
 {
-    if (count < 1 || !N || !L || !k || q < 0 || (precision != 0 && precision != 1)) return NULL;
+    if (count < 1 || !N || !L || !k || q < 0 || h < 0 || (precision != 0 && precision != 1)) return NULL;
     if (p <= 0) p = 8;
     int *bm = (int *)malloc(sizeof(int) * count), *km = (int *)malloc(sizeof(int) * count);
     if (!bm || !km) { free(bm); free(km); return NULL; }
@@ -774,16 +774,30 @@ ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const
     ssa_amd_batch *b = (ssa_amd_batch *)calloc(1, sizeof(ssa_amd_batch));
     int err = 0;
     if (b) b->eng = ssa_engine_create(precision, count, N, L, NULL, bm, km, q, &err);
-    free(bm);
-    free(km);
-    if (!b || !b->eng) { free(b); return NULL; }
+    if (!b || !b->eng) { free(bm); free(km); free(b); return NULL; }
     b->count = count;
     b->q = q;
-    if (ssa_engine_prepare(b->eng, 1, 0)) { ssa_amd_batch_free(b); return NULL; }
-    for (int i = 0; i < count; i++)
-        ssa_engine_set_rng(b->eng, i, seed ^ (0x9E3779B97F4A7C15ULL * (unsigned long long)(i + 1)), 0);
+    int rc = ssa_engine_prepare(b->eng, 1, h);
+    for (int i = 0; rc == 0 && i < count; i++) {
+        rc = ssa_engine_set_rng(b->eng, i, seed ^ (0x9E3779B97F4A7C15ULL * (unsigned long long)(i + 1)), 0);
+        if (rc == 0 && g) rc = ssa_engine_set_step(b->eng, i, imin(imax(g[i], 1), km[i]));
+    }
+    if (rc == 0 && h > 0) rc = ssa_engine_set_step_h(b->eng, h);
+    free(bm);
+    free(km);
+    if (rc) { ssa_amd_batch_free(b); return NULL; }
     return b;
 }
+
+ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const int *k, int p, int q,
+                                    int precision, unsigned long long seed)
+{
+    return ssa_amd_batch_create_ex(count, N, L, k, NULL, 0, p, q, precision, seed);
+}
+
+int ssa_amd_batch_query(ssa_amd_batch *b) { return b ? ssa_engine_query(b->eng) : -1; }
+
+int ssa_amd_batch_status(ssa_amd_batch *b, int i) { return b ? ssa_engine_status(b->eng, i) : -1; }
 
 int ssa_amd_batch_set_signals_f32(ssa_amd_batch *b, const float *cat, const long long *offsets)
 {
@@ -812,6 +826,21 @@ int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out)
 }
 
 long long ssa_amd_batch_bytes(const ssa_amd_batch *b) { return b ? ssa_engine_bytes(b->eng) : 0; }
+
+int ssa_amd_batch_profile(ssa_amd_batch *b, int max_rec, int *phase, int *lgn, float *ms)
+{
+    if (!b || !phase || !lgn || !ms) return -1;
+    return ssa_engine_profile_step(b->eng, b->q, max_rec, phase, lgn, ms);
+}
+
+int ssa_amd_batch_plan(const ssa_amd_batch *b, int i, int *out8)
+{
+    if (!b || i < 0 || i >= b->count || !out8) return -1;
+    const ssa_desc *d = ssa_engine_desc(b->eng, i);
+    out8[0] = d->n; out8[1] = d->CL; out8[2] = d->CK; out8[3] = d->AL; out8[4] = d->SK;
+    out8[5] = d->b; out8[6] = d->k; out8[7] = d->lgn;
+    return 0;
+}
 
 void ssa_amd_batch_free(ssa_amd_batch *b)
 {

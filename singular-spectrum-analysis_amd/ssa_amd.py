@@ -91,6 +91,11 @@ SIGNATURES = [
     ("ssa_amd_batch_output", _I, [_VP, _I, c_double_p]),
     ("ssa_amd_batch_bytes", ctypes.c_longlong, [_VP]),
     ("ssa_amd_batch_free", None, [_VP]),
+    ("ssa_amd_batch_profile", _I, [_VP, _I, c_int_p, c_int_p, c_float_p]),
+    ("ssa_amd_batch_create_ex", _VP, [_I, c_int_p, c_int_p, c_int_p, c_int_p, _I, _I, _I, _I, _U64]),
+    ("ssa_amd_batch_query", _I, [_VP]),
+    ("ssa_amd_batch_status", _I, [_VP, _I]),
+    ("ssa_amd_batch_plan", _I, [_VP, _I, c_int_p]),
 ]
 
 
@@ -384,14 +389,18 @@ class MSSA:
 class Batch:
     """Many uneven series in one device engine (no counterpart in the reference)."""
 
-    def __init__(self, xs, Ls, ks, p=10, q=2, precision=1, seed=4):
+    def __init__(self, xs, Ls, ks, p=10, q=2, precision=1, seed=4, groups=None, h=0):
+        """groups[i]: reconstruct (and forecast) components 0..groups[i]-1
+        of series i (default all k); h > 0 appends an h-step LRF forecast."""
         self._l = lib()
         self.count = len(xs)
+        self.horizon = int(h)
         self.N = np.ascontiguousarray([len(x) for x in xs], dtype=np.int32)
         self.L = np.ascontiguousarray(Ls, dtype=np.int32)
         self.k = np.ascontiguousarray(ks, dtype=np.int32)
-        self._h = self._l.ssa_amd_batch_create(self.count, _ip(self.N), _ip(self.L), _ip(self.k), p, q, precision,
-                                              seed)
+        g = None if groups is None else np.ascontiguousarray(groups, dtype=np.int32)
+        self._h = self._l.ssa_amd_batch_create_ex(self.count, _ip(self.N), _ip(self.L), _ip(self.k),
+                                                 None if g is None else _ip(g), self.horizon, p, q, precision, seed)
         if not self._h:
             raise RuntimeError("batch creation failed")
         cat = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float32) for x in xs]))
@@ -421,11 +430,39 @@ class Batch:
             raise RuntimeError("batch sigma failed")
         return out[:n]
 
+    def query(self):
+        r = self._l.ssa_amd_batch_query(self._h)
+        if r < 0:
+            raise RuntimeError("batch query failed")
+        return bool(r)
+
+    def status(self, i):
+        return int(self._l.ssa_amd_batch_status(self._h, i))
+
     def output(self, i):
-        out = np.zeros(int(self.N[i]))
+        out = np.zeros(int(self.N[i]) + self.horizon)
         if self._l.ssa_amd_batch_output(self._h, i, _dp(out)) != 0:
             raise RuntimeError("batch output failed")
         return out
 
     def bytes(self):
         return int(self._l.ssa_amd_batch_bytes(self._h))
+
+    PHASES = ("spectra", "matvec_H", "matvec_HT", "gram", "chol", "apply", "svd", "uv", "recon", "olap", "forecast")
+
+    def profile(self, max_rec=4096):
+        """One eager step with HIP events around every launch:
+        list of (phase name, log2 n or panel id, ms)."""
+        ph = np.zeros(max_rec, dtype=np.int32)
+        lg = np.zeros(max_rec, dtype=np.int32)
+        ms = np.zeros(max_rec, dtype=np.float32)
+        n = self._l.ssa_amd_batch_profile(self._h, max_rec, _ip(ph), _ip(lg), ms.ctypes.data_as(c_float_p))
+        if n < 0:
+            raise RuntimeError("batch profile failed")
+        return [(self.PHASES[ph[i]], int(lg[i]), float(ms[i])) for i in range(n)]
+
+    def plan(self, i):
+        out = np.zeros(8, dtype=np.int32)
+        if self._l.ssa_amd_batch_plan(self._h, i, _ip(out)) != 0:
+            raise RuntimeError("batch plan failed")
+        return dict(zip(("n", "CL", "CK", "AL", "SK", "b", "k", "lgn"), (int(v) for v in out)))
