@@ -85,11 +85,11 @@ int ssa_dev_matvec(int prec, int lgn, int dir, int gen_omega, int in_panel, int 
                    void *bulk, const void *tw, void *stream);
 int ssa_dev_gram(int prec, int panel, int use_m, int bmax, const ssa_desc *D, const ssa_item *items, int n_items,
                  void *bulk, double *small, void *stream);
-int ssa_dev_chol(int panel, int pass, const ssa_desc *D, const ssa_item *items, int n_items,
+int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_item *items, int n_items,
                  double *small, void *stream);
-int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, const ssa_desc *D,
+int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, int bmax, const ssa_desc *D,
                   const ssa_item *items, int n_items, void *bulk, double *small, void *stream);
-int ssa_dev_svd(const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream);
+int ssa_dev_svd(int bmax, const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream);
 /* reconstruction: items (s, gid, row chunk o, col chunk sk); groups: gdesc[gid] = {start, count},
  * glist = component indices; scale_sigma 1: sum_g sigma_g u_g (*) v_g, 0: u (*) v */
 int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, const ssa_desc *D, const ssa_item *items,

@@ -69,6 +69,7 @@ struct ssa_engine {
     /* per-launch profiler (HIP events on the engine stream) */
     int prof_on, prof_n, prof_cap;
     int step_h;                /* forecast horizon of the batched step (0: none) */
+    int cholqr2_all;           /* CholeskyQR2 for every orthonormalisation (default: last one only) */
     hipEvent_t *prof_ev;
     int *prof_phase, *prof_lgn;
 };
@@ -337,6 +338,10 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
     if (hipMalloc((void **)&e->dd, sizeof(ssa_desc) * count) != hipSuccess) goto fail;
     e->lists_dirty = 1;
     e->desc_dirty = 1;
+    {
+        const char *v = getenv("SSA_AMD_CHOLQR2_ALL");
+        e->cholqr2_all = (v && *v && *v != '0') ? 1 : 0;
+    }
     *err = 0;
     return e;
 fail:
@@ -628,24 +633,40 @@ static int cholqr_factor(ssa_engine *e, int panel, int bm)
     DEVCHK(ssa_dev_gram(e->prec, panel, 0, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
     prof_end(e);
     prof_begin(e, PH_CHOL, panel);
-    DEVCHK(ssa_dev_chol(panel, 1, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    DEVCHK(ssa_dev_chol(panel, 1, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
     prof_end(e);
     prof_begin(e, PH_GRAM, panel);
     DEVCHK(ssa_dev_gram(e->prec, panel, 1, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
     prof_end(e);
     prof_begin(e, PH_CHOL, panel);
-    DEVCHK(ssa_dev_chol(panel, 2, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    DEVCHK(ssa_dev_chol(panel, 2, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
     prof_end(e);
     return 0;
 }
 
-static int orth(ssa_engine *e, int panel, int bm)
+/* Orthonormalise a panel in place.  The last orthonormalisation of Y (the Q
+ * that forms B = H^T Q and U = Q W) is CholeskyQR2; the ones inside the
+ * power iterations only have to keep the span well conditioned, and a
+ * single CholeskyQR in FP64 leaves an orthogonality defect of about
+ * eps64 * cond^2 there, which the next orthonormalisation absorbs. */
+static int orth(ssa_engine *e, int panel, int bm, int two_pass)
 {
     span t = panel == SSA_PANEL_Y ? e->tY : e->tZ;
-    DEVCHK(cholqr_factor(e, panel, bm));
+    int slot;
+    if (two_pass) {
+        DEVCHK(cholqr_factor(e, panel, bm));
+        slot = panel == SSA_PANEL_Y ? SSA_SLOT_RINVY : SSA_SLOT_RINVZ;
+    } else {
+        prof_begin(e, PH_GRAM, panel);
+        DEVCHK(ssa_dev_gram(e->prec, panel, 0, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
+        prof_end(e);
+        prof_begin(e, PH_CHOL, panel);
+        DEVCHK(ssa_dev_chol(panel, 1, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+        prof_end(e);
+        slot = SSA_SLOT_R1INV;
+    }
     prof_begin(e, PH_APPLY, panel);
-    DEVCHK(ssa_dev_apply(e->prec, panel, panel, panel == SSA_PANEL_Y ? SSA_SLOT_RINVY : SSA_SLOT_RINVZ, 0, e->dd,
-                         it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
+    DEVCHK(ssa_dev_apply(e->prec, panel, panel, slot, 0, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
     prof_end(e);
     return 0;
 }
@@ -653,23 +674,24 @@ static int orth(ssa_engine *e, int panel, int bm)
 static int launch_decompose(ssa_engine *e, int q, int injected)
 {
     int bm = bmax_all(e);
+    const int all2 = e->cholqr2_all;
     DEVCHK(matvec_all(e, 0, !injected, SSA_PANEL_Z, SSA_PANEL_Y));
-    DEVCHK(orth(e, SSA_PANEL_Y, bm));
+    DEVCHK(orth(e, SSA_PANEL_Y, bm, all2 || q == 0));
     for (int it = 0; it < q; it++) {
         DEVCHK(matvec_all(e, 1, 0, SSA_PANEL_Y, SSA_PANEL_Z));
-        DEVCHK(orth(e, SSA_PANEL_Z, bm));
+        DEVCHK(orth(e, SSA_PANEL_Z, bm, all2));
         DEVCHK(matvec_all(e, 0, 0, SSA_PANEL_Z, SSA_PANEL_Y));
-        DEVCHK(orth(e, SSA_PANEL_Y, bm));
+        DEVCHK(orth(e, SSA_PANEL_Y, bm, all2 || it == q - 1));
     }
     DEVCHK(matvec_all(e, 1, 0, SSA_PANEL_Y, SSA_PANEL_Z));
     DEVCHK(cholqr_factor(e, SSA_PANEL_Z, bm));
     prof_begin(e, PH_SVD, 0);
-    DEVCHK(ssa_dev_svd(e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    DEVCHK(ssa_dev_svd(bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
     prof_end(e);
     prof_begin(e, PH_UV, 0);
-    DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Y, SSA_PANEL_U, SSA_SLOT_WK, 1, e->dd, it_ptr(e, e->tY), e->tY.n,
+    DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Y, SSA_PANEL_U, SSA_SLOT_WK, 1, bm, e->dd, it_ptr(e, e->tY), e->tY.n,
                          e->arena_a, e->small, e->st));
-    DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Z, SSA_PANEL_V, SSA_SLOT_WKS, 1, e->dd, it_ptr(e, e->tZ), e->tZ.n,
+    DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Z, SSA_PANEL_V, SSA_SLOT_WKS, 1, bm, e->dd, it_ptr(e, e->tZ), e->tZ.n,
                          e->arena_a, e->small, e->st));
     prof_end(e);
     return 0;

@@ -241,76 +241,138 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_matvec(const ssa_
 }
 
 // ---------------------------------------------------------------------------
-// partial Gram: item (s, tile) covers rows [tile*512, +512) of the panel.
-// G_part = T^T T with T = P (use_m = 0) or T = P * M (M = R1^-1 slot), FP64
-// accumulation; use_m = 0 also emits the column sums of P.
+// FP64 MFMA 16x16x4 (v_mfma_f64_16x16x4f64).  Operand layout: lane l holds
+// A[l%16][l/16] of the 16x4 A and B[l/16][l%16] of the 4x16 B; the 4
+// accumulator values of lane l are D[l/16 + 4v][l%16], v = 0..3.
 // ---------------------------------------------------------------------------
-template <typename T, int BB>
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4v mfma64(double a, double b, d4v c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int RT = 64;          // rows of an LDS sub-tile (4 waves x 16)
+constexpr int RP = RT + 1;      // padded row count of a staged column
+
+// Stage rows [r0, r0 + RT) of panel P (b columns) into Pt[c*RP + r] as FP64,
+// zero-filled beyond the panel and up to b16 columns.
+template <typename T>
+__device__ __forceinline__ void stage_tile(const Panel<T> &P, int b, int b16, int r0, int r_end, double *Pt)
+{
+    for (int idx = threadIdx.x; idx < RT * b16; idx += 256) {
+        const int c = idx / RT, r = idx - c * RT;
+        Pt[c * RP + r] = (c < b && r0 + r < r_end) ? (double)P.p[(long long)c * P.ld + r0 + r] : 0.0;
+    }
+}
+
+// Stage the b x nc column-major small matrix M (ld SSA_BMAX) into Ms[k*MP + c].
+__device__ __forceinline__ void stage_small(const double *M, int b, int nc, int b16, int nc16, double *Ms, int MP)
+{
+    for (int idx = threadIdx.x; idx < b16 * nc16; idx += 256) {
+        const int c = idx / b16, k = idx - c * b16;
+        Ms[k * MP + c] = (k < b && c < nc) ? M[(long long)c * SSA_BMAX + k] : 0.0;
+    }
+}
+
+// Rows 16w..16w+15 of the staged tile times Ms (b x nc): D for column block cb
+// is T[16w + j][cb*16 + i] for lane (i, j) of the accumulator layout.
+__device__ __forceinline__ d4v tile_times_small(const double *Pt, const double *Ms, int MP, int b16, int cb, int w,
+                                                bool upper)
+{
+    const int l = threadIdx.x & 63;
+    d4v acc = {0, 0, 0, 0};
+    // an upper-triangular M has no rows below the column block's last column
+    const int kend = upper ? min(b16, (cb + 1) * 16) : b16;
+    for (int k0 = 0; k0 < kend; k0 += 4) {
+        const double a = Ms[(k0 + (l >> 4)) * MP + cb * 16 + (l & 15)];
+        const double bb = Pt[(k0 + (l >> 4)) * RP + 16 * w + (l & 15)];
+        acc = mfma64(a, bb, acc);
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// partial Gram: item (s, tile) covers rows [tile*512, +512) of the panel.
+// G_part = T^T T with T = P (use_m = 0) or T = P * M (M = R1^-1 slot); FP64
+// MFMA with FP64 accumulation whatever the panel type.  use_m = 0 also emits
+// the column sums of P.  Only the upper block triangle is multiplied; the
+// partial is written in full (mirrored).
+// ---------------------------------------------------------------------------
+template <typename T>
 __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items,
                                               const T *__restrict__ bulk, double *small, int which, int use_m)
 {
     extern __shared__ double sm[];
-    double *A = sm;                    // SUB x LDP
-    double *Tm = sm + SUB * LDP;       // SUB x LDP (transformed rows)
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const Panel<T> P = panel_of<T>(const_cast<T *>(bulk), d, which);
-    const int b = P.cols;
+    const int b = P.cols, b16 = (b + 15) & ~15, nb = b16 >> 4;
+    const int MP = b16 + 1;
+    double *Pt = sm;                      // b16 x RP
+    double *Ms = sm + b16 * RP;           // b16 x MP (use_m)
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int r_begin = it.x * GRAM_ROWS_PER_WG;
     const int r_end = min(P.rows, r_begin + GRAM_ROWS_PER_WG);
-    const double *M = slot(small, d, SSA_SLOT_R1INV);
-    const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;   // 16 x 16 grid of BB x BB blocks
-    double acc[BB][BB];
+    if (use_m) stage_small(slot(small, d, SSA_SLOT_R1INV), b, b, b16, b16, Ms, MP);
+    // upper-triangle blocks (I <= J) dealt round-robin to the 4 waves
+    const int nblk = nb * (nb + 1) / 2;
+    // block m of this wave is the (4m + w)-th upper block in row-major order
+    d4v acc[4];
+    int bi[4], bj[4];
 #pragma unroll
-    for (int i = 0; i < BB; i++)
-#pragma unroll
-        for (int j = 0; j < BB; j++) acc[i][j] = 0;
+    for (int m = 0; m < 4; m++) {
+        acc[m] = d4v{0, 0, 0, 0};
+        int q = 4 * m + w, I = 0;
+        while (I < nb && q >= nb - I) { q -= nb - I; I++; }
+        bi[m] = I;                 /* I == nb: no block */
+        bj[m] = I + q;
+    }
+    (void)nblk;
     double csum = 0;
-    for (int r0 = r_begin; r0 < r_end; r0 += SUB) {
-        const int nr = min(SUB, r_end - r0);
+    for (int r0 = r_begin; r0 < r_end; r0 += RT) {
         __syncthreads();
-        for (int idx = threadIdx.x; idx < SUB * b; idx += 256) {
-            const int c = idx / SUB, r = idx - c * SUB;
-            A[r * LDP + c] = (r < nr) ? (double)P.p[(long long)c * P.ld + r0 + r] : 0.0;
-        }
+        stage_tile(P, b, b16, r0, r_end, Pt);
         __syncthreads();
-        const double *S = A;
         if (use_m) {
-            // Tm[r][j] = sum_{i<=j} A[r][i] M[i][j]   (M upper triangular, col-major b x b)
-            for (int idx = threadIdx.x; idx < SUB * b; idx += 256) {
-                const int j = idx / SUB, r = idx - j * SUB;
-                double s = 0;
-                const double *mj = M + (long long)j * SSA_BMAX;
-                for (int i = 0; i <= j; i++) s += A[r * LDP + i] * mj[i];
-                Tm[r * LDP + j] = s;
-            }
+            // T = P M for this sub-tile, in registers, then written over Pt
+            d4v t[5];
+#pragma unroll
+            for (int cb = 0; cb < 5; cb++)
+                if (cb < nb) t[cb] = tile_times_small(Pt, Ms, MP, b16, cb, w, true);
             __syncthreads();
-            S = Tm;
+#pragma unroll
+            for (int cb = 0; cb < 5; cb++)
+                if (cb < nb) {
+#pragma unroll
+                    for (int v = 0; v < 4; v++) Pt[(cb * 16 + (l >> 4) + 4 * v) * RP + 16 * w + (l & 15)] = t[cb][v];
+                }
+            __syncthreads();
         } else if (threadIdx.x < b) {
-            for (int r = 0; r < nr; r++) csum += A[r * LDP + threadIdx.x];
+            for (int r = 0; r < RT; r++) csum += Pt[threadIdx.x * RP + r];
         }
-        for (int r = 0; r < SUB; r++) {
-            double ai[BB], aj[BB];
 #pragma unroll
-            for (int u = 0; u < BB; u++) {
-                const int ci = ti * BB + u, cj = tj * BB + u;
-                ai[u] = (ci < b) ? S[r * LDP + ci] : 0.0;
-                aj[u] = (cj < b) ? S[r * LDP + cj] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < BB; u++)
-#pragma unroll
-                for (int v = 0; v < BB; v++) acc[u][v] += ai[u] * aj[v];
+        for (int m = 0; m < 4; m++) {
+            if (bi[m] >= nb) continue;
+            const double *ca = Pt + (bi[m] * 16 + (l & 15)) * RP + (l >> 4);
+            const double *cbp = Pt + (bj[m] * 16 + (l & 15)) * RP + (l >> 4);
+#pragma unroll 4
+            for (int k0 = 0; k0 < RT; k0 += 4) acc[m] = mfma64(ca[k0], cbp[k0], acc[m]);
         }
     }
     double *out = small + d.off_part + (long long)it.x * (b * b + b);
 #pragma unroll
-    for (int u = 0; u < BB; u++)
+    for (int m = 0; m < 4; m++) {
+        if (bi[m] >= nb) continue;
 #pragma unroll
-        for (int v = 0; v < BB; v++) {
-            const int ci = ti * BB + u, cj = tj * BB + v;
-            if (ci < b && cj < b) out[ci * b + cj] = acc[u][v];
+        for (int v = 0; v < 4; v++) {
+            const int i = bi[m] * 16 + (l >> 4) + 4 * v, j = bj[m] * 16 + (l & 15);
+            if (i < b && j < b) {
+                out[i * b + j] = acc[m][v];
+                out[j * b + i] = acc[m][v];
+            }
         }
+    }
     if (!use_m && threadIdx.x < b) out[b * b + threadIdx.x] = csum;
 }
 
@@ -326,14 +388,15 @@ __global__ __launch_bounds__(256) void k_chol(const ssa_desc *D, const ssa_item 
                                               int pass, double tol)
 {
     extern __shared__ double sm[];
+    const int b = D[items[blockIdx.x].s].b;
+    const int LDP = b + 1;             // padded row length
     double *A = sm;                    // b x LDP, row-major (upper part used)
-    double *X = sm + SSA_BMAX * LDP;   // b x LDP
+    double *X = sm + b * LDP;          // b x LDP
     __shared__ double diag0[SSA_BMAX];
     __shared__ int dead[SSA_BMAX];
     __shared__ double piv;
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
-    const int b = d.b;
     const int tiles = which == SSA_PANEL_Y ? d.tilesY : d.tilesZ;
     const double *part = small + d.off_part;
     const int stride = b * b + b;
@@ -432,39 +495,42 @@ __global__ __launch_bounds__(256) void k_chol(const ssa_desc *D, const ssa_item 
 }
 
 // ---------------------------------------------------------------------------
-// out (rows x ncols) = in (rows x b) * M (b x ncols, col-major, ld BMAX); in place allowed.
-// item (s, tile of 512 rows).  ncols_k: 0 -> ncols = b, 1 -> ncols = k.
-// scale_slot >= 0: M is scaled column-wise (unused; reserved).
+// out (rows x nc) = in (rows x b) * M (b x nc, col-major ld BMAX), FP64 MFMA
+// with FP64 accumulation, T-typed panels; in place allowed (each sub-tile is
+// staged in LDS before its rows are written).  item (s, tile of 512 rows);
+// ncols_k: 0 -> nc = b, 1 -> nc = k.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_apply(const ssa_desc *D, const ssa_item *items, T *bulk, const double *small,
                                                int src, int dst, int mslot, int ncols_k)
 {
     extern __shared__ double sm[];
-    double *A = sm;   // SUB x LDP
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const Panel<T> Pi = panel_of<T>(bulk, d, src);
     const Panel<T> Po = panel_of<T>(bulk, d, dst);
     const int b = d.b, nc = ncols_k ? d.k : d.b;
-    const double *M = small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX;
+    const int b16 = (b + 15) & ~15, nc16 = (nc + 15) & ~15, ncb = nc16 >> 4;
+    const int MP = nc16 + 1;
+    double *Pt = sm;                  // b16 x RP
+    double *Ms = sm + b16 * RP;       // b16 x MP
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    stage_small(small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX, b, nc, b16, nc16, Ms, MP);
     const int r_begin = it.x * GRAM_ROWS_PER_WG;
     const int r_end = min(Pi.rows, r_begin + GRAM_ROWS_PER_WG);
-    const int r = threadIdx.x & 63, cg = threadIdx.x >> 6;
-    for (int r0 = r_begin; r0 < r_end; r0 += SUB) {
-        const int nr = min(SUB, r_end - r0);
+    for (int r0 = r_begin; r0 < r_end; r0 += RT) {
         __syncthreads();
-        for (int idx = threadIdx.x; idx < SUB * b; idx += 256) {
-            const int c = idx / SUB, rr = idx - c * SUB;
-            A[rr * LDP + c] = (rr < nr) ? (double)Pi.p[(long long)c * Pi.ld + r0 + rr] : 0.0;
-        }
+        stage_tile(Pi, b, b16, r0, r_end, Pt);
         __syncthreads();
-        if (r < nr) {
-            for (int j = cg; j < nc; j += 4) {
-                const double *mj = M + (long long)j * SSA_BMAX;
-                double s = 0;
-                for (int i = 0; i < b; i++) s += A[r * LDP + i] * mj[i];
-                Po.p[(long long)j * Po.ld + r0 + r] = (T)s;
+        const int row = r0 + 16 * w + (l & 15);
+        for (int cb = 0; cb < ncb; cb++) {
+            const d4v t = tile_times_small(Pt, Ms, MP, b16, cb, w, !ncols_k);
+            if (row < r_end) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int c = cb * 16 + (l >> 4) + 4 * v;
+                    if (c < nc) Po.p[(long long)c * Po.ld + row] = (T)t[v];
+                }
             }
         }
     }
@@ -479,8 +545,10 @@ __global__ __launch_bounds__(256) void k_apply(const ssa_desc *D, const ssa_item
 __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *items, double *small)
 {
     extern __shared__ double sm[];
-    double *A = sm;                        // column j at A + j*LDP (rows 0..b-1)
-    double *W = sm + (SSA_BMAX + 1) * LDP; // same layout
+    const int bq = D[items[blockIdx.x].s].b;
+    const int LDP = bq + 1;                // padded column length
+    double *A = sm;                        // column j at A + j*LDP (rows 0..b-1), bp columns
+    double *W = sm + ((bq + 1) & ~1) * LDP; // same layout
     __shared__ int perm[SSA_BMAX + 1];
     __shared__ double S[SSA_BMAX + 1];
     __shared__ int rotated;
@@ -847,34 +915,35 @@ int ssa_dev_gram(int prec, int panel, int use_m, int bmax, const ssa_desc *D, co
 {
     if (n_items <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    const size_t lds = 2 * SUB * LDP * sizeof(double);
-    // 16 x 16 threads of BB x BB blocks: BB = 4 covers b <= 64, BB = 8 covers b <= 128
-    if (prec == SSA_PREC_F64) {
-        if (bmax <= 64)
-            hipLaunchKernelGGL((k_gram<double, 4>), dim3(n_items), dim3(256), lds, st, D, items,
-                               (const double *)bulk, small, panel, use_m);
-        else
-            hipLaunchKernelGGL((k_gram<double, 8>), dim3(n_items), dim3(256), lds, st, D, items,
-                               (const double *)bulk, small, panel, use_m);
-    } else {
-        if (bmax <= 64)
-            hipLaunchKernelGGL((k_gram<float, 4>), dim3(n_items), dim3(256), lds, st, D, items,
-                               (const float *)bulk, small, panel, use_m);
-        else
-            hipLaunchKernelGGL((k_gram<float, 8>), dim3(n_items), dim3(256), lds, st, D, items,
-                               (const float *)bulk, small, panel, use_m);
+    const int b16 = (bmax + 15) & ~15;
+    const size_t lds = (size_t)(b16 * RP + (use_m ? b16 * (b16 + 1) : 0)) * sizeof(double);
+    static int attr = 0;
+    if (!attr) {
+        const int mx = (int)((size_t)(96 * RP + 96 * 97) * sizeof(double));
+        hipError_t e = hipFuncSetAttribute((const void *)k_gram<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_gram<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+        if (e != hipSuccess) return (int)e;
+        attr = 1;
     }
+    if (prec == SSA_PREC_F64)
+        hipLaunchKernelGGL(k_gram<double>, dim3(n_items), dim3(256), lds, st, D, items, (const double *)bulk, small,
+                           panel, use_m);
+    else
+        hipLaunchKernelGGL(k_gram<float>, dim3(n_items), dim3(256), lds, st, D, items, (const float *)bulk, small,
+                           panel, use_m);
     return (int)hipGetLastError();
 }
 
-int ssa_dev_chol(int panel, int pass, const ssa_desc *D, const ssa_item *items, int n_items, double *small,
+int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_item *items, int n_items, double *small,
                  void *stream)
 {
     if (n_items <= 0) return 0;
-    const size_t lds = 2 * SSA_BMAX * LDP * sizeof(double);
+    const size_t lds = 2 * (size_t)bmax * (bmax + 1) * sizeof(double);
     static int attr = 0;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_chol, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute((const void *)k_chol, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(2 * SSA_BMAX * LDP * sizeof(double)));
         if (e != hipSuccess) return (int)e;
         attr = 1;
     }
@@ -883,11 +952,24 @@ int ssa_dev_chol(int panel, int pass, const ssa_desc *D, const ssa_item *items, 
     return (int)hipGetLastError();
 }
 
-int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, const ssa_desc *D, const ssa_item *items,
-                  int n_items, void *bulk, double *small, void *stream)
+int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, int bmax, const ssa_desc *D,
+                  const ssa_item *items, int n_items, void *bulk, double *small, void *stream)
 {
     if (n_items <= 0) return 0;
-    const size_t lds = SUB * LDP * sizeof(double);
+    const int b16 = (bmax + 15) & ~15;
+    const size_t lds = (size_t)(b16 * RP + b16 * (b16 + 1)) * sizeof(double);
+    const int b16max = (SSA_BMAX + 15) & ~15;
+    const size_t ldsmax = (size_t)(b16max * RP + b16max * (b16max + 1)) * sizeof(double);
+    static int attr = 0;
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_apply<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)ldsmax);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)k_apply<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)ldsmax);
+        if (e != hipSuccess) return (int)e;
+        attr = 1;
+    }
     if (prec == SSA_PREC_F64)
         hipLaunchKernelGGL(k_apply<double>, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items,
                            (double *)bulk, (const double *)small, src, dst, slot, ncols_k);
@@ -897,13 +979,14 @@ int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, const ssa_d
     return (int)hipGetLastError();
 }
 
-int ssa_dev_svd(const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream)
+int ssa_dev_svd(int bmax, const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream)
 {
     if (n_items <= 0) return 0;
-    const size_t lds = 2 * (SSA_BMAX + 1) * LDP * sizeof(double);
+    const size_t lds = 2 * (size_t)((bmax + 1) & ~1) * (bmax + 1) * sizeof(double);
     static int attr = 0;
     if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute((const void *)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(2 * (SSA_BMAX + 1) * LDP * sizeof(double)));
         if (e != hipSuccess) return (int)e;
         attr = 1;
     }

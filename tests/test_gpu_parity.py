@@ -123,12 +123,24 @@ def test_plain_parity_shapes(N, L, k):
     assert np.abs(dev.reconstruct(list(range(k))) - orc.reconstruct(list(range(k)))).max() / r < 1e-9
 
 
+def rel2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 def test_mixed_precision_error():
+    """Mixed form against the FP64 oracle: reconstruction error <= 1e-5
+    relative (2-norm) for every group, singular values 1e-5 relative.  The
+    pointwise error peaks at the two ends of the series, where diagonal
+    averaging divides the FP32 transform error (~eps32 of the convolution
+    peak) by a count of 1; it is held at 1e-3 of RMS(x)."""
     x, dev, orc = _pair(4000, 1000, 30, 10, 2, seed=5, precision=1)
     so = orc.sigma
     assert np.max(np.abs(dev.singular_values[:10] - so[:10]) / so[:10]) < 1e-5
     r = rms(x)
-    assert np.abs(dev.reconstruct(list(range(30))) - orc.reconstruct(list(range(30)))).max() / r < 1e-5
+    for g in ([0], [1, 2], [3, 4], [0, 1, 2, 3, 4], list(range(30))):
+        a, b = dev.reconstruct(g), orc.reconstruct(g)
+        assert rel2(a, b) < 1e-5, g
+        assert np.abs(a - b).max() / r < 1e-3, g
 
 
 def test_mssa_parity():
