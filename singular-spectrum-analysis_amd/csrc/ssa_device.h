@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 #define SSA_BMAX 80           /* max block width k + p */
-#define SSA_GRAM_TR 32        /* rows per LDS sub-tile of Gram / apply */
+#define SSA_GRAM_ROWS 1024    /* panel rows per Gram / apply work item */
 #define SSA_NMAX_LOG 16       /* twiddle table length 2^16 */
 
 enum { SSA_PREC_F64 = 0, SSA_PREC_F32 = 1 };
@@ -72,7 +72,10 @@ enum {
 typedef struct { int s, x, y, z; } ssa_item;
 
 /* panel selectors */
-enum { SSA_PANEL_Y = 0, SSA_PANEL_Z = 1, SSA_PANEL_U = 2, SSA_PANEL_V = 3 };
+/* SSA_PANEL_OUT: the N x k weighted component panel at off_out (W-correlation) */
+enum { SSA_PANEL_Y = 0, SSA_PANEL_Z = 1, SSA_PANEL_U = 2, SSA_PANEL_V = 3, SSA_PANEL_OUT = 4 };
+/* column blocks of 16 of a panel of width c (Gram / apply instantiations) */
+#define SSA_NB(c) (((c) + 15) / 16)
 
 /* ---- launchers (return 0 or a hipError_t value) ---- */
 int ssa_dev_twiddles(int prec, void **table_out, void *stream);
@@ -83,11 +86,12 @@ int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items,
 int ssa_dev_matvec(int prec, int lgn, int dir, int gen_omega, int in_panel, int out_panel,
                    const ssa_desc *D, const ssa_item *items, int n_items,
                    void *bulk, const void *tw, void *stream);
-int ssa_dev_gram(int prec, int panel, int use_m, int bmax, const ssa_desc *D, const ssa_item *items, int n_items,
+/* Gram / apply: every panel of the launch has at most nb*16 columns (1 <= nb <= 5) */
+int ssa_dev_gram(int prec, int panel, int use_m, int nb, const ssa_desc *D, const ssa_item *items, int n_items,
                  void *bulk, double *small, void *stream);
 int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_item *items, int n_items,
                  double *small, void *stream);
-int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, int bmax, const ssa_desc *D,
+int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, int nb, const ssa_desc *D,
                   const ssa_item *items, int n_items, void *bulk, double *small, void *stream);
 int ssa_dev_svd(int bmax, const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream);
 /* reconstruction: items (s, gid, row chunk o, col chunk sk); groups: gdesc[gid] = {start, count},
@@ -101,7 +105,7 @@ int ssa_dev_olap(int prec, int weight, const ssa_desc *D, const ssa_item *items,
                  const int *gdesc, void *bulk, void *stream);
 /* W-correlation: Gram of the sqrt(w)-weighted component panel (stored at off_out, N x k)
  * then normalisation; W written row-major k x k to small + off_small + SLOT_W */
-int ssa_dev_wcorr(int prec, const ssa_desc *D, const ssa_item *tile_items, int n_tiles,
+int ssa_dev_wcorr(int prec, int nb, const ssa_desc *D, const ssa_item *tile_items, int n_tiles,
                   const ssa_item *series_items, int n_series, void *bulk, double *small, void *stream);
 /* LRF + forecast: per series, group gdesc[0]; reads the reconstruction at off_out, writes
  * h forecasts after it (off_out + N) */

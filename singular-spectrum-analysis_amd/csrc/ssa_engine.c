@@ -27,7 +27,7 @@
 #include <string.h>
 
 #define NLG 14                 /* lgn buckets 0..13 (6..13 used) */
-#define ROWS_PER_WG 512        /* must match GRAM_ROWS_PER_WG of the kernels */
+#define NBW 6                  /* Gram / apply lists by column blocks nb = 1..5 */
 #define OLAP_TILE 1024
 
 typedef struct { long long off; int n; } span;
@@ -51,7 +51,7 @@ struct ssa_engine {
     long long ditems_cap;
     ssa_item *hitems;
     long long hitems_n, hitems_cap;
-    span spec[NLG], mvf[NLG], mva[NLG], tY, tZ, ser;
+    span spec[NLG], mvf[NLG], mva[NLG], tY[NBW], tZ[NBW], ser;   /* tY/tZ by nb */
     span rec_step[NLG], olap_step;
     int lists_dirty, desc_dirty;
     int dyn_base;              /* first item of the per-call region */
@@ -241,14 +241,21 @@ static int rebuild_lists(ssa_engine *e)
         if (build_matvec(e, 0, lg, &e->mvf[lg])) return -1;
         if (build_matvec(e, 1, lg, &e->mva[lg])) return -1;
     }
-    e->tY.off = e->hitems_n;
-    for (int s = 0; s < e->count; s++)
-        for (int t = 0; t < e->hd[s].tilesY; t++) if (items_push(e, s, t, 0, 0)) return -1;
-    e->tY.n = (int)(e->hitems_n - e->tY.off);
-    e->tZ.off = e->hitems_n;
-    for (int s = 0; s < e->count; s++)
-        for (int t = 0; t < e->hd[s].tilesZ; t++) if (items_push(e, s, t, 0, 0)) return -1;
-    e->tZ.n = (int)(e->hitems_n - e->tZ.off);
+    /* Gram / apply row tiles, one list per column-block count of the series */
+    for (int nb = 0; nb < NBW; nb++) {
+        e->tY[nb].off = e->hitems_n;
+        for (int s = 0; s < e->count; s++) {
+            if (e->hd[s].b <= 0 || SSA_NB(e->hd[s].b) != nb) continue;
+            for (int t = 0; t < e->hd[s].tilesY; t++) if (items_push(e, s, t, 0, 0)) return -1;
+        }
+        e->tY[nb].n = (int)(e->hitems_n - e->tY[nb].off);
+        e->tZ[nb].off = e->hitems_n;
+        for (int s = 0; s < e->count; s++) {
+            if (e->hd[s].b <= 0 || SSA_NB(e->hd[s].b) != nb) continue;
+            for (int t = 0; t < e->hd[s].tilesZ; t++) if (items_push(e, s, t, 0, 0)) return -1;
+        }
+        e->tZ[nb].n = (int)(e->hitems_n - e->tZ[nb].off);
+    }
     e->ser.off = e->hitems_n;
     for (int s = 0; s < e->count; s++) if (items_push(e, s, 0, 0, 0)) return -1;
     e->ser.n = (int)(e->hitems_n - e->ser.off);
@@ -327,8 +334,8 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
         int bb = e->bmax[s] > 0 ? e->bmax[s] : 60, kk = e->kmax[s] > 0 ? e->kmax[s] : 50;
         if (ssa_plan_series(prec, d->N, d->L, m, bb, kk, q_hint, -1, &pl)) goto fail;
         d->n = pl.n; d->lgn = pl.lgn; d->CL = pl.CL; d->CK = pl.CK; d->AL = pl.AL; d->SK = pl.SK;
-        d->tilesY = ceil_div(d->ML, ROWS_PER_WG);
-        d->tilesZ = ceil_div(d->K, ROWS_PER_WG);
+        d->tilesY = ceil_div(d->ML, SSA_GRAM_ROWS);
+        d->tilesZ = ceil_div(d->K, SSA_GRAM_ROWS);
         d->seed = 42;
         d->off_x = off; off = align64(off + (long long)m * d->N);
         d->off_spec = off; off = align64(off + (long long)m * d->AL * d->SK * 2 * d->n);
@@ -463,7 +470,7 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
         d->out_cap = (long long)gmax * d->N + max_h + 64;
         d->off_out = off; off = align64(off + d->out_cap);
         int tiles = d->tilesY > d->tilesZ ? d->tilesY : d->tilesZ;
-        int tout = ceil_div(d->N, ROWS_PER_WG);
+        int tout = ceil_div(d->N, SSA_GRAM_ROWS);
         if (tout > tiles) tiles = tout;
         long long part = (long long)tiles * (bm * bm + bm);
         long long need = 2LL * d->L + max_h + 256;
@@ -626,21 +633,48 @@ static int matvec_all(ssa_engine *e, int dir, int gen, int in_panel, int out_pan
     return 0;
 }
 
+/* partial Grams of a panel, one launch per column-block count */
+static int gram_all(ssa_engine *e, int panel, int use_m)
+{
+    const span *t = panel == SSA_PANEL_Y ? e->tY : e->tZ;
+    for (int nb = 1; nb < NBW; nb++) {
+        if (!t[nb].n) continue;
+        prof_begin(e, PH_GRAM, panel);
+        DEVCHK(ssa_dev_gram(e->prec, panel, use_m, nb, e->dd, it_ptr(e, t[nb]), t[nb].n, e->arena_a, e->small,
+                            e->st));
+        prof_end(e);
+    }
+    return 0;
+}
+
+/* dst = src * (small matrix in slot) over the row tiles of src */
+static int apply_all(ssa_engine *e, int phase, int src, int dst, int slot, int ncols_k)
+{
+    const span *t = src == SSA_PANEL_Y ? e->tY : e->tZ;
+    for (int nb = 1; nb < NBW; nb++) {
+        if (!t[nb].n) continue;
+        prof_begin(e, phase, src);
+        DEVCHK(ssa_dev_apply(e->prec, src, dst, slot, ncols_k, nb, e->dd, it_ptr(e, t[nb]), t[nb].n, e->arena_a,
+                             e->small, e->st));
+        prof_end(e);
+    }
+    return 0;
+}
+
+static int chol_all(ssa_engine *e, int panel, int pass, int bm)
+{
+    prof_begin(e, PH_CHOL, panel);
+    DEVCHK(ssa_dev_chol(panel, pass, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    prof_end(e);
+    return 0;
+}
+
 static int cholqr_factor(ssa_engine *e, int panel, int bm)
 {
-    span t = panel == SSA_PANEL_Y ? e->tY : e->tZ;
-    prof_begin(e, PH_GRAM, panel);
-    DEVCHK(ssa_dev_gram(e->prec, panel, 0, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
-    prof_end(e);
-    prof_begin(e, PH_CHOL, panel);
-    DEVCHK(ssa_dev_chol(panel, 1, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
-    prof_end(e);
-    prof_begin(e, PH_GRAM, panel);
-    DEVCHK(ssa_dev_gram(e->prec, panel, 1, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
-    prof_end(e);
-    prof_begin(e, PH_CHOL, panel);
-    DEVCHK(ssa_dev_chol(panel, 2, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
-    prof_end(e);
+    DEVCHK(gram_all(e, panel, 0));
+    DEVCHK(chol_all(e, panel, 1, bm));
+    DEVCHK(gram_all(e, panel, 1));
+    DEVCHK(chol_all(e, panel, 2, bm));
     return 0;
 }
 
@@ -651,24 +685,16 @@ static int cholqr_factor(ssa_engine *e, int panel, int bm)
  * eps64 * cond^2 there, which the next orthonormalisation absorbs. */
 static int orth(ssa_engine *e, int panel, int bm, int two_pass)
 {
-    span t = panel == SSA_PANEL_Y ? e->tY : e->tZ;
     int slot;
     if (two_pass) {
         DEVCHK(cholqr_factor(e, panel, bm));
         slot = panel == SSA_PANEL_Y ? SSA_SLOT_RINVY : SSA_SLOT_RINVZ;
     } else {
-        prof_begin(e, PH_GRAM, panel);
-        DEVCHK(ssa_dev_gram(e->prec, panel, 0, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
-        prof_end(e);
-        prof_begin(e, PH_CHOL, panel);
-        DEVCHK(ssa_dev_chol(panel, 1, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
-        prof_end(e);
+        DEVCHK(gram_all(e, panel, 0));
+        DEVCHK(chol_all(e, panel, 1, bm));
         slot = SSA_SLOT_R1INV;
     }
-    prof_begin(e, PH_APPLY, panel);
-    DEVCHK(ssa_dev_apply(e->prec, panel, panel, slot, 0, bm, e->dd, it_ptr(e, t), t.n, e->arena_a, e->small, e->st));
-    prof_end(e);
-    return 0;
+    return apply_all(e, PH_APPLY, panel, panel, slot, 0);
 }
 
 static int launch_decompose(ssa_engine *e, int q, int injected)
@@ -688,12 +714,8 @@ static int launch_decompose(ssa_engine *e, int q, int injected)
     prof_begin(e, PH_SVD, 0);
     DEVCHK(ssa_dev_svd(bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
     prof_end(e);
-    prof_begin(e, PH_UV, 0);
-    DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Y, SSA_PANEL_U, SSA_SLOT_WK, 1, bm, e->dd, it_ptr(e, e->tY), e->tY.n,
-                         e->arena_a, e->small, e->st));
-    DEVCHK(ssa_dev_apply(e->prec, SSA_PANEL_Z, SSA_PANEL_V, SSA_SLOT_WKS, 1, bm, e->dd, it_ptr(e, e->tZ), e->tZ.n,
-                         e->arena_a, e->small, e->st));
-    prof_end(e);
+    DEVCHK(apply_all(e, PH_UV, SSA_PANEL_Y, SSA_PANEL_U, SSA_SLOT_WK, 1));
+    DEVCHK(apply_all(e, PH_UV, SSA_PANEL_Z, SSA_PANEL_V, SSA_SLOT_WKS, 1));
     return 0;
 }
 
@@ -838,13 +860,13 @@ int ssa_engine_wcorr(ssa_engine *e, int s, double *W)
     e->hitems_n = e->dyn_base;
     span t, one;
     t.off = e->hitems_n;
-    for (int i = 0; i < ceil_div(d->N, ROWS_PER_WG); i++) DEVCHK(items_push(e, s, i, 0, 0));
+    for (int i = 0; i < ceil_div(d->N, SSA_GRAM_ROWS); i++) DEVCHK(items_push(e, s, i, 0, 0));
     t.n = (int)(e->hitems_n - t.off);
     one.off = e->hitems_n;
     DEVCHK(items_push(e, s, 0, 0, 0));
     one.n = 1;
     DEVCHK(items_upload(e, e->dyn_base));
-    DEVCHK(ssa_dev_wcorr(e->prec, e->dd, it_ptr(e, t), t.n, it_ptr(e, one), 1, e->arena_a, e->small, e->st));
+    DEVCHK(ssa_dev_wcorr(e->prec, SSA_NB(k), e->dd, it_ptr(e, t), t.n, it_ptr(e, one), 1, e->arena_a, e->small, e->st));
     DEVCHK(pin_reserve(e, sizeof(double) * k * k));
     HIPCHK(hipMemcpyAsync(e->hpin, e->small + d->off_small + (long long)SSA_SLOT_W * SSA_BMAX * SSA_BMAX,
                           sizeof(double) * k * k, hipMemcpyDeviceToHost, e->st));

@@ -33,8 +33,6 @@ using namespace ssa;
 
 namespace {
 
-constexpr int GRAM_ROWS_PER_WG = 512;   // rows of a panel handled by one Gram / apply workgroup
-constexpr int SUB = SSA_GRAM_TR;        // LDS sub-tile rows
 constexpr int LDP = SSA_BMAX + 1;       // padded LDS row length of small matrices
 
 __device__ __forceinline__ double *slot(double *small, const ssa_desc &d, int s)
@@ -252,128 +250,233 @@ __device__ __forceinline__ d4v mfma64(double a, double b, d4v c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-constexpr int RT = 64;          // rows of an LDS sub-tile (4 waves x 16)
-constexpr int RP = RT + 1;      // padded row count of a staged column
+// Panels are read straight from HBM into MFMA operand registers; only the
+// small b x b matrix of an apply (or of the fused second CholeskyQR pass)
+// goes through LDS.  Two register layouts are used:
+//  - columns in lanes (Gram of P): lane (c = l%16, q = l/16) holds 4
+//    consecutive rows 4q..4q+3 of column c of a 16-row slab; MFMA step v
+//    uses row 4q+v, which permutes the summation index of the Gram
+//    identically for both operands.
+//  - rows in lanes (P M, and (P M)^T): lane (r = l%16, q = l/16) holds VR
+//    consecutive rows VR*r..VR*r+VR-1 of column 4kk+q (one 8-byte load);
+//    row block rb takes physical row VR*r + rb for MFMA row r.
+// ---------------------------------------------------------------------------
+typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
 
-// Stage rows [r0, r0 + RT) of panel P (b columns) into Pt[c*RP + r] as FP64,
-// zero-filled beyond the panel and up to b16 columns.
-template <typename T>
-__device__ __forceinline__ void stage_tile(const Panel<T> &P, int b, int b16, int r0, int r_end, double *Pt)
+// 4 consecutive elements p[0..3], those at or beyond n read as zero
+template <typename T> __device__ __forceinline__ void load4(const T *p, int n, T (&o)[4])
 {
-    for (int idx = threadIdx.x; idx < RT * b16; idx += 256) {
-        const int c = idx / RT, r = idx - c * RT;
-        Pt[c * RP + r] = (c < b && r0 + r < r_end) ? (double)P.p[(long long)c * P.ld + r0 + r] : 0.0;
+    if (n >= 4) {
+        if constexpr (sizeof(T) == 4) {
+            const f4u v = *(const f4u *)p;
+            o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+        } else {
+            const d2u v0 = *(const d2u *)p, v1 = *(const d2u *)(p + 2);
+            o[0] = v0.x; o[1] = v0.y; o[2] = v1.x; o[3] = v1.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) o[i] = (i < n) ? p[i] : (T)0;
     }
 }
 
-// Stage the b x nc column-major small matrix M (ld SSA_BMAX) into Ms[k*MP + c].
-__device__ __forceinline__ void stage_small(const double *M, int b, int nc, int b16, int nc16, double *Ms, int MP)
+// VR = 8 / sizeof(T) consecutive elements (one 8-byte load), zero at or beyond n
+template <typename T> struct RowVec { static constexpr int VR = 8 / (int)sizeof(T); };
+
+template <typename T> __device__ __forceinline__ void load_vr(const T *p, int n, T (&o)[RowVec<T>::VR])
 {
-    for (int idx = threadIdx.x; idx < b16 * nc16; idx += 256) {
-        const int c = idx / b16, k = idx - c * b16;
+    if constexpr (sizeof(T) == 4) {
+        if (n >= 2) {
+            const f2u v = *(const f2u *)p;
+            o[0] = v.x; o[1] = v.y;
+        } else {
+            o[0] = (n >= 1) ? p[0] : 0.0f;
+            o[1] = 0.0f;
+        }
+    } else {
+        o[0] = (n >= 1) ? p[0] : 0.0;
+    }
+}
+
+template <typename T> __device__ __forceinline__ void store_vr(T *p, int n, const T (&o)[RowVec<T>::VR])
+{
+    if constexpr (sizeof(T) == 4) {
+        if (n >= 2) {
+            *(f2u *)p = f2u{o[0], o[1]};
+        } else if (n >= 1) {
+            p[0] = o[0];
+        }
+    } else {
+        if (n >= 1) p[0] = o[0];
+    }
+}
+
+// LDS row stride (doubles) of a staged small matrix with NB column blocks:
+// == 16 (mod 32) so the 4 row groups of one MFMA operand read hit disjoint banks
+template <int NB> constexpr int small_ld() { return (NB & 1) ? NB * 16 : NB * 16 + 16; }
+
+// Stage the b x nc column-major small matrix M (ld SSA_BMAX) into Ms[k*MP + c],
+// zero beyond b rows / nc columns, NB*16 rows and columns.
+template <int NB>
+__device__ __forceinline__ void stage_small(const double *M, int b, int nc, double *Ms, int nthreads)
+{
+    constexpr int B16 = NB * 16, MP = small_ld<NB>();
+    for (int idx = threadIdx.x; idx < B16 * B16; idx += nthreads) {
+        const int c = idx / B16, k = idx - c * B16;
         Ms[k * MP + c] = (k < b && c < nc) ? M[(long long)c * SSA_BMAX + k] : 0.0;
     }
 }
 
-// Rows 16w..16w+15 of the staged tile times Ms (b x nc): D for column block cb
-// is T[16w + j][cb*16 + i] for lane (i, j) of the accumulator layout.
-__device__ __forceinline__ d4v tile_times_small(const double *Pt, const double *Ms, int MP, int b16, int cb, int w,
-                                                bool upper)
-{
-    const int l = threadIdx.x & 63;
-    d4v acc = {0, 0, 0, 0};
-    // an upper-triangular M has no rows below the column block's last column
-    const int kend = upper ? min(b16, (cb + 1) * 16) : b16;
-    for (int k0 = 0; k0 < kend; k0 += 4) {
-        const double a = Ms[(k0 + (l >> 4)) * MP + cb * 16 + (l & 15)];
-        const double bb = Pt[(k0 + (l >> 4)) * RP + 16 * w + (l & 15)];
-        acc = mfma64(a, bb, acc);
-    }
-    return acc;
-}
+template <int NB> constexpr int n_upper_blocks() { return NB * (NB + 1) / 2; }
 
 // ---------------------------------------------------------------------------
-// partial Gram: item (s, tile) covers rows [tile*512, +512) of the panel.
-// G_part = T^T T with T = P (use_m = 0) or T = P * M (M = R1^-1 slot); FP64
-// MFMA with FP64 accumulation whatever the panel type.  use_m = 0 also emits
-// the column sums of P.  Only the upper block triangle is multiplied; the
-// partial is written in full (mirrored).
+// partial Gram: item (s, tile) covers rows [tile*SSA_GRAM_ROWS, +SSA_GRAM_ROWS)
+// of the panel, dealt to the 4 waves in slabs.  G_part = T^T T with T = P
+// (USE_M = 0, columns-in-lanes layout) or T = P M, M = R1^-1 upper triangular
+// (USE_M = 1, one row per lane: the accumulator layout of P M is the
+// operand layout of the Gram, so T never leaves registers).  FP64 MFMA, FP64
+// accumulation whatever the panel type; only the upper block triangle is
+// multiplied, the partial is written in full.  USE_M = 0 also emits the column
+// sums of P.  The 4 waves' sums are added in a fixed order (deterministic).
+// NB = column blocks of 16 covering every panel of the launch.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int NB, int USE_M>
 __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items,
-                                              const T *__restrict__ bulk, double *small, int which, int use_m)
+                                              const T *__restrict__ bulk, double *small, int which)
 {
     extern __shared__ double sm[];
+    constexpr int NBLK = n_upper_blocks<NB>(), MP = small_ld<NB>();
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const Panel<T> P = panel_of<T>(const_cast<T *>(bulk), d, which);
-    const int b = P.cols, b16 = (b + 15) & ~15, nb = b16 >> 4;
-    const int MP = b16 + 1;
-    double *Pt = sm;                      // b16 x RP
-    double *Ms = sm + b16 * RP;           // b16 x MP (use_m)
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int r_begin = it.x * GRAM_ROWS_PER_WG;
-    const int r_end = min(P.rows, r_begin + GRAM_ROWS_PER_WG);
-    if (use_m) stage_small(slot(small, d, SSA_SLOT_R1INV), b, b, b16, b16, Ms, MP);
-    // upper-triangle blocks (I <= J) dealt round-robin to the 4 waves
-    const int nblk = nb * (nb + 1) / 2;
-    // block m of this wave is the (4m + w)-th upper block in row-major order
-    d4v acc[4];
-    int bi[4], bj[4];
+    const int b = P.cols;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, cl = l & 15, kq = l >> 4;
+    const int r_begin = it.x * SSA_GRAM_ROWS;
+    const int r_end = min(P.rows, r_begin + SSA_GRAM_ROWS);
+    d4v acc[NBLK];
 #pragma unroll
-    for (int m = 0; m < 4; m++) {
-        acc[m] = d4v{0, 0, 0, 0};
-        int q = 4 * m + w, I = 0;
-        while (I < nb && q >= nb - I) { q -= nb - I; I++; }
-        bi[m] = I;                 /* I == nb: no block */
-        bj[m] = I + q;
-    }
-    (void)nblk;
-    double csum = 0;
-    for (int r0 = r_begin; r0 < r_end; r0 += RT) {
-        __syncthreads();
-        stage_tile(P, b, b16, r0, r_end, Pt);
-        __syncthreads();
-        if (use_m) {
-            // T = P M for this sub-tile, in registers, then written over Pt
-            d4v t[5];
+    for (int m = 0; m < NBLK; m++) acc[m] = d4v{0, 0, 0, 0};
+    double cs[NB];
 #pragma unroll
-            for (int cb = 0; cb < 5; cb++)
-                if (cb < nb) t[cb] = tile_times_small(Pt, Ms, MP, b16, cb, w, true);
-            __syncthreads();
+    for (int I = 0; I < NB; I++) cs[I] = 0.0;
+
+    if constexpr (!USE_M) {
+        // 16-row slabs, slab j of the item to wave j % 4
+        T cur[NB][4], nxt[NB][4];
+        auto load = [&](int r0, T (&x)[NB][4]) {
 #pragma unroll
-            for (int cb = 0; cb < 5; cb++)
-                if (cb < nb) {
+            for (int I = 0; I < NB; I++) {
+                const int c = I * 16 + cl, r = r0 + 4 * kq;
+                if (c < b && r0 < r_end) load4(P.p + (long long)c * P.ld + r, r_end - r, x[I]);
+                else {
 #pragma unroll
-                    for (int v = 0; v < 4; v++) Pt[(cb * 16 + (l >> 4) + 4 * v) * RP + 16 * w + (l & 15)] = t[cb][v];
+                    for (int v = 0; v < 4; v++) x[I][v] = (T)0;
                 }
-            __syncthreads();
-        } else if (threadIdx.x < b) {
-            for (int r = 0; r < RT; r++) csum += Pt[threadIdx.x * RP + r];
-        }
+            }
+        };
+        int r0 = r_begin + 16 * w;
+        load(r0, cur);
+        for (; r0 < r_end; r0 += 64) {
+            load(r0 + 64, nxt);
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            if (bi[m] >= nb) continue;
-            const double *ca = Pt + (bi[m] * 16 + (l & 15)) * RP + (l >> 4);
-            const double *cbp = Pt + (bj[m] * 16 + (l & 15)) * RP + (l >> 4);
-#pragma unroll 4
-            for (int k0 = 0; k0 < RT; k0 += 4) acc[m] = mfma64(ca[k0], cbp[k0], acc[m]);
+            for (int v = 0; v < 4; v++) {
+#pragma unroll
+                for (int I = 0; I < NB; I++) cs[I] += (double)cur[I][v];
+                int m = 0;
+#pragma unroll
+                for (int I = 0; I < NB; I++)
+#pragma unroll
+                    for (int J = I; J < NB; J++, m++) acc[m] = mfma64((double)cur[I][v], (double)cur[J][v], acc[m]);
+            }
+#pragma unroll
+            for (int I = 0; I < NB; I++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) cur[I][v] = nxt[I][v];
+        }
+    } else {
+        // one row per lane (16-row slabs): P M for a slab stays in registers
+        constexpr int KK = NB * 4;
+        double *Ms = sm;
+        stage_small<NB>(slot(small, d, SSA_SLOT_R1INV), b, b, Ms, 256);
+        __syncthreads();
+        T cur[KK], nxt[KK];
+        auto load = [&](int r0, T (&a)[KK]) {
+            const int r = r0 + cl;
+#pragma unroll
+            for (int kk = 0; kk < KK; kk++) {
+                const int c = 4 * kk + kq;
+                a[kk] = (c < b && r < r_end) ? P.p[(long long)c * P.ld + r] : (T)0;
+            }
+        };
+        int r0 = r_begin + 16 * w;
+        load(r0, cur);
+        for (; r0 < r_end; r0 += 64) {
+            load(r0 + 64, nxt);
+            d4v t[NB];
+#pragma unroll
+            for (int cb = 0; cb < NB; cb++) {
+                t[cb] = d4v{0, 0, 0, 0};
+                // M upper triangular: rows k < 16 (cb + 1) only
+#pragma unroll
+                for (int kk = 0; kk < 4 * (cb + 1); kk++)
+                    t[cb] = mfma64((double)cur[kk], Ms[(4 * kk + kq) * MP + cb * 16 + cl], t[cb]);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                int m = 0;
+#pragma unroll
+                for (int I = 0; I < NB; I++)
+#pragma unroll
+                    for (int J = I; J < NB; J++, m++) acc[m] = mfma64(t[I][v], t[J][v], acc[m]);
+            }
+#pragma unroll
+            for (int kk = 0; kk < KK; kk++) cur[kk] = nxt[kk];
         }
     }
-    double *out = small + d.off_part + (long long)it.x * (b * b + b);
+    // column sums: lanes of one column (kq = 0..3) first, then the waves
 #pragma unroll
-    for (int m = 0; m < 4; m++) {
-        if (bi[m] >= nb) continue;
+    for (int I = 0; I < NB; I++) {
+        cs[I] += __shfl_xor(cs[I], 16);
+        cs[I] += __shfl_xor(cs[I], 32);
+    }
+    __syncthreads();   // LDS (small matrix) free from here
+    double *red = sm;  // NBLK x 4 x 64 accumulator values, then NB x 16 column sums
+    double *rcs = sm + NBLK * 256;
+    for (int ww = 0; ww < 4; ww++) {
+        if (w == ww) {
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int i = bi[m] * 16 + (l >> 4) + 4 * v, j = bj[m] * 16 + (l & 15);
-            if (i < b && j < b) {
-                out[i * b + j] = acc[m][v];
-                out[j * b + i] = acc[m][v];
+            for (int m = 0; m < NBLK; m++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    double *p = red + (m * 4 + v) * 64 + l;
+                    *p = (ww == 0 ? 0.0 : *p) + acc[m][v];
+                }
+            if (kq == 0) {
+#pragma unroll
+                for (int I = 0; I < NB; I++) {
+                    double *p = rcs + I * 16 + cl;
+                    *p = (ww == 0 ? 0.0 : *p) + cs[I];
+                }
             }
         }
+        __syncthreads();
     }
-    if (!use_m && threadIdx.x < b) out[b * b + threadIdx.x] = csum;
+    double *out = small + d.off_part + (long long)it.x * (b * b + b);
+    for (int idx = threadIdx.x; idx < NBLK * 256; idx += 256) {
+        const int m = idx >> 8, v = (idx >> 6) & 3, ll = idx & 63;
+        int I = 0, mm = m;
+        while (mm >= NB - I) { mm -= NB - I; I++; }
+        const int J = I + mm;
+        const int i = I * 16 + (ll >> 4) + 4 * v, j = J * 16 + (ll & 15);
+        if (i < b && j < b) {
+            const double g = red[idx];
+            out[i * b + j] = g;
+            if (I != J) out[j * b + i] = g;
+        }
+    }
+    if (!USE_M && threadIdx.x < b) out[b * b + threadIdx.x] = rcs[threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -495,44 +598,79 @@ __global__ __launch_bounds__(256) void k_chol(const ssa_desc *D, const ssa_item 
 }
 
 // ---------------------------------------------------------------------------
-// out (rows x nc) = in (rows x b) * M (b x nc, col-major ld BMAX), FP64 MFMA
-// with FP64 accumulation, T-typed panels; in place allowed (each sub-tile is
-// staged in LDS before its rows are written).  item (s, tile of 512 rows);
-// ncols_k: 0 -> nc = b, 1 -> nc = k.
+// out (rows x nc) = in (rows x b) * M (b x nc, col-major ld BMAX) as
+// out^T = M^T in^T: FP64 MFMA with FP64 accumulation, T-typed panels, rows in
+// lanes so that each store is 16 consecutive rows of a column.  item (s, tile
+// of SSA_GRAM_ROWS rows), 8 waves sharing the staged M; in place allowed (a
+// wave loads all b columns of its rows before it stores them).  ncols_k: 0 ->
+// nc = b and M upper triangular (R^-1), 1 -> nc = k, M full.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_apply(const ssa_desc *D, const ssa_item *items, T *bulk, const double *small,
-                                               int src, int dst, int mslot, int ncols_k)
+constexpr int APPLY_WAVES = 8;
+
+template <typename T, int NB>
+__global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply(const ssa_desc *D, const ssa_item *items, T *bulk,
+                                                            const double *small, int src, int dst, int mslot,
+                                                            int ncols_k)
 {
     extern __shared__ double sm[];
+    constexpr int VR = RowVec<T>::VR, SL = 16 * VR, KK = NB * 4, MP = small_ld<NB>();
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const Panel<T> Pi = panel_of<T>(bulk, d, src);
     const Panel<T> Po = panel_of<T>(bulk, d, dst);
     const int b = d.b, nc = ncols_k ? d.k : d.b;
-    const int b16 = (b + 15) & ~15, nc16 = (nc + 15) & ~15, ncb = nc16 >> 4;
-    const int MP = nc16 + 1;
-    double *Pt = sm;                  // b16 x RP
-    double *Ms = sm + b16 * RP;       // b16 x MP
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    stage_small(small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX, b, nc, b16, nc16, Ms, MP);
-    const int r_begin = it.x * GRAM_ROWS_PER_WG;
-    const int r_end = min(Pi.rows, r_begin + GRAM_ROWS_PER_WG);
-    for (int r0 = r_begin; r0 < r_end; r0 += RT) {
-        __syncthreads();
-        stage_tile(Pi, b, b16, r0, r_end, Pt);
-        __syncthreads();
-        const int row = r0 + 16 * w + (l & 15);
-        for (int cb = 0; cb < ncb; cb++) {
-            const d4v t = tile_times_small(Pt, Ms, MP, b16, cb, w, !ncols_k);
-            if (row < r_end) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, cl = l & 15, kq = l >> 4;
+    double *Ms = sm;
+    stage_small<NB>(small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX, b, nc, Ms, 64 * APPLY_WAVES);
+    __syncthreads();
+    const int r_begin = it.x * SSA_GRAM_ROWS;
+    const int r_end = min(Pi.rows, r_begin + SSA_GRAM_ROWS);
+    T cur[KK][VR], nxt[KK][VR];
+    auto load = [&](int r0, T (&a)[KK][VR]) {
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int c = cb * 16 + (l >> 4) + 4 * v;
-                    if (c < nc) Po.p[(long long)c * Po.ld + row] = (T)t[v];
+        for (int kk = 0; kk < KK; kk++) {
+            const int c = 4 * kk + kq, r = r0 + VR * cl;
+            if (c < b && r < r_end) load_vr(Pi.p + (long long)c * Pi.ld + r, r_end - r, a[kk]);
+            else {
+#pragma unroll
+                for (int v = 0; v < VR; v++) a[kk][v] = (T)0;
+            }
+        }
+    };
+    int r0 = r_begin + SL * w;
+    load(r0, cur);
+    for (; r0 < r_end; r0 += APPLY_WAVES * SL) {
+        load(r0 + APPLY_WAVES * SL, nxt);
+        const int r = r0 + VR * cl;
+#pragma unroll
+        for (int cb = 0; cb < NB; cb++) {
+            if (cb * 16 >= nc) break;
+            d4v t[VR];
+#pragma unroll
+            for (int rb = 0; rb < VR; rb++) t[rb] = d4v{0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < KK; kk++) {
+                // an upper-triangular M has no rows below the column block's last column
+                if (!ncols_k && kk >= 4 * (cb + 1)) break;
+                const double mv = Ms[(4 * kk + kq) * MP + cb * 16 + cl];
+#pragma unroll
+                for (int rb = 0; rb < VR; rb++) t[rb] = mfma64(mv, (double)cur[kk][rb], t[rb]);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int c = cb * 16 + kq + 4 * v;
+                if (c < nc && r < r_end) {
+                    T o[VR];
+#pragma unroll
+                    for (int rb = 0; rb < VR; rb++) o[rb] = (T)t[rb][v];
+                    store_vr(Po.p + (long long)c * Po.ld + r, r_end - r, o);
                 }
             }
         }
+#pragma unroll
+        for (int kk = 0; kk < KK; kk++)
+#pragma unroll
+            for (int v = 0; v < VR; v++) cur[kk][v] = nxt[kk][v];
     }
 }
 
@@ -564,12 +702,16 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nwave = blockDim.x >> 6;
+    // 32 groups of 8 lanes; group g rotates pair g of the round (bp/2 <= 40 pairs:
+    // groups take pairs g and g + 32), lanes stride the b rows
+    const int grp = threadIdx.x >> 3, gl = threadIdx.x & 7;
     int sweeps = 0;
     for (; sweeps < 40; sweeps++) {
         if (threadIdx.x == 0) rotated = 0;
         __syncthreads();
+        int any_local = 0;
         for (int round = 0; round < bp - 1; round++) {
-            for (int pi = wave; pi < bp / 2; pi += nwave) {
+            for (int pi = grp; pi < bp / 2; pi += 32) {
                 int p, q;
                 if (pi == 0) { p = round; q = bp - 1; }
                 else {
@@ -578,11 +720,11 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                 }
                 if (p > q) { int t = p; p = q; q = t; }
                 double al = 0, be = 0, ga = 0;
-                for (int i = lane; i < b; i += 64) {
+                for (int i = gl; i < b; i += 8) {
                     const double x = A[p * LDP + i], y = A[q * LDP + i];
                     al += x * x; be += y * y; ga += x * y;
                 }
-                for (int o = 32; o > 0; o >>= 1) {
+                for (int o = 4; o > 0; o >>= 1) {
                     al += __shfl_xor(al, o);
                     be += __shfl_xor(be, o);
                     ga += __shfl_xor(ga, o);
@@ -591,7 +733,7 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                     const double zeta = (be - al) / (2.0 * ga);
                     const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-                    for (int i = lane; i < b; i += 64) {
+                    for (int i = gl; i < b; i += 8) {
                         const double x = A[p * LDP + i], y = A[q * LDP + i];
                         A[p * LDP + i] = c * x - s * y;
                         A[q * LDP + i] = s * x + c * y;
@@ -599,11 +741,13 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                         W[p * LDP + i] = c * wx - s * wy;
                         W[q * LDP + i] = s * wx + c * wy;
                     }
-                    if (lane == 0) rotated = 1;
+                    any_local = 1;
                 }
             }
             __syncthreads();
         }
+        if (any_local) rotated = 1;
+        __syncthreads();
         const int any = rotated;
         __syncthreads();
         if (!any) break;
@@ -748,7 +892,7 @@ __global__ __launch_bounds__(256) void k_wnorm(const ssa_desc *D, const ssa_item
 {
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
-    const int k = d.k, tiles = (d.N + GRAM_ROWS_PER_WG - 1) / GRAM_ROWS_PER_WG;
+    const int k = d.k, tiles = (d.N + SSA_GRAM_ROWS - 1) / SSA_GRAM_ROWS;
     const double *part = small + d.off_part;
     const double *sig = slot(small, d, SSA_SLOT_MISC) + SSA_MISC_SIGMA;
     double *W = slot(small, d, SSA_SLOT_W);
@@ -910,28 +1054,38 @@ int ssa_dev_matvec(int prec, int lgn, int dir, int gen_omega, int in_panel, int 
     return (int)hipGetLastError();
 }
 
-int ssa_dev_gram(int prec, int panel, int use_m, int bmax, const ssa_desc *D, const ssa_item *items, int n_items,
+extern "C++" {
+template <typename T, int NB>
+static void launch_gram(int panel, int use_m, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk,
+                        double *small, hipStream_t st)
+{
+    constexpr int NBLK = n_upper_blocks<NB>(), B16 = NB * 16, MP = small_ld<NB>();
+    const size_t red = (size_t)(NBLK * 256 + NB * 16) * sizeof(double);
+    const size_t ms = (size_t)B16 * MP * sizeof(double);
+    if (use_m)
+        hipLaunchKernelGGL((k_gram<T, NB, 1>), dim3(n_items), dim3(256), red > ms ? red : ms, st, D, items,
+                           (const T *)bulk, small, panel);
+    else
+        hipLaunchKernelGGL((k_gram<T, NB, 0>), dim3(n_items), dim3(256), red, st, D, items, (const T *)bulk, small,
+                           panel);
+}
+}  // extern "C++"
+
+int ssa_dev_gram(int prec, int panel, int use_m, int nb, const ssa_desc *D, const ssa_item *items, int n_items,
                  void *bulk, double *small, void *stream)
 {
     if (n_items <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    const int b16 = (bmax + 15) & ~15;
-    const size_t lds = (size_t)(b16 * RP + (use_m ? b16 * (b16 + 1) : 0)) * sizeof(double);
-    static int attr = 0;
-    if (!attr) {
-        const int mx = (int)((size_t)(96 * RP + 96 * 97) * sizeof(double));
-        hipError_t e = hipFuncSetAttribute((const void *)k_gram<double>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_gram<float>, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-        if (e != hipSuccess) return (int)e;
-        attr = 1;
+#define G(NB)                                                                                   \
+    case NB:                                                                                    \
+        if (prec == SSA_PREC_F64) launch_gram<double, NB>(panel, use_m, D, items, n_items, bulk, small, st); \
+        else launch_gram<float, NB>(panel, use_m, D, items, n_items, bulk, small, st);          \
+        break;
+    switch (nb) {
+        G(1) G(2) G(3) G(4) G(5)
+    default: return -1;
     }
-    if (prec == SSA_PREC_F64)
-        hipLaunchKernelGGL(k_gram<double>, dim3(n_items), dim3(256), lds, st, D, items, (const double *)bulk, small,
-                           panel, use_m);
-    else
-        hipLaunchKernelGGL(k_gram<float>, dim3(n_items), dim3(256), lds, st, D, items, (const float *)bulk, small,
-                           panel, use_m);
+#undef G
     return (int)hipGetLastError();
 }
 
@@ -952,30 +1106,32 @@ int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_ite
     return (int)hipGetLastError();
 }
 
-int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, int bmax, const ssa_desc *D,
+extern "C++" {
+template <typename T, int NB>
+static void launch_apply(int src, int dst, int slot, int ncols_k, const ssa_desc *D, const ssa_item *items,
+                         int n_items, void *bulk, const double *small, hipStream_t st)
+{
+    const size_t lds = (size_t)NB * 16 * small_ld<NB>() * sizeof(double);
+    hipLaunchKernelGGL((k_apply<T, NB>), dim3(n_items), dim3(64 * APPLY_WAVES), lds, st, D, items, (T *)bulk, small,
+                       src, dst, slot, ncols_k);
+}
+}  // extern "C++"
+
+int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, int nb, const ssa_desc *D,
                   const ssa_item *items, int n_items, void *bulk, double *small, void *stream)
 {
     if (n_items <= 0) return 0;
-    const int b16 = (bmax + 15) & ~15;
-    const size_t lds = (size_t)(b16 * RP + b16 * (b16 + 1)) * sizeof(double);
-    const int b16max = (SSA_BMAX + 15) & ~15;
-    const size_t ldsmax = (size_t)(b16max * RP + b16max * (b16max + 1)) * sizeof(double);
-    static int attr = 0;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_apply<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)ldsmax);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)k_apply<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)ldsmax);
-        if (e != hipSuccess) return (int)e;
-        attr = 1;
+    hipStream_t st = (hipStream_t)stream;
+#define A(NB)                                                                                                  \
+    case NB:                                                                                                   \
+        if (prec == SSA_PREC_F64) launch_apply<double, NB>(src, dst, slot, ncols_k, D, items, n_items, bulk, small, st); \
+        else launch_apply<float, NB>(src, dst, slot, ncols_k, D, items, n_items, bulk, small, st);             \
+        break;
+    switch (nb) {
+        A(1) A(2) A(3) A(4) A(5)
+    default: return -1;
     }
-    if (prec == SSA_PREC_F64)
-        hipLaunchKernelGGL(k_apply<double>, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items,
-                           (double *)bulk, (const double *)small, src, dst, slot, ncols_k);
-    else
-        hipLaunchKernelGGL(k_apply<float>, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items,
-                           (float *)bulk, (const double *)small, src, dst, slot, ncols_k);
+#undef A
     return (int)hipGetLastError();
 }
 
@@ -1030,10 +1186,10 @@ int ssa_dev_olap(int prec, int weight, const ssa_desc *D, const ssa_item *items,
     return (int)hipGetLastError();
 }
 
-int ssa_dev_wcorr(int prec, const ssa_desc *D, const ssa_item *tile_items, int n_tiles,
+int ssa_dev_wcorr(int prec, int nb, const ssa_desc *D, const ssa_item *tile_items, int n_tiles,
                   const ssa_item *series_items, int n_series, void *bulk, double *small, void *stream)
 {
-    int e = ssa_dev_gram(prec, 4, 0, SSA_BMAX, D, tile_items, n_tiles, bulk, small, stream);
+    int e = ssa_dev_gram(prec, SSA_PANEL_OUT, 0, nb, D, tile_items, n_tiles, bulk, small, stream);
     if (e) return e;
     if (n_series <= 0) return 0;
     hipLaunchKernelGGL(k_wnorm, dim3(n_series), dim3(256), 0, (hipStream_t)stream, D, series_items, small);
