@@ -76,7 +76,7 @@ struct ssa_engine {
 
 /* profiler phases */
 enum { PH_SPECTRA = 0, PH_MV_FWD, PH_MV_ADJ, PH_GRAM, PH_CHOL, PH_APPLY, PH_SVD, PH_UV, PH_RECON, PH_OLAP, PH_FORECAST,
-       PH_N };
+       PH_MV_GEN, PH_N };
 
 static void prof_begin(struct ssa_engine *e, int phase, int lgn)
 {
@@ -625,7 +625,7 @@ static int matvec_all(ssa_engine *e, int dir, int gen, int in_panel, int out_pan
     for (int lg = 6; lg < NLG; lg++) {
         span sp = dir == 0 ? e->mvf[lg] : e->mva[lg];
         if (!sp.n) continue;
-        prof_begin(e, dir == 0 ? PH_MV_FWD : PH_MV_ADJ, lg);
+        prof_begin(e, gen ? PH_MV_GEN : dir == 0 ? PH_MV_FWD : PH_MV_ADJ, lg);
         DEVCHK(ssa_dev_matvec(e->prec, lg, dir, gen, in_panel, out_panel, e->dd, it_ptr(e, sp), sp.n, e->arena_a,
                               e->tw, e->st));
         prof_end(e);

@@ -89,6 +89,35 @@ __device__ __forceinline__ void gauss_pair(uint64_t seed, uint64_t call, int row
     g1 = rad * sin(ang);
 }
 
+// The same draw with the transcendental steps in FP32 (the FP32 panels of
+// the mixed form): the uniforms and 1 - u1 are formed exactly as above in
+// FP64, then rounded, so the tail of the radius is kept.
+__device__ __forceinline__ void gauss_pair_f(uint64_t seed, uint64_t call, int row, int pair, float &g0, float &g1)
+{
+    uint32_t c[4] = {(uint32_t)row, (uint32_t)pair, (uint32_t)call, (uint32_t)(call >> 32)};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u1 = (double)(((uint64_t)(c[0] >> 5) << 26) | (c[1] >> 6)) * 0x1p-53;
+    const double u2 = (double)(((uint64_t)(c[2] >> 5) << 26) | (c[3] >> 6)) * 0x1p-53;
+    const float rad = sqrtf(-2.0f * logf((float)(1.0 - u1)));
+    float sn, cs;
+    sincosf(6.2831853071795865f * (float)u2, &sn, &cs);
+    g0 = rad * cs;
+    g1 = rad * sn;
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin; launches over 8*ceil(n/8)
+// workgroups map XCD x to the contiguous items [x*per, (x+1)*per), so the
+// items of one series (which read the same spectra / U, V chunks) share that
+// XCD's L2.  Returns -1 for the padding workgroups.
+constexpr int N_XCD = 8;
+__device__ __forceinline__ int xcd_item(int n_items)
+{
+    const int per = (n_items + N_XCD - 1) / N_XCD;
+    const int i = (int)(blockIdx.x % N_XCD) * per + (int)(blockIdx.x / N_XCD);
+    return i < n_items ? i : -1;
+}
+static inline unsigned xcd_grid(int n_items) { return (unsigned)(((n_items + N_XCD - 1) / N_XCD) * N_XCD); }
+
 template <typename T> __device__ __forceinline__ void block_sum2(double &a, double &b, double *red)
 {
     // 256-thread sum of two values; result broadcast to every thread
@@ -149,17 +178,19 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_spectra(const ssa
 // out[o][i] = sum_p sum_j seg_(o,p)[i + j] in_p[j]  via  T+( X_(o,p) . T+(in_p) ).
 // ---------------------------------------------------------------------------
 template <typename T, int LOGN, int GEN>
-__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_matvec(const ssa_desc *__restrict__ D,
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (sizeof(T) == 4 && !GEN ? 8 : 4)) void k_matvec(const ssa_desc *__restrict__ D,
                                                                      const ssa_item *__restrict__ items, T *bulk,
                                                                      const C2<T> *__restrict__ tw, int dir,
-                                                                     int in_panel, int out_panel)
+                                                                     int in_panel, int out_panel, int n_items)
 {
     using G = FftGeom<T, LOGN>;
     constexpr int E = G::E, NT = G::NT;
     constexpr bool SPLIT = G::SPLIT;
     __shared__ T lre[G::LDS];
     __shared__ T lim[SPLIT ? 1 : G::LDS];
-    const ssa_item it = items[blockIdx.x];
+    const int item = xcd_item(n_items);
+    if (item < 0) return;
+    const ssa_item it = items[item];
     const ssa_desc d = D[it.s];
     const int c0 = 2 * it.x, c1 = c0 + 1;
     const bool two = c1 < d.b;
@@ -188,10 +219,13 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_matvec(const ssa_
 #pragma unroll
                     for (int e = 0; e < E; e++) {
                         const int q = threadIdx.x + e * NT;
-                        double g0 = 0, g1 = 0;
-                        if (q < len) gauss_pair(d.seed, d.call, row0 + q, it.x, g0, g1);
-                        z[e].x = (T)g0;
-                        z[e].y = two ? (T)g1 : (T)0;
+                        T g0 = 0, g1 = 0;
+                        if (q < len) {
+                            if constexpr (sizeof(T) == 4) gauss_pair_f(d.seed, d.call, row0 + q, it.x, g0, g1);
+                            else gauss_pair(d.seed, d.call, row0 + q, it.x, g0, g1);
+                        }
+                        z[e].x = g0;
+                        z[e].y = two ? g1 : (T)0;
                     }
                 } else {
                     const T *a0 = pin.p + (long long)c0 * pin.ld + row0;
@@ -477,6 +511,104 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
         }
     }
     if (!USE_M && threadIdx.x < b) out[b * b + threadIdx.x] = rcs[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// First-pass Cholesky in one wave per series, the b x b matrix in registers
+// (b <= NB*16 <= 64): lane j holds column j, slot i = A[i][j].  Row k of the
+// factor is broadcast with uniform readlanes, so no LDS and no barriers.
+// Same definition as k_chol pass 1: the Gram partials summed in tile order,
+// pivots below tol * (original diagonal) mark dead columns (zero row of R,
+// zero row and column of R^-1).  R^-1 is formed in place by rows (X R = I):
+// lane i computes row i of X while lane j's slots below its diagonal still
+// hold column j of R until step j has read them.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rdlane(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+template <int NB>
+__global__ __launch_bounds__(64) void k_chol1(const ssa_desc *D, const ssa_item *items, double *small, int which,
+                                              double tol)
+{
+    constexpr int B16 = NB * 16;
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int b = d.b, ln = threadIdx.x;
+    const int tiles = which == SSA_PANEL_Y ? d.tilesY : d.tilesZ;
+    const int stride = b * b + b;
+    const double *part = small + d.off_part;
+    double a[B16];
+#pragma unroll
+    for (int i = 0; i < B16; i++) a[i] = 0.0;
+    double dg = 0.0, cy = 0.0;
+    if (ln < b) {
+        // 16 slots per sweep over the tiles (bounded loads in flight)
+#pragma unroll
+        for (int h = 0; h < NB; h++) {
+            for (int t = 0; t < tiles; t++) {
+                const double *pt = part + (long long)t * stride;
+#pragma unroll
+                for (int i = 16 * h; i < 16 * h + 16; i++)
+                    if (i < b) a[i] += pt[i * b + ln];
+                if (h == 0) {
+                    dg += pt[ln * b + ln];
+                    cy += pt[b * b + ln];
+                }
+            }
+        }
+    }
+    double *misc = slot(small, d, SSA_SLOT_MISC);
+    if (which == SSA_PANEL_Y && ln < b) misc[SSA_MISC_COLSUMY + ln] = cy;
+    int ndead = 0;
+    bool mydead = true;
+    // (constant trip counts with uniform guards: the loops unroll fully and
+    // every slot index is static, so a[] stays in registers)
+#pragma unroll
+    for (int k = 0; k < B16; k++) {
+        if (k < b) {
+            const double p = rdlane(a[k], k), d0 = rdlane(dg, k);
+            const bool dead = !(p > tol * d0) || !(d0 > 0.0);
+            const double piv = dead ? 0.0 : sqrt(p);
+            ndead += dead;
+            if (ln == k) mydead = dead;
+            if (ln >= k) a[k] = dead ? 0.0 : (ln == k ? piv : a[k] / piv);
+#pragma unroll
+            for (int i = k + 1; i < B16; i++)
+                if (i < b) a[i] = fma(-rdlane(a[k], i), a[k], a[i]);
+        }
+    }
+    double *R1 = slot(small, d, SSA_SLOT_R1), *R1i = slot(small, d, SSA_SLOT_R1INV);
+    if (ln < b) {
+#pragma unroll
+        for (int i = 0; i < B16; i++)
+            if (i < b) R1[(long long)ln * SSA_BMAX + i] = (i <= ln) ? a[i] : 0.0;
+    }
+    // X = R^-1 by rows: X[i][j] = (delta_ij - sum_{l=i}^{j-1} X[i][l] R[l][j]) / R[j][j]
+#pragma unroll
+    for (int j = 0; j < B16; j++) {
+        if (j >= b) continue;
+        const double rjj = rdlane(a[j], j);
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < j; l++) s = fma(-a[l], rdlane(a[l], j), s);
+        const bool deadj = !(rjj > 0.0);
+        if (ln <= j) a[j] = (deadj || mydead) ? 0.0 : (ln == j ? 1.0 / rjj : s / rjj);
+        // column j of R is consumed: lane j's slots below the diagonal become X[j][l] = 0
+        if (ln == j) {
+#pragma unroll
+            for (int l = 0; l < j; l++) a[l] = 0.0;
+        }
+    }
+    if (ln < b) {
+#pragma unroll
+        for (int j = 0; j < B16; j++)
+            if (j < b) R1i[(long long)j * SSA_BMAX + ln] = a[j];
+    }
+    if (ln == 0) misc[SSA_MISC_DEAD] = ndead;
 }
 
 // ---------------------------------------------------------------------------
@@ -806,20 +938,22 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
 // (the imaginary part of the self-convolution of a + ib is 2 a (*) b).
 // ---------------------------------------------------------------------------
 template <typename T, int LOGN>
-__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_recon(const ssa_desc *__restrict__ D,
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (sizeof(T) == 4 ? 8 : 4)) void k_recon(const ssa_desc *__restrict__ D,
                                                               const ssa_item *__restrict__ items, T *bulk,
                                                               const double *__restrict__ small,
                                                               const int4 *__restrict__ gdesc,
                                                               const int *__restrict__ glist,
                                                               const C2<T> *__restrict__ tw, int scale_sigma,
-                                                              int limit_gsize)
+                                                              int limit_gsize, int n_items)
 {
     using G = FftGeom<T, LOGN>;
     constexpr int E = G::E, NT = G::NT;
     constexpr bool SPLIT = G::SPLIT;
     __shared__ T lre[G::LDS];
     __shared__ T lim[SPLIT ? 1 : G::LDS];
-    const ssa_item it = items[blockIdx.x];
+    const int item = xcd_item(n_items);
+    if (item < 0) return;
+    const ssa_item it = items[item];
     const ssa_desc d = D[it.s];
     const int4 gd = gdesc[it.x];
     const int a = it.y, sk = it.z, m = gd.z;
@@ -1036,8 +1170,8 @@ int ssa_dev_matvec(int prec, int lgn, int dir, int gen_omega, int in_panel, int 
     if (!lgn_ok(prec, lgn)) return -2;
     hipStream_t st = (hipStream_t)stream;
 #define LAUNCH(T, L, GEN)                                                                                     \
-    hipLaunchKernelGGL((k_matvec<T, L, GEN>), dim3(n_items), dim3(FftGeom<T, L>::NT), 0, st, D, items,        \
-                       (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel)
+    hipLaunchKernelGGL((k_matvec<T, L, GEN>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L>::NT), 0, st, D,     \
+                       items, (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel, n_items)
 #define CASE(L)                                                                                               \
     case L:                                                                                                   \
         if (prec == SSA_PREC_F64) {                                                                           \
@@ -1101,6 +1235,16 @@ int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_ite
         if (e != hipSuccess) return (int)e;
         attr = 1;
     }
+    if (pass == 1 && bmax <= 64) {
+        hipStream_t st = (hipStream_t)stream;
+        switch (SSA_NB(bmax)) {
+        case 1: hipLaunchKernelGGL(k_chol1<1>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, 1e-13); break;
+        case 2: hipLaunchKernelGGL(k_chol1<2>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, 1e-13); break;
+        case 3: hipLaunchKernelGGL(k_chol1<3>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, 1e-13); break;
+        default: hipLaunchKernelGGL(k_chol1<4>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, 1e-13); break;
+        }
+        return (int)hipGetLastError();
+    }
     hipLaunchKernelGGL(k_chol, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items, small, panel, pass,
                        1e-13);
     return (int)hipGetLastError();
@@ -1160,13 +1304,13 @@ int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, const ssa
 #define CASE(L)                                                                                               \
     case L:                                                                                                   \
         if (prec == SSA_PREC_F64)                                                                             \
-            hipLaunchKernelGGL((k_recon<double, L>), dim3(n_items),                           \
-                               dim3(FftGeom<double, L>::NT), 0, st, D, items, (double *)bulk, \
-                               small, (const int4 *)gdesc, glist, (const C2<double> *)tw, scale_sigma, limit_gsize);       \
+            hipLaunchKernelGGL((k_recon<double, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<double, L>::NT), 0, \
+                               st, D, items, (double *)bulk, small, (const int4 *)gdesc, glist,               \
+                               (const C2<double> *)tw, scale_sigma, limit_gsize, n_items);                    \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_recon<float, L>), dim3(n_items), dim3(FftGeom<float, L>::NT), 0, st, D,     \
-                               items, (float *)bulk, small, (const int4 *)gdesc, glist,                       \
-                               (const C2<float> *)tw, scale_sigma, limit_gsize);                              \
+            hipLaunchKernelGGL((k_recon<float, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<float, L>::NT), 0,   \
+                               st, D, items, (float *)bulk, small, (const int4 *)gdesc, glist,                \
+                               (const C2<float> *)tw, scale_sigma, limit_gsize, n_items);                     \
         break;
     switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
 #undef CASE
