@@ -19,7 +19,7 @@ extern "C" {
 #endif
 
 #define SSA_BMAX 80           /* max block width k + p */
-#define SSA_GRAM_ROWS 1024    /* panel rows per Gram / apply work item */
+#define SSA_GRAM_ROWS 2048    /* panel rows per Gram / apply work item */
 #define SSA_NMAX_LOG 16       /* twiddle table length 2^16 */
 
 enum { SSA_PREC_F64 = 0, SSA_PREC_F32 = 1 };
@@ -53,9 +53,13 @@ typedef struct ssa_desc {
     long long out_cap;        /* output capacity in T elements */
 } ssa_desc;
 
-/* FP64 small-matrix slots, each SSA_BMAX*SSA_BMAX doubles, relative to off_small */
+/* FP64 small-matrix slots, each SSA_BMAX*SSA_BMAX doubles, relative to off_small.
+ * CholeskyQR pass 1 writes R1, R1INV; pass 2 writes R2, R2INV; the products
+ * R = R2 R1 (R factor of the last CholeskyQR2 of Z, i.e. of B) and
+ * RINV = R1INV R2INV (of the last CholeskyQR2 of Y, applied to form Q) are
+ * formed only where they are used. */
 enum {
-    SSA_SLOT_R1 = 0, SSA_SLOT_R1INV, SSA_SLOT_RY, SSA_SLOT_RINVY, SSA_SLOT_RZ, SSA_SLOT_RINVZ,
+    SSA_SLOT_R1 = 0, SSA_SLOT_R1INV, SSA_SLOT_R2, SSA_SLOT_R2INV, SSA_SLOT_R, SSA_SLOT_RINV,
     SSA_SLOT_W, SSA_SLOT_WK, SSA_SLOT_WKS, SSA_SLOT_MISC, SSA_NSLOTS
 };
 /* SSA_SLOT_MISC layout (doubles): [0, BMAX) colsum of raw Y, [BMAX, 2BMAX) colsum of Q,
@@ -93,6 +97,9 @@ int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_ite
                  double *small, void *stream);
 int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, int nb, const ssa_desc *D,
                   const ssa_item *items, int n_items, void *bulk, double *small, void *stream);
+/* C = A B of the upper-triangular b x b slot matrices (FP64 MFMA), one workgroup per series */
+int ssa_dev_trmm(int nb, int slot_a, int slot_b, int slot_c, const ssa_desc *D, const ssa_item *items, int n_items,
+                 double *small, void *stream);
 int ssa_dev_svd(int bmax, const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream);
 /* reconstruction: items (s, gid, row chunk o, col chunk sk); groups: gdesc[gid] = {start, count},
  * glist = component indices; scale_sigma 1: sum_g sigma_g u_g (*) v_g, 0: u (*) v */

@@ -51,7 +51,7 @@ struct ssa_engine {
     long long ditems_cap;
     ssa_item *hitems;
     long long hitems_n, hitems_cap;
-    span spec[NLG], mvf[NLG], mva[NLG], tY[NBW], tZ[NBW], ser;   /* tY/tZ by nb */
+    span spec[NLG], mvf[NLG], mva[NLG], tY[NBW], tZ[NBW], ser, ser_nb[NBW];   /* tY/tZ/ser_nb by nb */
     span rec_step[NLG], olap_step;
     int lists_dirty, desc_dirty;
     int dyn_base;              /* first item of the per-call region */
@@ -259,6 +259,12 @@ static int rebuild_lists(ssa_engine *e)
     e->ser.off = e->hitems_n;
     for (int s = 0; s < e->count; s++) if (items_push(e, s, 0, 0, 0)) return -1;
     e->ser.n = (int)(e->hitems_n - e->ser.off);
+    for (int nb = 0; nb < NBW; nb++) {
+        e->ser_nb[nb].off = e->hitems_n;
+        for (int s = 0; s < e->count; s++)
+            if (e->hd[s].b > 0 && SSA_NB(e->hd[s].b) == nb && items_push(e, s, 0, 0, 0)) return -1;
+        e->ser_nb[nb].n = (int)(e->hitems_n - e->ser_nb[nb].off);
+    }
     /* batched step: group gid = s covering components 0..k-1 */
     for (int lg = 0; lg < NLG; lg++) {
         e->rec_step[lg].off = e->hitems_n;
@@ -669,12 +675,28 @@ static int chol_all(ssa_engine *e, int panel, int pass, int bm)
     return 0;
 }
 
-static int cholqr_factor(ssa_engine *e, int panel, int bm)
+/* the products of a CholeskyQR2 that a caller needs */
+enum { NEED_R = 1, NEED_RINV = 2 };
+
+static int trmm_all(ssa_engine *e, int sa, int sb, int sc)
+{
+    for (int nb = 1; nb < NBW; nb++) {
+        if (!e->ser_nb[nb].n) continue;
+        prof_begin(e, PH_CHOL, nb);
+        DEVCHK(ssa_dev_trmm(nb, sa, sb, sc, e->dd, it_ptr(e, e->ser_nb[nb]), e->ser_nb[nb].n, e->small, e->st));
+        prof_end(e);
+    }
+    return 0;
+}
+
+static int cholqr_factor(ssa_engine *e, int panel, int bm, int need)
 {
     DEVCHK(gram_all(e, panel, 0));
     DEVCHK(chol_all(e, panel, 1, bm));
     DEVCHK(gram_all(e, panel, 1));
     DEVCHK(chol_all(e, panel, 2, bm));
+    if (need & NEED_R) DEVCHK(trmm_all(e, SSA_SLOT_R2, SSA_SLOT_R1, SSA_SLOT_R));
+    if (need & NEED_RINV) DEVCHK(trmm_all(e, SSA_SLOT_R1INV, SSA_SLOT_R2INV, SSA_SLOT_RINV));
     return 0;
 }
 
@@ -687,8 +709,8 @@ static int orth(ssa_engine *e, int panel, int bm, int two_pass)
 {
     int slot;
     if (two_pass) {
-        DEVCHK(cholqr_factor(e, panel, bm));
-        slot = panel == SSA_PANEL_Y ? SSA_SLOT_RINVY : SSA_SLOT_RINVZ;
+        DEVCHK(cholqr_factor(e, panel, bm, NEED_RINV));
+        slot = SSA_SLOT_RINV;
     } else {
         DEVCHK(gram_all(e, panel, 0));
         DEVCHK(chol_all(e, panel, 1, bm));
@@ -710,7 +732,7 @@ static int launch_decompose(ssa_engine *e, int q, int injected)
         DEVCHK(orth(e, SSA_PANEL_Y, bm, all2 || it == q - 1));
     }
     DEVCHK(matvec_all(e, 1, 0, SSA_PANEL_Y, SSA_PANEL_Z));
-    DEVCHK(cholqr_factor(e, SSA_PANEL_Z, bm));
+    DEVCHK(cholqr_factor(e, SSA_PANEL_Z, bm, NEED_R));
     prof_begin(e, PH_SVD, 0);
     DEVCHK(ssa_dev_svd(bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
     prof_end(e);

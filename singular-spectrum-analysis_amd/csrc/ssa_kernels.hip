@@ -530,43 +530,45 @@ __device__ __forceinline__ double rdlane(double v, int lane)
     return __hiloint2double(hi, lo);
 }
 
+// Sum the Gram partials of a series in tile order: lane j gets column j
+// (slots i < b), its diagonal dg and its column sum cy.
 template <int NB>
-__global__ __launch_bounds__(64) void k_chol1(const ssa_desc *D, const ssa_item *items, double *small, int which,
-                                              double tol)
+__device__ __forceinline__ void gram_sum_regs(const double *part, int tiles, int b, int ln, double (&a)[NB * 16],
+                                              double &dg, double &cy)
 {
     constexpr int B16 = NB * 16;
-    const ssa_item it = items[blockIdx.x];
-    const ssa_desc d = D[it.s];
-    const int b = d.b, ln = threadIdx.x;
-    const int tiles = which == SSA_PANEL_Y ? d.tilesY : d.tilesZ;
     const int stride = b * b + b;
-    const double *part = small + d.off_part;
-    double a[B16];
 #pragma unroll
     for (int i = 0; i < B16; i++) a[i] = 0.0;
-    double dg = 0.0, cy = 0.0;
-    if (ln < b) {
-        // 16 slots per sweep over the tiles (bounded loads in flight)
+    dg = 0.0;
+    cy = 0.0;
+    if (ln >= b) return;
+    // 16 slots per sweep over the tiles (bounded loads in flight)
 #pragma unroll
-        for (int h = 0; h < NB; h++) {
-            for (int t = 0; t < tiles; t++) {
-                const double *pt = part + (long long)t * stride;
+    for (int h = 0; h < NB; h++) {
+        for (int t = 0; t < tiles; t++) {
+            const double *pt = part + (long long)t * stride;
 #pragma unroll
-                for (int i = 16 * h; i < 16 * h + 16; i++)
-                    if (i < b) a[i] += pt[i * b + ln];
-                if (h == 0) {
-                    dg += pt[ln * b + ln];
-                    cy += pt[b * b + ln];
-                }
+            for (int i = 16 * h; i < 16 * h + 16; i++)
+                if (i < b) a[i] += pt[i * b + ln];
+            if (h == 0) {
+                dg += pt[ln * b + ln];
+                cy += pt[b * b + ln];
             }
         }
     }
-    double *misc = slot(small, d, SSA_SLOT_MISC);
-    if (which == SSA_PANEL_Y && ln < b) misc[SSA_MISC_COLSUMY + ln] = cy;
-    int ndead = 0;
-    bool mydead = true;
-    // (constant trip counts with uniform guards: the loops unroll fully and
-    // every slot index is static, so a[] stays in registers)
+}
+
+// Cholesky A = R^T R in place (lane j: column j of R in slots 0..j).  Loops
+// have constant trip counts with uniform guards so that they unroll fully
+// and every slot index is static (a[] stays in registers).
+template <int NB>
+__device__ __forceinline__ void chol_regs(double (&a)[NB * 16], double dg, int b, int ln, double tol, int &ndead,
+                                          bool &mydead)
+{
+    constexpr int B16 = NB * 16;
+    ndead = 0;
+    mydead = true;
 #pragma unroll
     for (int k = 0; k < B16; k++) {
         if (k < b) {
@@ -581,13 +583,15 @@ __global__ __launch_bounds__(64) void k_chol1(const ssa_desc *D, const ssa_item 
                 if (i < b) a[i] = fma(-rdlane(a[k], i), a[k], a[i]);
         }
     }
-    double *R1 = slot(small, d, SSA_SLOT_R1), *R1i = slot(small, d, SSA_SLOT_R1INV);
-    if (ln < b) {
-#pragma unroll
-        for (int i = 0; i < B16; i++)
-            if (i < b) R1[(long long)ln * SSA_BMAX + i] = (i <= ln) ? a[i] : 0.0;
-    }
-    // X = R^-1 by rows: X[i][j] = (delta_ij - sum_{l=i}^{j-1} X[i][l] R[l][j]) / R[j][j]
+}
+
+// X = R^-1 in place, by rows (X R = I): afterwards lane i holds row i of X
+// (slots j >= i, zero below).  Lane j's slots below its diagonal keep column
+// j of R until step j has read them.
+template <int NB>
+__device__ __forceinline__ void inv_rows(double (&a)[NB * 16], int b, int ln, bool mydead)
+{
+    constexpr int B16 = NB * 16;
 #pragma unroll
     for (int j = 0; j < B16; j++) {
         if (j >= b) continue;
@@ -597,25 +601,99 @@ __global__ __launch_bounds__(64) void k_chol1(const ssa_desc *D, const ssa_item 
         for (int l = 0; l < j; l++) s = fma(-a[l], rdlane(a[l], j), s);
         const bool deadj = !(rjj > 0.0);
         if (ln <= j) a[j] = (deadj || mydead) ? 0.0 : (ln == j ? 1.0 / rjj : s / rjj);
-        // column j of R is consumed: lane j's slots below the diagonal become X[j][l] = 0
         if (ln == j) {
 #pragma unroll
             for (int l = 0; l < j; l++) a[l] = 0.0;
         }
     }
+}
+
+// Stage the b x b column-major slot matrix M (ld SSA_BMAX) into Ms[l*MP + c]
+// = M[l][c] (zero beyond b), 64 threads.
+template <int NB> __device__ __forceinline__ void stage_rows(const double *M, int b, int ln, double *Ms)
+{
+    constexpr int B16 = NB * 16, MP = B16 + 1;
+    for (int c = 0; c < B16; c++)
+        if (ln < B16) Ms[ln * MP + c] = (ln < b && c < b) ? M[(long long)c * SSA_BMAX + ln] : 0.0;
+}
+
+// One wave per series: R = chol(G) and R^-1 of the summed Gram into the
+// pass's slots (pass 1: R1, R1INV, colsum(Y) for Y, the dead-column count;
+// pass 2: R2, R2INV).  Dead pivots as in k_chol.
+template <int NB>
+__global__ __launch_bounds__(64) void k_chol_w(const ssa_desc *D, const ssa_item *items, double *small, int which,
+                                               int pass, double tol)
+{
+    constexpr int B16 = NB * 16;
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int b = d.b, ln = threadIdx.x;
+    const int tiles = which == SSA_PANEL_Y ? d.tilesY : d.tilesZ;
+    double a[B16];
+    double dg, cy;
+    gram_sum_regs<NB>(small + d.off_part, tiles, b, ln, a, dg, cy);
+    double *misc = slot(small, d, SSA_SLOT_MISC);
+    if (pass == 1 && which == SSA_PANEL_Y && ln < b) misc[SSA_MISC_COLSUMY + ln] = cy;
+    int ndead;
+    bool mydead;
+    chol_regs<NB>(a, dg, b, ln, tol, ndead, mydead);
+    double *Ro = slot(small, d, pass == 1 ? SSA_SLOT_R1 : SSA_SLOT_R2);
+    double *Rio = slot(small, d, pass == 1 ? SSA_SLOT_R1INV : SSA_SLOT_R2INV);
+    if (ln < b) {
+#pragma unroll
+        for (int i = 0; i < B16; i++)
+            if (i < b) Ro[(long long)ln * SSA_BMAX + i] = (i <= ln) ? a[i] : 0.0;
+    }
+    inv_rows<NB>(a, b, ln, mydead);
     if (ln < b) {
 #pragma unroll
         for (int j = 0; j < B16; j++)
-            if (j < b) R1i[(long long)j * SSA_BMAX + ln] = a[j];
+            if (j < b) Rio[(long long)j * SSA_BMAX + ln] = a[j];
     }
-    if (ln == 0) misc[SSA_MISC_DEAD] = ndead;
+    if (pass == 1 && ln == 0) misc[SSA_MISC_DEAD] = ndead;
 }
 
 // ---------------------------------------------------------------------------
-// Cholesky of the summed Gram (one workgroup per series).
+// C = A B for upper-triangular b x b slot matrices (column-major, ld
+// SSA_BMAX), FP64 MFMA.  One workgroup (4 waves) per series; block (I, J) of
+// C needs k in [16 I, 16 J + 16) only; blocks below the diagonal are written
+// as zeros so C is a clean upper-triangular matrix.
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(256) void k_trmm(const ssa_desc *D, const ssa_item *items, double *small, int sa,
+                                              int sb, int sc)
+{
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int b = d.b;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, cl = l & 15, kq = l >> 4;
+    const double *A = small + d.off_small + (long long)sa * SSA_BMAX * SSA_BMAX;
+    const double *B = small + d.off_small + (long long)sb * SSA_BMAX * SSA_BMAX;
+    double *C = small + d.off_small + (long long)sc * SSA_BMAX * SSA_BMAX;
+    for (int m = w; m < NB * NB; m += 4) {
+        const int I = m / NB, J = m - I * NB;
+        d4v acc = {0, 0, 0, 0};
+        if (I <= J) {
+            for (int k0 = 16 * I; k0 < 16 * J + 16; k0 += 4) {
+                const int ra = 16 * I + cl, ka = k0 + kq;     // A[ra][ka]
+                const int kb = k0 + kq, cb = 16 * J + cl;     // B[kb][cb]
+                const double av = (ra < b && ka < b) ? A[(long long)ka * SSA_BMAX + ra] : 0.0;
+                const double bv = (kb < b && cb < b) ? B[(long long)cb * SSA_BMAX + kb] : 0.0;
+                acc = mfma64(av, bv, acc);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = 16 * I + kq + 4 * v, j = 16 * J + cl;
+            if (i < b && j < b) C[(long long)j * SSA_BMAX + i] = acc[v];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky of the summed Gram (one workgroup per series; widths above 64).
 // pass 1: R1 = chol(G), R1^-1 -> slots; panel Y also stores its column sums.
-// pass 2: R2 = chol(G2); R = R2 R1, R^-1 = R1^-1 R2^-1 -> RY/RINVY or RZ/RINVZ;
-//         panel Y stores colsum(Q) = colsum(Y) R^-1.
+// pass 2: R2 = chol(G2), R2^-1 -> slots.
 // A pivot below tol * (original diagonal) marks a numerically dependent
 // column: its row of R and column of R^-1 are zero (the column of Q is zero).
 // ---------------------------------------------------------------------------
@@ -690,42 +768,17 @@ __global__ __launch_bounds__(256) void k_chol(const ssa_desc *D, const ssa_item 
     }
     __syncthreads();
     double *misc = slot(small, d, SSA_SLOT_MISC);
-    if (pass == 1) {
-        double *R1 = slot(small, d, SSA_SLOT_R1), *R1i = slot(small, d, SSA_SLOT_R1INV);
-        for (int idx = threadIdx.x; idx < b * b; idx += 256) {
-            const int j = idx / b, i = idx - j * b;   // column-major store
-            R1[(long long)j * SSA_BMAX + i] = A[i * LDP + j];
-            R1i[(long long)j * SSA_BMAX + i] = X[i * LDP + j];
-        }
-        if (threadIdx.x == 0) {
-            int nd = 0;
-            for (int j = 0; j < b; j++) nd += dead[j];
-            misc[SSA_MISC_DEAD] = nd;
-        }
-        return;
-    }
-    const double *R1 = slot(small, d, SSA_SLOT_R1), *R1i = slot(small, d, SSA_SLOT_R1INV);
-    double *Ro = slot(small, d, which == SSA_PANEL_Y ? SSA_SLOT_RY : SSA_SLOT_RZ);
-    double *Rio = slot(small, d, which == SSA_PANEL_Y ? SSA_SLOT_RINVY : SSA_SLOT_RINVZ);
-    // R = R2 R1 (upper x upper), Rinv = R1inv X
+    double *Ro = slot(small, d, pass == 1 ? SSA_SLOT_R1 : SSA_SLOT_R2);
+    double *Rio = slot(small, d, pass == 1 ? SSA_SLOT_R1INV : SSA_SLOT_R2INV);
     for (int idx = threadIdx.x; idx < b * b; idx += 256) {
-        const int j = idx / b, i = idx - j * b;
-        double s = 0, t = 0;
-        if (i <= j) {
-            for (int l = i; l <= j; l++) {
-                s += A[i * LDP + l] * R1[(long long)j * SSA_BMAX + l];
-                t += R1i[(long long)l * SSA_BMAX + i] * X[l * LDP + j];
-            }
-        }
-        Ro[(long long)j * SSA_BMAX + i] = s;
-        Rio[(long long)j * SSA_BMAX + i] = t;
+        const int j = idx / b, i = idx - j * b;   // column-major store
+        Ro[(long long)j * SSA_BMAX + i] = A[i * LDP + j];
+        Rio[(long long)j * SSA_BMAX + i] = X[i * LDP + j];
     }
-    __syncthreads();
-    if (which == SSA_PANEL_Y && threadIdx.x < b) {
-        const int j = threadIdx.x;
-        double s = 0;
-        for (int i = 0; i <= j; i++) s += misc[SSA_MISC_COLSUMY + i] * Rio[(long long)j * SSA_BMAX + i];
-        misc[SSA_MISC_COLSUMQ + j] = s;
+    if (pass == 1 && threadIdx.x == 0) {
+        int nd = 0;
+        for (int j = 0; j < b; j++) nd += dead[j];
+        misc[SSA_MISC_DEAD] = nd;
     }
 }
 
@@ -807,9 +860,11 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply(const ssa_desc *D, c
 }
 
 // ---------------------------------------------------------------------------
-// one-sided Jacobi SVD of R (b x b, FP64) with a round-robin parallel order:
-// R W = U S, W accumulated.  Sorted descending; signs chosen so that
-// colsum(Q) . W_i >= 0, i.e. sum(U[:,i]) >= 0 for U = Q W.
+// SVD of R (b x b, FP64) by one-sided Jacobi on R^T with a round-robin
+// parallel order: R^T V = W S, so the converged columns of R^T, normalised,
+// are the right singular vectors W of R (R W = U S) and no rotation has to be
+// accumulated.  Sorted descending; signs chosen so that colsum(Q) . W_i >= 0,
+// i.e. sum(U[:,i]) >= 0 for U = Q W.
 // Outputs: WK = W[:, :k] (signed), WKS = WK S^-1, sigma, lambda, total variance.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *items, double *small)
@@ -818,18 +873,17 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     const int bq = D[items[blockIdx.x].s].b;
     const int LDP = bq + 1;                // padded column length
     double *A = sm;                        // column j at A + j*LDP (rows 0..b-1), bp columns
-    double *W = sm + ((bq + 1) & ~1) * LDP; // same layout
     __shared__ int perm[SSA_BMAX + 1];
     __shared__ double S[SSA_BMAX + 1];
     __shared__ int rotated;
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const int b = d.b, bp = (b + 1) & ~1;
-    const double *R = slot(small, d, SSA_SLOT_RZ);
+    const double *R = slot(small, d, SSA_SLOT_R);
+    // A = R^T: column j is row j of R (R column-major, ld SSA_BMAX)
     for (int idx = threadIdx.x; idx < bp * b; idx += 256) {
-        const int j = idx / b, i = idx - j * b;
-        A[j * LDP + i] = (j < b) ? R[(long long)j * SSA_BMAX + i] : 0.0;
-        W[j * LDP + i] = (i == j) ? 1.0 : 0.0;
+        const int i = idx / bp, j = idx - i * bp;
+        A[j * LDP + i] = (j < b) ? R[(long long)i * SSA_BMAX + j] : 0.0;
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -844,17 +898,32 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
         int any_local = 0;
         for (int round = 0; round < bp - 1; round++) {
             for (int pi = grp; pi < bp / 2; pi += 32) {
+                // circle-method pairing without integer division
                 int p, q;
                 if (pi == 0) { p = round; q = bp - 1; }
                 else {
-                    p = (round + pi) % (bp - 1);
-                    q = (round - pi + (bp - 1)) % (bp - 1);
+                    p = round + pi;
+                    if (p >= bp - 1) p -= bp - 1;
+                    q = round - pi;
+                    if (q < 0) q += bp - 1;
                 }
                 if (p > q) { int t = p; p = q; q = t; }
+                // rows gl + 8m of both columns (and of W), all loads issued up front
+                constexpr int MB = SSA_BMAX / 8;
+                double x[MB], y[MB];
+#pragma unroll
+                for (int m = 0; m < MB; m++) {
+                    const int i = gl + 8 * m;
+                    const bool in = i < b;
+                    x[m] = in ? A[p * LDP + i] : 0.0;
+                    y[m] = in ? A[q * LDP + i] : 0.0;
+                }
                 double al = 0, be = 0, ga = 0;
-                for (int i = gl; i < b; i += 8) {
-                    const double x = A[p * LDP + i], y = A[q * LDP + i];
-                    al += x * x; be += y * y; ga += x * y;
+#pragma unroll
+                for (int m = 0; m < MB; m++) {
+                    al += x[m] * x[m];
+                    be += y[m] * y[m];
+                    ga += x[m] * y[m];
                 }
                 for (int o = 4; o > 0; o >>= 1) {
                     al += __shfl_xor(al, o);
@@ -865,13 +934,13 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                     const double zeta = (be - al) / (2.0 * ga);
                     const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-                    for (int i = gl; i < b; i += 8) {
-                        const double x = A[p * LDP + i], y = A[q * LDP + i];
-                        A[p * LDP + i] = c * x - s * y;
-                        A[q * LDP + i] = s * x + c * y;
-                        const double wx = W[p * LDP + i], wy = W[q * LDP + i];
-                        W[p * LDP + i] = c * wx - s * wy;
-                        W[q * LDP + i] = s * wx + c * wy;
+#pragma unroll
+                    for (int m = 0; m < MB; m++) {
+                        const int i = gl + 8 * m;
+                        if (i < b) {
+                            A[p * LDP + i] = c * x[m] - s * y[m];
+                            A[q * LDP + i] = s * x[m] + c * y[m];
+                        }
                     }
                     any_local = 1;
                 }
@@ -901,17 +970,28 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     __syncthreads();
     double *misc = slot(small, d, SSA_SLOT_MISC);
     double *WK = slot(small, d, SSA_SLOT_WK), *WKS = slot(small, d, SSA_SLOT_WKS);
+    // colsum(Q) = colsum(Y) R_Y^-1 of the final orthonormalisation of Y
+    if (threadIdx.x < b) {
+        const double *Ri = slot(small, d, SSA_SLOT_RINV);
+        const int j = threadIdx.x;
+        double cq = 0.0;
+        for (int i = 0; i <= j; i++) cq += misc[SSA_MISC_COLSUMY + i] * Ri[(long long)j * SSA_BMAX + i];
+        misc[SSA_MISC_COLSUMQ + j] = cq;
+    }
+    __syncthreads();
     const double smax = S[perm[0]];
     for (int j = wave; j < d.k; j += nwave) {
         const int src = perm[j];
         double dot = 0;
-        for (int i = lane; i < b; i += 64) dot += misc[SSA_MISC_COLSUMQ + i] * W[src * LDP + i];
+        for (int i = lane; i < b; i += 64) dot += misc[SSA_MISC_COLSUMQ + i] * A[src * LDP + i];
         for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
         const double sg = dot < 0.0 ? -1.0 : 1.0;
         const double sv = S[src];
         const double inv = (sv > 1e-300 && sv > 1e-15 * smax) ? 1.0 / sv : 0.0;
+        // right singular vector of R: the normalised converged column of R^T
+        const double wn = sv > 1e-300 ? sg / sv : 0.0;
         for (int i = lane; i < b; i += 64) {
-            const double w = sg * W[src * LDP + i];
+            const double w = wn * A[src * LDP + i];
             WK[(long long)j * SSA_BMAX + i] = w;
             WKS[(long long)j * SSA_BMAX + i] = w * inv;
         }
@@ -1235,13 +1315,14 @@ int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_ite
         if (e != hipSuccess) return (int)e;
         attr = 1;
     }
-    if (pass == 1 && bmax <= 64) {
+    if (bmax <= 64) {
+        // one wave per series, the matrix in registers
         hipStream_t st = (hipStream_t)stream;
         switch (SSA_NB(bmax)) {
-        case 1: hipLaunchKernelGGL(k_chol1<1>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, 1e-13); break;
-        case 2: hipLaunchKernelGGL(k_chol1<2>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, 1e-13); break;
-        case 3: hipLaunchKernelGGL(k_chol1<3>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, 1e-13); break;
-        default: hipLaunchKernelGGL(k_chol1<4>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, 1e-13); break;
+        case 1: hipLaunchKernelGGL(k_chol_w<1>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, 1e-13); break;
+        case 2: hipLaunchKernelGGL(k_chol_w<2>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, 1e-13); break;
+        case 3: hipLaunchKernelGGL(k_chol_w<3>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, 1e-13); break;
+        default: hipLaunchKernelGGL(k_chol_w<4>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, 1e-13); break;
         }
         return (int)hipGetLastError();
     }
@@ -1279,17 +1360,26 @@ int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, int nb, con
     return (int)hipGetLastError();
 }
 
+int ssa_dev_trmm(int nb, int slot_a, int slot_b, int slot_c, const ssa_desc *D, const ssa_item *items, int n_items,
+                 double *small, void *stream)
+{
+    if (n_items <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    switch (nb) {
+    case 1: hipLaunchKernelGGL(k_trmm<1>, dim3(n_items), dim3(256), 0, st, D, items, small, slot_a, slot_b, slot_c); break;
+    case 2: hipLaunchKernelGGL(k_trmm<2>, dim3(n_items), dim3(256), 0, st, D, items, small, slot_a, slot_b, slot_c); break;
+    case 3: hipLaunchKernelGGL(k_trmm<3>, dim3(n_items), dim3(256), 0, st, D, items, small, slot_a, slot_b, slot_c); break;
+    case 4: hipLaunchKernelGGL(k_trmm<4>, dim3(n_items), dim3(256), 0, st, D, items, small, slot_a, slot_b, slot_c); break;
+    case 5: hipLaunchKernelGGL(k_trmm<5>, dim3(n_items), dim3(256), 0, st, D, items, small, slot_a, slot_b, slot_c); break;
+    default: return -1;
+    }
+    return (int)hipGetLastError();
+}
+
 int ssa_dev_svd(int bmax, const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream)
 {
     if (n_items <= 0) return 0;
-    const size_t lds = 2 * (size_t)((bmax + 1) & ~1) * (bmax + 1) * sizeof(double);
-    static int attr = 0;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_svd, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(2 * (SSA_BMAX + 1) * LDP * sizeof(double)));
-        if (e != hipSuccess) return (int)e;
-        attr = 1;
-    }
+    const size_t lds = (size_t)((bmax + 1) & ~1) * (bmax + 1) * sizeof(double);
     hipLaunchKernelGGL(k_svd, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items, small);
     return (int)hipGetLastError();
 }
