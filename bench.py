@@ -87,7 +87,7 @@ def roofline_of(batch, plans, Ns, Ls, s):
     return {
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_matvec<{'float' if s == 4 else 'double'}, {lg}, 0>",
+                     "kernel": f"k_matvec<{'float' if s == 4 else 'double'}, {lg}>",
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_ms, 4),
                      "series_in_launch": len(sel), "launches_timed": len(v)},
         "phase_ms": {k: round(v, 3) for k, v in sorted(phase_tot.items(), key=lambda kv: -kv[1])},

@@ -85,11 +85,12 @@ enum { SSA_PANEL_Y = 0, SSA_PANEL_Z = 1, SSA_PANEL_U = 2, SSA_PANEL_V = 3, SSA_P
 int ssa_dev_twiddles(int prec, void **table_out, void *stream);
 int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items, int n_items,
                     void *bulk, const void *tw, void *stream);
-/* dir 0: out = H * in (in: K-dim panel, out: L-dim panel); dir 1: out = H^T * in.
- * gen_omega: the input is the Philox test matrix, generated on the fly. */
-int ssa_dev_matvec(int prec, int lgn, int dir, int gen_omega, int in_panel, int out_panel,
+/* dir 0: out = H * in (in: K-dim panel, out: L-dim panel); dir 1: out = H^T * in. */
+int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel,
                    const ssa_desc *D, const ssa_item *items, int n_items,
                    void *bulk, const void *tw, void *stream);
+/* Philox Gaussian test matrix into panel Z; items (s, row tile) of the K-dim panel */
+int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, void *stream);
 /* Gram / apply: every panel of the launch has at most nb*16 columns (1 <= nb <= 5) */
 int ssa_dev_gram(int prec, int panel, int use_m, int nb, const ssa_desc *D, const ssa_item *items, int n_items,
                  void *bulk, double *small, void *stream);

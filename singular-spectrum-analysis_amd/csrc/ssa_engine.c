@@ -76,7 +76,7 @@ struct ssa_engine {
 
 /* profiler phases */
 enum { PH_SPECTRA = 0, PH_MV_FWD, PH_MV_ADJ, PH_GRAM, PH_CHOL, PH_APPLY, PH_SVD, PH_UV, PH_RECON, PH_OLAP, PH_FORECAST,
-       PH_MV_GEN, PH_N };
+       PH_OMEGA, PH_N };
 
 static void prof_begin(struct ssa_engine *e, int phase, int lgn)
 {
@@ -626,14 +626,26 @@ static int bmax_all(const ssa_engine *e)
     return m;
 }
 
-static int matvec_all(ssa_engine *e, int dir, int gen, int in_panel, int out_panel)
+static int matvec_all(ssa_engine *e, int dir, int in_panel, int out_panel)
 {
     for (int lg = 6; lg < NLG; lg++) {
         span sp = dir == 0 ? e->mvf[lg] : e->mva[lg];
         if (!sp.n) continue;
-        prof_begin(e, gen ? PH_MV_GEN : dir == 0 ? PH_MV_FWD : PH_MV_ADJ, lg);
-        DEVCHK(ssa_dev_matvec(e->prec, lg, dir, gen, in_panel, out_panel, e->dd, it_ptr(e, sp), sp.n, e->arena_a,
+        prof_begin(e, dir == 0 ? PH_MV_FWD : PH_MV_ADJ, lg);
+        DEVCHK(ssa_dev_matvec(e->prec, lg, dir, in_panel, out_panel, e->dd, it_ptr(e, sp), sp.n, e->arena_a,
                               e->tw, e->st));
+        prof_end(e);
+    }
+    return 0;
+}
+
+/* the Philox test matrix of every series into its Z panel */
+static int omega_all(ssa_engine *e)
+{
+    for (int nb = 1; nb < NBW; nb++) {
+        if (!e->tZ[nb].n) continue;
+        prof_begin(e, PH_OMEGA, nb);
+        DEVCHK(ssa_dev_omega(e->prec, e->dd, it_ptr(e, e->tZ[nb]), e->tZ[nb].n, e->arena_a, e->st));
         prof_end(e);
     }
     return 0;
@@ -723,15 +735,16 @@ static int launch_decompose(ssa_engine *e, int q, int injected)
 {
     int bm = bmax_all(e);
     const int all2 = e->cholqr2_all;
-    DEVCHK(matvec_all(e, 0, !injected, SSA_PANEL_Z, SSA_PANEL_Y));
+    if (!injected) DEVCHK(omega_all(e));
+    DEVCHK(matvec_all(e, 0, SSA_PANEL_Z, SSA_PANEL_Y));
     DEVCHK(orth(e, SSA_PANEL_Y, bm, all2 || q == 0));
     for (int it = 0; it < q; it++) {
-        DEVCHK(matvec_all(e, 1, 0, SSA_PANEL_Y, SSA_PANEL_Z));
+        DEVCHK(matvec_all(e, 1, SSA_PANEL_Y, SSA_PANEL_Z));
         DEVCHK(orth(e, SSA_PANEL_Z, bm, all2));
-        DEVCHK(matvec_all(e, 0, 0, SSA_PANEL_Z, SSA_PANEL_Y));
+        DEVCHK(matvec_all(e, 0, SSA_PANEL_Z, SSA_PANEL_Y));
         DEVCHK(orth(e, SSA_PANEL_Y, bm, all2 || it == q - 1));
     }
-    DEVCHK(matvec_all(e, 1, 0, SSA_PANEL_Y, SSA_PANEL_Z));
+    DEVCHK(matvec_all(e, 1, SSA_PANEL_Y, SSA_PANEL_Z));
     DEVCHK(cholqr_factor(e, SSA_PANEL_Z, bm, NEED_R));
     prof_begin(e, PH_SVD, 0);
     DEVCHK(ssa_dev_svd(bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
@@ -961,7 +974,7 @@ int ssa_engine_matvec(ssa_engine *e, int s, int dir, const double *in, double *o
         sp.n = (int)(e->hitems_n - sp.off);
         rc = items_upload(e, e->dyn_base);
         if (rc == 0)
-            rc = ssa_dev_matvec(e->prec, d->lgn, dir, 0, dir == 0 ? SSA_PANEL_Z : SSA_PANEL_Y,
+            rc = ssa_dev_matvec(e->prec, d->lgn, dir, dir == 0 ? SSA_PANEL_Z : SSA_PANEL_Y,
                                 dir == 0 ? SSA_PANEL_Y : SSA_PANEL_Z, e->dd, it_ptr(e, sp), sp.n, e->arena_a, e->tw,
                                 e->st);
     }

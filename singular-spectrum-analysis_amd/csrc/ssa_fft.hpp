@@ -108,20 +108,31 @@ template <typename T, int S, int R> __device__ __forceinline__ void dftR(C2<T> *
     else if constexpr (R == 16) dft16<T, S>(v);
 }
 
-// LDS index with one pad element every 16 (breaks the power-of-two strides
-// of the scatter writes across banks)
-__device__ __forceinline__ int lpad(int p) { return p + (p >> 4); }
+// LDS index of element p in the exchange that follows a pass of stride
+// NS = 2^NSLOG and radix R = 2^RLOG.  The scatter writes groups of NS
+// consecutive positions R*NS apart; the gather reads consecutive positions.
+// XOR-ing the group bits above max(LW, NSLOG + RLOG) into bits [NSLOG, LW)
+// spreads both over all 2^LW banks an element width covers (32 four-byte
+// banks: LW = 5 for 4-byte values, 4 for 8-byte values).  The map is a
+// bijection inside each block of 2^LW elements, so no padding is needed.
+template <typename T, int NSLOG, int RLOG> __device__ __forceinline__ int lswz(int p)
+{
+    constexpr int LW = sizeof(T) == 4 ? 5 : 4;
+    constexpr int S = (NSLOG + RLOG > LW) ? NSLOG + RLOG : LW;
+    if constexpr (NSLOG >= LW) return p;
+    else return p ^ (((p >> S) << NSLOG) & ((1 << LW) - 1));
+}
 
 // E = 8 values per thread (radix-8 passes), so a transform plus an
 // accumulator plus the butterfly temporaries fit the 128-VGPR budget of
 // 4 waves per SIMD; 1024 threads cover N = 8192.
 template <typename T, int LOGN> struct FftGeom {
     static constexpr int N = 1 << LOGN;
-    static constexpr int E = N >= 16 ? 8 : N / 2;
+    static constexpr int E = (sizeof(T) == 4 && LOGN >= 10) ? 16 : N >= 16 ? 8 : N / 2;
     static constexpr int NT = N / E;
     static constexpr int LOGE = (E == 16) ? 4 : (E == 8) ? 3 : (E == 4) ? 2 : 1;
     static constexpr int FIRST = (LOGN % LOGE) ? (LOGN % LOGE) : LOGE;   // radix log of pass 0
-    static constexpr int LDS = N + (N >> 4) + 1;                         // elements per plane
+    static constexpr int LDS = N;                                        // elements per plane
     // exchange one plane at a time when both planes would keep a second
     // workgroup off the CU (160 KB LDS)
     static constexpr bool SPLIT = 2 * LDS * sizeof(T) > 80000;
@@ -180,7 +191,7 @@ __device__ __forceinline__ void fft_pass(FftRegs<T, LOGN> &c, T *lre, T *lim, co
                 const int base = ((j >> NSLOG) << (NSLOG + RLOG)) + (j & (NS - 1));
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    const int p = lpad(base + r * NS);
+                    const int p = lswz<T, NSLOG, RLOG>(base + r * NS);
                     if (SPLIT) {
                         if (ph == 0) lre[p] = c[m + r * BPT].x;
                         else lre[p] = c[m + r * BPT].y;
@@ -193,7 +204,7 @@ __device__ __forceinline__ void fft_pass(FftRegs<T, LOGN> &c, T *lre, T *lim, co
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                const int p = lpad(tid + e * NT);
+                const int p = lswz<T, NSLOG, RLOG>(tid + e * NT);
                 if (SPLIT) {
                     if (ph == 0) c[e].x = lre[p];
                     else c[e].y = lre[p];

@@ -172,13 +172,42 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_spectra(const ssa
 }
 
 // ---------------------------------------------------------------------------
+// Gaussian test matrix into panel Z (K x b): column pair j of row r is the
+// Philox draw (row r, pair j) of the series' key and call counter; FP32
+// panels use the FP32 Box-Muller of the same uniforms.  Item (s, tile of
+// SSA_GRAM_ROWS rows).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_omega(const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items,
+                                               T *bulk)
+{
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const Panel<T> P = panel_of<T>(bulk, d, SSA_PANEL_Z);
+    const int r0 = it.x * SSA_GRAM_ROWS, r1 = min(P.rows, r0 + SSA_GRAM_ROWS);
+    const int pairs = (d.b + 1) / 2;
+    for (int j = 0; j < pairs; j++) {
+        T *c0 = P.p + (long long)(2 * j) * P.ld;
+        T *c1 = c0 + P.ld;
+        const bool two = 2 * j + 1 < d.b;
+        for (int r = r0 + (int)threadIdx.x; r < r1; r += 256) {
+            T g0, g1;
+            if constexpr (sizeof(T) == 4) gauss_pair_f(d.seed, d.call, r, j, g0, g1);
+            else gauss_pair(d.seed, d.call, r, j, g0, g1);
+            c0[r] = g0;
+            if (two) c1[r] = g1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // fused block matvec.  Item (s, pair, o0, o1): columns 2*pair, 2*pair+1 and
 // output chunks [o0, o1).  dir 0: out rows are the stacked L dimension
 // (chunk o = m*AL + a), input chunks run over K; dir 1: the reverse.
 // out[o][i] = sum_p sum_j seg_(o,p)[i + j] in_p[j]  via  T+( X_(o,p) . T+(in_p) ).
 // ---------------------------------------------------------------------------
-template <typename T, int LOGN, int GEN>
-__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (sizeof(T) == 4 && !GEN ? 8 : 4)) void k_matvec(const ssa_desc *__restrict__ D,
+template <typename T, int LOGN>
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::E == 8 && sizeof(T) == 4 ? 8 : 4)) void k_matvec(const ssa_desc *__restrict__ D,
                                                                      const ssa_item *__restrict__ items, T *bulk,
                                                                      const C2<T> *__restrict__ tw, int dir,
                                                                      int in_panel, int out_panel, int n_items)
@@ -215,19 +244,7 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (sizeof(T) == 4 && !GEN ? 8
                     row0 = m * d.L + a * d.CL;
                     len = min(d.CL, d.L - a * d.CL);
                 }
-                if constexpr (GEN) {
-#pragma unroll
-                    for (int e = 0; e < E; e++) {
-                        const int q = threadIdx.x + e * NT;
-                        T g0 = 0, g1 = 0;
-                        if (q < len) {
-                            if constexpr (sizeof(T) == 4) gauss_pair_f(d.seed, d.call, row0 + q, it.x, g0, g1);
-                            else gauss_pair(d.seed, d.call, row0 + q, it.x, g0, g1);
-                        }
-                        z[e].x = g0;
-                        z[e].y = two ? g1 : (T)0;
-                    }
-                } else {
+                {
                     const T *a0 = pin.p + (long long)c0 * pin.ld + row0;
                     const T *a1 = pin.p + (long long)(two ? c1 : c0) * pin.ld + row0;
 #pragma unroll
@@ -1018,7 +1035,7 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
 // (the imaginary part of the self-convolution of a + ib is 2 a (*) b).
 // ---------------------------------------------------------------------------
 template <typename T, int LOGN>
-__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (sizeof(T) == 4 ? 8 : 4)) void k_recon(const ssa_desc *__restrict__ D,
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::E == 8 && sizeof(T) == 4 ? 8 : 4)) void k_recon(const ssa_desc *__restrict__ D,
                                                               const ssa_item *__restrict__ items, T *bulk,
                                                               const double *__restrict__ small,
                                                               const int4 *__restrict__ gdesc,
@@ -1243,28 +1260,32 @@ int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items,
     return (int)hipGetLastError();
 }
 
-int ssa_dev_matvec(int prec, int lgn, int dir, int gen_omega, int in_panel, int out_panel, const ssa_desc *D,
+int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel, const ssa_desc *D,
                    const ssa_item *items, int n_items, void *bulk, const void *tw, void *stream)
 {
     if (n_items <= 0) return 0;
     if (!lgn_ok(prec, lgn)) return -2;
     hipStream_t st = (hipStream_t)stream;
-#define LAUNCH(T, L, GEN)                                                                                     \
-    hipLaunchKernelGGL((k_matvec<T, L, GEN>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L>::NT), 0, st, D,     \
-                       items, (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel, n_items)
+#define LAUNCH(T, L)                                                                                          \
+    hipLaunchKernelGGL((k_matvec<T, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L>::NT), 0, st, D, items,   \
+                       (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel, n_items)
 #define CASE(L)                                                                                               \
     case L:                                                                                                   \
-        if (prec == SSA_PREC_F64) {                                                                           \
-            if (gen_omega) LAUNCH(double, L, 1);                                              \
-            else LAUNCH(double, L, 0);                                                        \
-        } else {                                                                                              \
-            if (gen_omega) LAUNCH(float, L, 1);                                                               \
-            else LAUNCH(float, L, 0);                                                                         \
-        }                                                                                                     \
+        if (prec == SSA_PREC_F64) LAUNCH(double, L);                                                          \
+        else LAUNCH(float, L);                                                                                \
         break;
     switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
 #undef CASE
 #undef LAUNCH
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, void *stream)
+{
+    if (n_items <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == SSA_PREC_F64) hipLaunchKernelGGL(k_omega<double>, dim3(n_items), dim3(256), 0, st, D, items, (double *)bulk);
+    else hipLaunchKernelGGL(k_omega<float>, dim3(n_items), dim3(256), 0, st, D, items, (float *)bulk);
     return (int)hipGetLastError();
 }
 

@@ -449,7 +449,7 @@ class Batch:
         return int(self._l.ssa_amd_batch_bytes(self._h))
 
     PHASES = ("spectra", "matvec_H", "matvec_HT", "gram", "chol", "apply", "svd", "uv", "recon", "olap", "forecast",
-              "matvec_H_gen")
+              "omega")
 
     def profile(self, max_rec=4096):
         """One eager step with HIP events around every launch:
