@@ -62,6 +62,22 @@ def timed_steps(batch, steps, warmup):
     return total, np.array(lat)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes recorded in profiles/traffic.json (each
+    counter scaled by what it reads for a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            entry = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not entry:
+        return None
+    return float(entry["fetch_kib"]) * 1024 * float(entry["fetch_scale"]) + \
+        float(entry["write_kib"]) * 1024 * float(entry["write_scale"])
+
+
 def roofline_of(batch, plans, Ns, Ls, s):
     """Live HIP-event timing of every launch of one eager step; the dominant
     kernel's algorithmic bytes per launch over its average duration."""
@@ -84,10 +100,13 @@ def roofline_of(batch, plans, Ns, Ls, s):
     bytes_launch = float(sum(matvec_bytes(int(Ns[i]), int(Ls[i]), plans[i]["b"], s) for i in sel))
     avg_ms = float(np.mean(v))
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    kernel = f"k_matvec<{'float' if s == 4 else 'double'}, {lg}>"
+    traffic = measured_traffic(kernel)
     return {
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_matvec<{'float' if s == 4 else 'double'}, {lg}>",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else round(traffic),
+                     "kernel": kernel,
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_ms, 4),
                      "series_in_launch": len(sel), "launches_timed": len(v)},
         "phase_ms": {k: round(v, 3) for k, v in sorted(phase_tot.items(), key=lambda kv: -kv[1])},

@@ -70,6 +70,7 @@ struct ssa_engine {
     int prof_on, prof_n, prof_cap;
     int step_h;                /* forecast horizon of the batched step (0: none) */
     int cholqr2_all;           /* CholeskyQR2 for every orthonormalisation (default: last one only) */
+    int final2;                /* CholeskyQR2 for the last orthonormalisation of Y and for B */
     hipEvent_t *prof_ev;
     int *prof_phase, *prof_lgn;
 };
@@ -354,6 +355,13 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
     {
         const char *v = getenv("SSA_AMD_CHOLQR2_ALL");
         e->cholqr2_all = (v && *v && *v != '0') ? 1 : 0;
+        /* FP32 panels store Q in FP32, which bounds its orthogonality at
+         * about eps32 whatever the factorisation; one FP64 CholeskyQR
+         * (defect eps64 * cond^2) is then enough for the last
+         * orthonormalisation and for the R factor of B.  FP64 panels keep
+         * CholeskyQR2 there. */
+        const char *f = getenv("SSA_AMD_FINAL_CHOLQR2");
+        e->final2 = (prec == SSA_PREC_F64) || (f && *f && *f != '0');
     }
     *err = 0;
     return e;
@@ -679,11 +687,17 @@ static int apply_all(ssa_engine *e, int phase, int src, int dst, int slot, int n
     return 0;
 }
 
+/* Cholesky of every series' summed Gram, one launch per column-block count */
 static int chol_all(ssa_engine *e, int panel, int pass, int bm)
 {
-    prof_begin(e, PH_CHOL, panel);
-    DEVCHK(ssa_dev_chol(panel, pass, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
-    prof_end(e);
+    (void)bm;
+    for (int nb = 1; nb < NBW; nb++) {
+        if (!e->ser_nb[nb].n) continue;
+        prof_begin(e, PH_CHOL, panel);
+        DEVCHK(ssa_dev_chol(panel, pass, 16 * nb < SSA_BMAX ? 16 * nb : SSA_BMAX, e->dd, it_ptr(e, e->ser_nb[nb]),
+                            e->ser_nb[nb].n, e->small, e->st));
+        prof_end(e);
+    }
     return 0;
 }
 
@@ -734,21 +748,36 @@ static int orth(ssa_engine *e, int panel, int bm, int two_pass)
 static int launch_decompose(ssa_engine *e, int q, int injected)
 {
     int bm = bmax_all(e);
-    const int all2 = e->cholqr2_all;
+    const int all2 = e->cholqr2_all, fin2 = e->final2 || all2;
     if (!injected) DEVCHK(omega_all(e));
     DEVCHK(matvec_all(e, 0, SSA_PANEL_Z, SSA_PANEL_Y));
-    DEVCHK(orth(e, SSA_PANEL_Y, bm, all2 || q == 0));
+    DEVCHK(orth(e, SSA_PANEL_Y, bm, all2 || (fin2 && q == 0)));
     for (int it = 0; it < q; it++) {
         DEVCHK(matvec_all(e, 1, SSA_PANEL_Y, SSA_PANEL_Z));
         DEVCHK(orth(e, SSA_PANEL_Z, bm, all2));
         DEVCHK(matvec_all(e, 0, SSA_PANEL_Z, SSA_PANEL_Y));
-        DEVCHK(orth(e, SSA_PANEL_Y, bm, all2 || it == q - 1));
+        DEVCHK(orth(e, SSA_PANEL_Y, bm, all2 || (fin2 && it == q - 1)));
     }
     DEVCHK(matvec_all(e, 1, SSA_PANEL_Y, SSA_PANEL_Z));
-    DEVCHK(cholqr_factor(e, SSA_PANEL_Z, bm, NEED_R));
-    prof_begin(e, PH_SVD, 0);
-    DEVCHK(ssa_dev_svd(bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
-    prof_end(e);
+    /* R factor of B; the R^-1 of the final Q (for colsum(Q)) stays in its slot */
+    int r_slot, rinv_slot;
+    if (fin2) {
+        DEVCHK(cholqr_factor(e, SSA_PANEL_Z, bm, NEED_R));
+        r_slot = SSA_SLOT_R;
+        rinv_slot = SSA_SLOT_RINV;
+    } else {
+        DEVCHK(gram_all(e, SSA_PANEL_Z, 0));
+        DEVCHK(chol_all(e, SSA_PANEL_Z, 2, bm));   /* into the R2 slots: R1^-1 of Y is still needed */
+        r_slot = SSA_SLOT_R2;
+        rinv_slot = SSA_SLOT_R1INV;
+    }
+    for (int nb = 1; nb < NBW; nb++) {
+        if (!e->ser_nb[nb].n) continue;
+        prof_begin(e, PH_SVD, nb);
+        DEVCHK(ssa_dev_svd(16 * nb < SSA_BMAX ? 16 * nb : SSA_BMAX, r_slot, rinv_slot, e->dd, it_ptr(e, e->ser_nb[nb]),
+                           e->ser_nb[nb].n, e->small, e->st));
+        prof_end(e);
+    }
     DEVCHK(apply_all(e, PH_UV, SSA_PANEL_Y, SSA_PANEL_U, SSA_SLOT_WK, 1));
     DEVCHK(apply_all(e, PH_UV, SSA_PANEL_Z, SSA_PANEL_V, SSA_SLOT_WKS, 1));
     return 0;

@@ -429,6 +429,7 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
         };
         int r0 = r_begin + 16 * w;
         load(r0, cur);
+#pragma unroll 1
         for (; r0 < r_end; r0 += 64) {
             load(r0 + 64, nxt);
 #pragma unroll
@@ -884,7 +885,9 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply(const ssa_desc *D, c
 // i.e. sum(U[:,i]) >= 0 for U = Q W.
 // Outputs: WK = W[:, :k] (signed), WKS = WK S^-1, sigma, lambda, total variance.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *items, double *small)
+template <int NB>
+__global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *items, double *small, int r_slot,
+                                             int rinv_slot)
 {
     extern __shared__ double sm[];
     const int bq = D[items[blockIdx.x].s].b;
@@ -896,7 +899,7 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const int b = d.b, bp = (b + 1) & ~1;
-    const double *R = slot(small, d, SSA_SLOT_R);
+    const double *R = small + d.off_small + (long long)r_slot * SSA_BMAX * SSA_BMAX;
     // A = R^T: column j is row j of R (R column-major, ld SSA_BMAX)
     for (int idx = threadIdx.x; idx < bp * b; idx += 256) {
         const int i = idx / bp, j = idx - i * bp;
@@ -926,7 +929,7 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                 }
                 if (p > q) { int t = p; p = q; q = t; }
                 // rows gl + 8m of both columns (and of W), all loads issued up front
-                constexpr int MB = SSA_BMAX / 8;
+                constexpr int MB = 2 * NB;     // rows per lane: b <= 16 NB rows over 8 lanes
                 double x[MB], y[MB];
 #pragma unroll
                 for (int m = 0; m < MB; m++) {
@@ -989,7 +992,7 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     double *WK = slot(small, d, SSA_SLOT_WK), *WKS = slot(small, d, SSA_SLOT_WKS);
     // colsum(Q) = colsum(Y) R_Y^-1 of the final orthonormalisation of Y
     if (threadIdx.x < b) {
-        const double *Ri = slot(small, d, SSA_SLOT_RINV);
+        const double *Ri = small + d.off_small + (long long)rinv_slot * SSA_BMAX * SSA_BMAX;
         const int j = threadIdx.x;
         double cq = 0.0;
         for (int i = 0; i <= j; i++) cq += misc[SSA_MISC_COLSUMY + i] * Ri[(long long)j * SSA_BMAX + i];
@@ -1397,11 +1400,20 @@ int ssa_dev_trmm(int nb, int slot_a, int slot_b, int slot_c, const ssa_desc *D, 
     return (int)hipGetLastError();
 }
 
-int ssa_dev_svd(int bmax, const ssa_desc *D, const ssa_item *items, int n_items, double *small, void *stream)
+int ssa_dev_svd(int bmax, int r_slot, int rinv_slot, const ssa_desc *D, const ssa_item *items, int n_items,
+                double *small, void *stream)
 {
     if (n_items <= 0) return 0;
     const size_t lds = (size_t)((bmax + 1) & ~1) * (bmax + 1) * sizeof(double);
-    hipLaunchKernelGGL(k_svd, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items, small);
+    hipStream_t st = (hipStream_t)stream;
+    switch (SSA_NB(bmax)) {
+    case 1: hipLaunchKernelGGL(k_svd<1>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
+    case 2: hipLaunchKernelGGL(k_svd<2>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
+    case 3: hipLaunchKernelGGL(k_svd<3>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
+    case 4: hipLaunchKernelGGL(k_svd<4>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
+    case 5: hipLaunchKernelGGL(k_svd<5>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
+    default: return -1;
+    }
     return (int)hipGetLastError();
 }
 

@@ -143,6 +143,32 @@ def test_mixed_precision_error():
         assert np.abs(a - b).max() / r < 1e-3, g
 
 
+def test_mixed_batch_against_oracle():
+    """The mixed batched path (FP32 panels and transforms, FP64 Gram,
+    Cholesky and SVD, the final factorisations single-pass) against the FP64
+    oracle on series of the benchmark's distribution, each with its own
+    Philox key: sigma[:k] within 1e-5 relative, the reconstruction of all k
+    components within 1e-5 relative (2-norm)."""
+    S = _ssa()
+    xs, Ns, Ls, ks = workloads.ragged_batch(count=8, n_lo=500, n_hi=12000, seed=21, dtype=np.float32)
+    b = S.Batch(xs, Ls, ks, p=10, q=2, precision=1, seed=9)
+    b.run(use_graph=False)
+    b.sync()
+    worst_s = worst_r = 0.0
+    for i in range(len(xs)):
+        x = np.asarray(xs[i], dtype=np.float64)
+        L, k = int(Ls[i]), int(ks[i])
+        K = len(x) - L + 1
+        bw = min(k + 10, L, K)
+        key = (9 ^ (0x9E3779B97F4A7C15 * (i + 1))) & ((1 << 64) - 1)
+        orc = O.OracleSSA(x, L).decompose(k, 10, 2, O.philox_gaussian(key, 0, K, bw))
+        worst_s = max(worst_s, float(np.max(np.abs(b.sigma(i)[:k] - orc.sigma[:k]) / orc.sigma[:k])))
+        worst_r = max(worst_r, rel2(b.output(i), orc.reconstruct(list(range(k)))))
+    print(f"mixed batch: worst sigma rel {worst_s:.2e}, worst reconstruction rel2 {worst_r:.2e}")
+    assert worst_s < 1e-5
+    assert worst_r < 1e-5
+
+
 def test_mssa_parity():
     S = _ssa()
     X = workloads.mssa_channels(M=4, N=1200, seed=5)
