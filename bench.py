@@ -24,8 +24,9 @@ for p in (os.path.join(ROOT, "singular-spectrum-analysis_amd"), os.path.join(ROO
     if p not in sys.path:
         sys.path.insert(0, p)
 
-import ssa_amd  # noqa: E402
 import workloads  # noqa: E402
+
+ssa_amd = None   # loaded after the CPU baseline has forked its workers
 
 HBM_PEAK_GBS = 8000.0
 P_OVERSAMPLE = 10
@@ -97,9 +98,12 @@ def roofline_of(batch, plans, Ns, Ls, s):
             mv.setdefault(lg, []).extend(v)
     lg, v = max(mv.items(), key=lambda kv: sum(kv[1]))
     sel = [i for i, p in enumerate(plans) if p["lgn"] == lg]
-    bytes_launch = float(sum(matvec_bytes(int(Ns[i]), int(Ls[i]), plans[i]["b"], s) for i in sel))
+    # a step applies H or H^T 2(q+1) times to every series of the bucket; the
+    # bucket may be split over several launches (parallel branches)
+    step_bytes = 2 * (Q_POWER + 1) * float(sum(matvec_bytes(int(Ns[i]), int(Ls[i]), plans[i]["b"], s) for i in sel))
+    bytes_launch = step_bytes / len(v)
     avg_ms = float(np.mean(v))
-    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    achieved = step_bytes / (float(np.sum(v)) * 1e-3) / 1e9
     kernel = f"k_matvec<{'float' if s == 4 else 'double'}, {lg}>"
     traffic = measured_traffic(kernel)
     return {
@@ -108,35 +112,80 @@ def roofline_of(batch, plans, Ns, Ls, s):
                      "traffic": None if traffic is None else round(traffic),
                      "kernel": kernel,
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_ms, 4),
-                     "series_in_launch": len(sel), "launches_timed": len(v)},
+                     "series_in_bucket": len(sel), "launches_timed": len(v)},
         "phase_ms": {k: round(v, 3) for k, v in sorted(phase_tot.items(), key=lambda kv: -kv[1])},
         "profiled_step_ms": round(total, 3),
     }
 
 
-def cpu_baseline(xs, Ls, ks, budget_s):
-    """The FP64 oracle (a C restatement of the reference's call sequence,
-    one thread) on the first series of the same batch until budget_s."""
+def _oracle_ssa(args):
+    """One SSA of series i by the FP64 oracle: decompose q=2 p=10 with the
+    series' Philox test matrix, reconstruct all k components."""
+    x, L, k = args
     import oracle as O
-    done = 0
-    nsum = 0
+    x = np.asarray(x, dtype=np.float64)
+    K = len(x) - L + 1
+    b = min(k + P_OVERSAMPLE, L, K)
+    s = O.OracleSSA(x, L).decompose(k, P_OVERSAMPLE, Q_POWER, O.philox_gaussian(4, 0, K, b))
+    s.reconstruct(list(range(k)))
+    return len(x)
+
+
+def _oracle_worker(w, workers, xs, Ls, ks, budget_s):
+    done = nsum = 0
     t0 = time.perf_counter()
-    for i in range(len(xs)):
-        x = np.asarray(xs[i], dtype=np.float64)
-        L, k = int(Ls[i]), int(ks[i])
-        K = len(x) - L + 1
-        b = min(k + P_OVERSAMPLE, L, K)
-        om = O.philox_gaussian(4, 0, K, b)
-        s = O.OracleSSA(x, L).decompose(k, P_OVERSAMPLE, Q_POWER, om)
-        s.reconstruct(list(range(k)))
+    for i in range(w, len(xs), workers):
+        nsum += _oracle_ssa((xs[i], int(Ls[i]), int(ks[i])))
         done += 1
-        nsum += len(x)
         if time.perf_counter() - t0 > budget_s:
             break
-    el = time.perf_counter() - t0
-    return {"value": round(done / el, 2), "unit": "SSA/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} series of the same batch (sum N = {nsum}), decompose q=2 p=10 + "
-                      f"reconstruct all k, FP64, {el:.1f} s"}
+    return done, nsum, time.perf_counter() - t0
+
+
+def cpu_baseline(xs, Ls, ks, budget_s, workers):
+    """The FP64 oracle (a C restatement of the reference's call sequence) on
+    the same batch: one thread, then one process per core (forked before any
+    GPU call), each over its share of the series until budget_s."""
+    import multiprocessing as mp
+    d1, n1, t1 = _oracle_worker(0, 1, xs, Ls, ks, budget_s)
+    one = done_all = nsum_all = 0
+    one = d1 / t1
+    ctx = mp.get_context("fork")
+    with ctx.Pool(workers) as pool:
+        res = pool.starmap(_oracle_worker, [(w, workers, xs, Ls, ks, budget_s) for w in range(workers)])
+    done_all = sum(r[0] for r in res)
+    nsum_all = sum(r[1] for r in res)
+    el = max(r[2] for r in res)
+    return {"value": round(done_all / el, 2), "unit": "SSA/s", "cores": workers, "kind": "port",
+            "sample": f"{done_all} series of the same batch (sum N = {nsum_all}) over {workers} processes "
+                      f"(series i to process i mod {workers}), decompose q=2 p=10 + reconstruct all k, FP64, "
+                      f"{el:.1f} s; one thread: {one:.2f} SSA/s over the first {d1} series ({n1} samples)",
+            "value_1_core": round(one, 2)}
+
+
+def serving(seconds):
+    """cfg5: requests with N log-uniform in [500, 50000], L = N // 4,
+    k ~ U{10..50}, reconstruct + forecast h = 50 of components
+    0..floor(2k/5)-1, mixed form.  Capacity from a saturated backlog; then an
+    open-loop Poisson stream at 70 % of it for `seconds`; then the same kind
+    of request served alone."""
+    import serve
+    serve.Server(max_wave=64).run(serve.synthetic_requests(64, rate=np.inf, seed=60))   # warm-up
+    backlog = serve.synthetic_requests(2048, rate=np.inf, seed=6)
+    done = serve.Server(max_wave=512, inflight=2).run(backlog)
+    capacity = len(done) / max(r.done for r in done)
+    rate = 0.7 * capacity
+    count = max(64, int(rate * seconds))
+    stream = serve.Server(max_wave=512, inflight=2).run(serve.synthetic_requests(count, rate=rate, seed=61))
+    alone = []
+    for r in serve.synthetic_requests(48, rate=np.inf, seed=62):
+        alone.extend(serve.Server(max_wave=1, inflight=1).run([r]))
+    mix, solo = serve.latency_stats(stream), serve.latency_stats(alone)
+    return {"workload": "N log-uniform in [500, 50000], L = N // 4, k ~ U{10..50}, group 0..floor(2k/5)-1, "
+                        "h = 50, Poisson arrivals, mixed form",
+            "capacity_req_s": round(capacity, 1), "rate_req_s": round(rate, 1),
+            "in_mix": mix, "alone": solo,
+            "p50_in_mix_over_alone": round(mix["p50_ms"] / solo["p50_ms"], 3)}
 
 
 def main():
@@ -146,9 +195,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--plain-steps", type=int, default=5)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this SSA path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -157,6 +208,16 @@ def main():
 
     xs, Ns, Ls, ks = workloads.ragged_batch(count=args.count, seed=4, dtype=np.float32)
     print(f"[bench] batch: {len(xs)} series, sum N = {int(Ns.sum())}", flush=True)
+
+    # CPU baseline first: its worker processes are forked before any GPU call
+    workers = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
+    cpu = None if args.no_cpu else cpu_baseline(xs, Ls, ks, args.cpu_seconds, workers)
+    if cpu:
+        print(f"[bench] cpu baseline: {cpu['value']} SSA/s on {workers} cores, {cpu['value_1_core']} on one", flush=True)
+
+    global ssa_amd
+    import ssa_amd as _ssa_amd
+    ssa_amd = _ssa_amd
 
     mixed = ssa_amd.Batch(xs, Ls, ks, p=P_OVERSAMPLE, q=Q_POWER, precision=1, seed=4)
     plans = [mixed.plan(i) for i in range(len(xs))]
@@ -175,9 +236,13 @@ def main():
         print(f"[bench] plain FP64: {plain:.0f} SSA/s, {1e3 * ptot / args.plain_steps:.2f} ms/step", flush=True)
         del pb
 
-    cpu = None if args.no_cpu else cpu_baseline(xs, Ls, ks, args.cpu_seconds)
-    if cpu:
-        print(f"[bench] cpu baseline: {cpu['value']} SSA/s", flush=True)
+    serving_out = None
+    if not args.no_serve:
+        serving_out = serving(args.serve_seconds)
+        print(f"[bench] serving: capacity {serving_out['capacity_req_s']} req/s, at {serving_out['rate_req_s']} "
+              f"req/s p50 {serving_out['in_mix']['p50_ms']} ms p99 {serving_out['in_mix']['p99_ms']} ms; alone "
+              f"p50 {serving_out['alone']['p50_ms']} ms", flush=True)
+
 
     ms_step = 1e3 * total / args.steps
     out = {
@@ -205,6 +270,7 @@ def main():
         "step_achieved_GBs": round(alg / (ms_step * 1e-3) / 1e9, 1),
     }
     out.update(roof)
+    out["serving"] = serving_out
     out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 

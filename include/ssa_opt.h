@@ -176,7 +176,8 @@ int ssa_amd_reconstruct_many(const SSA_Opt *ssa, const int *glist, const int *gc
 /* the device correlation plan of the context: out[0..5] = n, CL, CK, AL, SK, Jacobi sweeps */
 int ssa_amd_plan_info(const SSA_Opt *ssa, int *out6);
 
-/* batched path: many uneven series in one device engine */
+/* batched path: many uneven series in device engines (parallel graph
+ * branches, SSA_AMD_BRANCHES, default 2) */
 typedef struct ssa_amd_batch ssa_amd_batch;
 ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const int *k, int p, int q,
                                     int precision, unsigned long long seed);
@@ -190,6 +191,9 @@ int ssa_amd_batch_query(ssa_amd_batch *b);
 int ssa_amd_batch_status(ssa_amd_batch *b, int i);
 /* signals concatenated as float32 (offsets[i] = first sample of series i) */
 int ssa_amd_batch_set_signals_f32(ssa_amd_batch *b, const float *cat, const long long *offsets);
+/* Philox key of series i (default: seed ^ 0x9E3779B97F4A7C15 * (i + 1)); a
+ * request keyed by its own id draws the same test matrix in any batch */
+int ssa_amd_batch_set_key(ssa_amd_batch *b, int i, unsigned long long key);
 /* one SSA of every series: spectra, decomposition, reconstruction of all
  * components; use_graph captures the launch sequence in a HIP graph */
 int ssa_amd_batch_run(ssa_amd_batch *b, int use_graph);
@@ -199,8 +203,9 @@ int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out);         /* N_i
 long long ssa_amd_batch_bytes(const ssa_amd_batch *b);
 /* one eager step with HIP events around every kernel launch: per launch the
  * phase (0 spectra, 1 matvec H, 2 matvec H^T, 3 Gram, 4 Cholesky, 5 apply,
- * 6 small SVD, 7 U/V, 8 reconstruction, 9 overlap-add), the FFT length log2
- * (or panel id) and the milliseconds; returns the number of launches */
+ * 6 small SVD, 7 U/V, 8 reconstruction, 9 overlap-add, 10 forecast, 11 test
+ * matrix), the FFT length log2 (or panel id / column blocks) and the
+ * milliseconds; branch after branch; returns the number of launches */
 int ssa_amd_batch_profile(ssa_amd_batch *b, int max_rec, int *phase, int *lgn, float *ms);
 /* plan of series i: n, CL, CK, AL, SK, b, k, log2 n */
 int ssa_amd_batch_plan(const ssa_amd_batch *b, int i, int *out8);

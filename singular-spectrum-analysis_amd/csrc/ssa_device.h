@@ -20,7 +20,6 @@ extern "C" {
 
 #define SSA_BMAX 80           /* max block width k + p */
 #define SSA_GRAM_ROWS 2048    /* panel rows per Gram / apply work item */
-#define SSA_NMAX_LOG 16       /* twiddle table length 2^16 */
 
 enum { SSA_PREC_F64 = 0, SSA_PREC_F32 = 1 };
 

@@ -424,16 +424,21 @@ int ssa_engine_upload_signal(ssa_engine *e, int s, const double *x)
 
 int ssa_engine_upload_signals_f32(ssa_engine *e, const float *cat, const long long *offsets)
 {
+    /* all signals through one pinned staging buffer, one synchronisation */
+    size_t total = 0;
+    for (int s = 0; s < e->count; s++) total += (size_t)e->hd[s].M * e->hd[s].N;
+    DEVCHK(pin_reserve(e, total * e->esz));
+    size_t at = 0;
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
         size_t n = (size_t)d->M * d->N;
-        DEVCHK(pin_reserve(e, n * e->esz));
-        if (e->prec == SSA_PREC_F32) memcpy(e->hpin, cat + offsets[s], n * 4);
-        else for (size_t i = 0; i < n; i++) ((double *)e->hpin)[i] = cat[offsets[s] + i];
-        HIPCHK(hipMemcpyAsync((char *)e->arena_a + e->esz * d->off_x, e->hpin, n * e->esz, hipMemcpyHostToDevice,
-                              e->st));
-        HIPCHK(hipStreamSynchronize(e->st));
+        char *dst = (char *)e->hpin + at * e->esz;
+        if (e->prec == SSA_PREC_F32) memcpy(dst, cat + offsets[s], n * 4);
+        else for (size_t i = 0; i < n; i++) ((double *)dst)[i] = cat[offsets[s] + i];
+        HIPCHK(hipMemcpyAsync((char *)e->arena_a + e->esz * d->off_x, dst, n * e->esz, hipMemcpyHostToDevice, e->st));
+        at += n;
     }
+    HIPCHK(hipStreamSynchronize(e->st));
     return 0;
 }
 
@@ -687,17 +692,14 @@ static int apply_all(ssa_engine *e, int phase, int src, int dst, int slot, int n
     return 0;
 }
 
-/* Cholesky of every series' summed Gram, one launch per column-block count */
+/* Cholesky of every series' summed Gram in one launch (sized for the widest
+ * series): each series is one latency-bound wave, so separate launches per
+ * width would each pay the slowest series' latency */
 static int chol_all(ssa_engine *e, int panel, int pass, int bm)
 {
-    (void)bm;
-    for (int nb = 1; nb < NBW; nb++) {
-        if (!e->ser_nb[nb].n) continue;
-        prof_begin(e, PH_CHOL, panel);
-        DEVCHK(ssa_dev_chol(panel, pass, 16 * nb < SSA_BMAX ? 16 * nb : SSA_BMAX, e->dd, it_ptr(e, e->ser_nb[nb]),
-                            e->ser_nb[nb].n, e->small, e->st));
-        prof_end(e);
-    }
+    prof_begin(e, PH_CHOL, panel);
+    DEVCHK(ssa_dev_chol(panel, pass, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    prof_end(e);
     return 0;
 }
 
@@ -1082,6 +1084,91 @@ int ssa_engine_step(ssa_engine *e, int q, int use_graph)
     HIPCHK(hipGraphLaunch(e->gexec, e->st));
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* several engines as parallel branches of one step                          */
+/* ------------------------------------------------------------------------ */
+
+struct ssa_multi {
+    ssa_engine **es;
+    int G;
+    hipEvent_t fork;
+    hipEvent_t *join;
+    hipGraphExec_t gexec;
+    int has_graph, graph_q;
+};
+
+ssa_multi *ssa_multi_create(ssa_engine *const *es, int G)
+{
+    if (G < 1 || !es) return NULL;
+    ssa_multi *m = (ssa_multi *)calloc(1, sizeof(ssa_multi));
+    if (!m) return NULL;
+    m->es = (ssa_engine **)malloc(sizeof(ssa_engine *) * G);
+    m->join = (hipEvent_t *)calloc(G, sizeof(hipEvent_t));
+    if (!m->es || !m->join) { free(m->es); free(m->join); free(m); return NULL; }
+    memcpy(m->es, es, sizeof(ssa_engine *) * G);
+    m->G = G;
+    int ok = hipEventCreateWithFlags(&m->fork, hipEventDisableTiming) == hipSuccess;
+    for (int g = 0; ok && g < G; g++) ok = hipEventCreateWithFlags(&m->join[g], hipEventDisableTiming) == hipSuccess;
+    if (!ok) { ssa_multi_destroy(m); return NULL; }
+    return m;
+}
+
+void ssa_multi_destroy(ssa_multi *m)
+{
+    if (!m) return;
+    if (m->has_graph) hipGraphExecDestroy(m->gexec);
+    if (m->fork) hipEventDestroy(m->fork);
+    for (int g = 0; g < m->G; g++) if (m->join[g]) hipEventDestroy(m->join[g]);
+    free(m->join);
+    free(m->es);
+    free(m);
+}
+
+/* every engine's step on its own stream, forked from and joined into the
+ * first engine's stream */
+static int multi_launch(ssa_multi *m, int q)
+{
+    HIPCHK(hipEventRecord(m->fork, m->es[0]->st));
+    for (int g = 1; g < m->G; g++) HIPCHK(hipStreamWaitEvent(m->es[g]->st, m->fork, 0));
+    for (int g = 0; g < m->G; g++) DEVCHK(launch_step(m->es[g], q));
+    for (int g = 1; g < m->G; g++) {
+        HIPCHK(hipEventRecord(m->join[g], m->es[g]->st));
+        HIPCHK(hipStreamWaitEvent(m->es[0]->st, m->join[g], 0));
+    }
+    return 0;
+}
+
+int ssa_multi_step(ssa_multi *m, int q, int use_graph)
+{
+    if (!m || q < 0) return -1;
+    for (int g = 0; g < m->G; g++) {
+        ssa_engine *e = m->es[g];
+        if (!e->prepared) return -1;
+        DEVCHK(sync_desc(e));
+        DEVCHK(prepare_step_groups(e));
+    }
+    if (!use_graph) return multi_launch(m, q);
+    if (!m->has_graph || m->graph_q != q) {
+        if (m->has_graph) { hipGraphExecDestroy(m->gexec); m->has_graph = 0; }
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(m->es[0]->st, hipStreamCaptureModeThreadLocal));
+        int rc = multi_launch(m, q);
+        hipError_t ce = hipStreamEndCapture(m->es[0]->st, &gr);
+        if (rc != 0 || ce != hipSuccess) return -1;
+        HIPCHK(hipGraphInstantiate(&m->gexec, gr, NULL, NULL, 0));
+        hipGraphDestroy(gr);
+        m->has_graph = 1;
+        m->graph_q = q;
+    }
+    HIPCHK(hipGraphLaunch(m->gexec, m->es[0]->st));
+    return 0;
+}
+
+/* the whole step is joined into the first engine's stream */
+int ssa_multi_sync(ssa_multi *m) { return m ? ssa_engine_sync(m->es[0]) : -1; }
+
+int ssa_multi_query(ssa_multi *m) { return m ? ssa_engine_query(m->es[0]) : -1; }
 
 int ssa_engine_profile_step(ssa_engine *e, int q, int max_rec, int *phase, int *lgn, float *ms)
 {

@@ -96,6 +96,17 @@ int ssa_engine_status(ssa_engine *e, int s);
  * 10 forecast */
 int ssa_engine_profile_step(ssa_engine *e, int q, int max_rec, int *phase, int *lgn, float *ms);
 int ssa_engine_sync(ssa_engine *e);
+
+/* Several engines (disjoint sets of series) stepped as parallel branches of
+ * one launch: forked from and joined into the first engine's stream, captured
+ * as one HIP graph when use_graph.  The kernels of one branch fill the tails
+ * of the other branches' launches. */
+typedef struct ssa_multi ssa_multi;
+ssa_multi *ssa_multi_create(ssa_engine *const *es, int G);
+void ssa_multi_destroy(ssa_multi *m);
+int ssa_multi_step(ssa_multi *m, int q, int use_graph);
+int ssa_multi_sync(ssa_multi *m);
+int ssa_multi_query(ssa_multi *m);
 void *ssa_engine_stream(ssa_engine *e);
 /* bytes of device memory held */
 long long ssa_engine_bytes(const ssa_engine *e);

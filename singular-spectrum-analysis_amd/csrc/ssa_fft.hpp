@@ -13,8 +13,11 @@
 // pointwise stage and the next transform.
 //
 // SIGN = -1 is the forward DFT sum_t z_t exp(-2 pi i k t/N), SIGN = +1 the
-// unscaled inverse.  Twiddles come from a table tw[m] = exp(+2 pi i m/2^16)
-// built on the host in long double.
+// unscaled inverse.  Twiddles come from tables built on the host in long
+// double.  FP64 transforms read every twiddle from per-pass tables (exactly
+// rounded, one load each); FP32 transforms read w, w^2, w^4, w^8 from a
+// table exp(2 pi i m / 2^16) and form the others with at most three products
+// (a few ulp), which costs fewer memory instructions than 15 loads.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -126,6 +129,22 @@ template <typename T, int NSLOG, int RLOG> __device__ __forceinline__ int lswz(i
 // E = 8 values per thread (radix-8 passes), so a transform plus an
 // accumulator plus the butterfly temporaries fit the 128-VGPR budget of
 // 4 waves per SIMD; 1024 threads cover N = 8192.
+constexpr int TW_BASE_LOG = 16;   // FP32 base table exp(2 pi i m / 2^16)
+
+// Per-pass twiddle tables (FP64): a pass of stride NS = 2^NSLOG and radix
+// R = 2^RLOG reads entry tw_offset(NSLOG, RLOG) + (r-1)*NS + kk =
+// exp(+2 pi i r kk / (NS R)) for butterfly kk < NS and r = 1..R-1 (r-major,
+// so consecutive lanes read consecutive entries).  Tables for RLOG = 1..4
+// and NSLOG = 0..TW_NSLOG_MAX, RLOG-major.
+constexpr int TW_NSLOG_MAX = 12;
+__host__ __device__ constexpr long long tw_offset(int nslog, int rlog)
+{
+    long long off = 0;
+    for (int r = 1; r < rlog; r++) off += ((1LL << (TW_NSLOG_MAX + 1)) - 1) * ((1LL << r) - 1);
+    return off + ((1LL << nslog) - 1) * ((1LL << rlog) - 1);
+}
+constexpr long long TW_ENTRIES = tw_offset(0, 5);
+
 template <typename T, int LOGN> struct FftGeom {
     static constexpr int N = 1 << LOGN;
     static constexpr int E = (sizeof(T) == 4 && LOGN >= 10) ? 16 : N >= 16 ? 8 : N / 2;
@@ -152,12 +171,21 @@ __device__ __forceinline__ void fft_pass(FftRegs<T, LOGN> &c, T *lre, T *lim, co
         C2<T> v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = c[m + r * BPT];
-        if constexpr (NS > 1) {
-            // w^r for r < R from the table entries w^1, w^2, w^4, w^8 and at most
-            // two products (a few ulp), keeping few twiddles live at once
+        if constexpr (NS > 1 && sizeof(T) == 8) {
+            static_assert(NSLOG <= TW_NSLOG_MAX, "twiddle table too short");
             const int kk = j & (NS - 1);
-            const int step = kk << (SSA_NMAX_LOG_DEV - NSLOG - RLOG);
-            C2<T> wp[5];
+            const C2<T> *tp = tw + tw_offset(NSLOG, RLOG) + kk;
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                C2<T> w = tp[(r - 1) * NS];
+                if constexpr (SIGN < 0) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        } else if constexpr (NS > 1) {
+            // w^r for r < R from the base-table entries w^1, w^2, w^4, w^8
+            const int kk = j & (NS - 1);
+            const int step = kk << (TW_BASE_LOG - NSLOG - RLOG);
+            C2<T> wp[4];
 #pragma unroll
             for (int b = 0; (1 << b) < R; b++) {
                 wp[b] = tw[(1 << b) * step];

@@ -25,7 +25,6 @@
 #include <math.h>
 #include <stdint.h>
 
-#define SSA_NMAX_LOG_DEV 16
 #include "ssa_device.h"
 #include "ssa_fft.hpp"
 
@@ -1210,24 +1209,38 @@ __global__ __launch_bounds__(256) void k_forecast(ssa_desc *D, const ssa_item *i
 // ===========================================================================
 extern "C" {
 
+static void put_twiddle(int prec, void *host, long long m, long double a)
+{
+    const long double c = cosl(a), s = sinl(a);
+    if (prec == SSA_PREC_F64) {
+        ((double *)host)[2 * m] = (double)c;
+        ((double *)host)[2 * m + 1] = (double)s;
+    } else {
+        ((float *)host)[2 * m] = (float)c;
+        ((float *)host)[2 * m + 1] = (float)s;
+    }
+}
+
 int ssa_dev_twiddles(int prec, void **table_out, void *stream)
 {
-    const int n = 1 << SSA_NMAX_LOG;
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    const long long n = prec == SSA_PREC_F64 ? TW_ENTRIES : (1LL << TW_BASE_LOG);
     const size_t esz = prec == SSA_PREC_F64 ? 16 : 8;
     void *host = malloc(n * esz);
     if (!host) return -1;
-    for (int m = 0; m < n; m++) {
-        // exp(+2 pi i m / n) in long double, octant-symmetric evaluation
-        long double a = 6.283185307179586476925286766559005768L * (long double)m / (long double)n;
-        long double c = cosl(a), s = sinl(a);
-        if (prec == SSA_PREC_F64) {
-            ((double *)host)[2 * m] = (double)c;
-            ((double *)host)[2 * m + 1] = (double)s;
-        } else {
-            ((float *)host)[2 * m] = (float)c;
-            ((float *)host)[2 * m + 1] = (float)s;
+    if (prec != SSA_PREC_F64)   // base table exp(+2 pi i m / 2^16)
+        for (long long m = 0; m < n; m++) put_twiddle(prec, host, m, two_pi * (long double)m / (long double)n);
+    for (int rlog = 1; prec == SSA_PREC_F64 && rlog <= 4; rlog++)
+        for (int nslog = 0; nslog <= TW_NSLOG_MAX; nslog++) {
+            const long long base = tw_offset(nslog, rlog);
+            const int NS = 1 << nslog, R = 1 << rlog;
+            for (int kk = 0; kk < NS; kk++)
+                for (int r = 1; r < R; r++) {
+                    // exp(+2 pi i r kk / (NS R))
+                    put_twiddle(prec, host, base + (long long)(r - 1) * NS + kk,
+                                two_pi * (long double)(r * kk) / (long double)(NS * R));
+                }
         }
-    }
     void *dev = nullptr;
     hipError_t e = hipMalloc(&dev, n * esz);
     if (e == hipSuccess) e = hipMemcpyAsync(dev, host, n * esz, hipMemcpyHostToDevice, (hipStream_t)stream);

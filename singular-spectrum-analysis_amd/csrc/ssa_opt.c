@@ -751,10 +751,36 @@ ssa_real mssa_opt_variance_explained(const MSSA_Opt *mssa, int start, int end)
 /* batched path                                                              */
 /* ------------------------------------------------------------------------ */
 
+/* A batch is G engines over disjoint sets of series, stepped as parallel
+ * branches of one graph (G = 1: a single engine).  Series are dealt to the
+ * branches round-robin in order of decreasing cost, so every branch holds a
+ * similar mix of sizes; the Philox key of series i depends on i only. */
+#define BATCH_MAX_BRANCHES 4
+
 struct ssa_amd_batch {
-    ssa_engine *eng;
-    int count, q;
+    ssa_engine *eng[BATCH_MAX_BRANCHES];
+    ssa_multi *multi;
+    int G, count, q;
+    int *gid, *lid;            /* series i -> branch, index within that branch */
 };
+
+static int batch_branches(int count)
+{
+    const char *v = getenv("SSA_AMD_BRANCHES");
+    int G = (v && *v) ? atoi(v) : 2;
+    if (G < 1) G = 1;
+    if (G > BATCH_MAX_BRANCHES) G = BATCH_MAX_BRANCHES;
+    if (G > count) G = count;
+    return G;
+}
+
+static double *g_sort_cost;
+static int by_cost_desc(const void *a, const void *b)
+{
+    const double ca = g_sort_cost[*(const int *)a], cb = g_sort_cost[*(const int *)b];
+    if (ca != cb) return ca < cb ? 1 : -1;
+    return *(const int *)a - *(const int *)b;
+}
 
 ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, const int *k, const int *g, int h,
                                        int p, int q, int precision, unsigned long long seed)
@@ -762,29 +788,61 @@ ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, co
     if (count < 1 || !N || !L || !k || q < 0 || h < 0 || (precision != 0 && precision != 1)) return NULL;
     if (p <= 0) p = 8;
     int *bm = (int *)malloc(sizeof(int) * count), *km = (int *)malloc(sizeof(int) * count);
-    if (!bm || !km) { free(bm); free(km); return NULL; }
+    int *ord = (int *)malloc(sizeof(int) * count);
+    double *cost = (double *)malloc(sizeof(double) * count);
+    ssa_amd_batch *b = (ssa_amd_batch *)calloc(1, sizeof(ssa_amd_batch));
+    int *sN = (int *)malloc(sizeof(int) * count), *sL = (int *)malloc(sizeof(int) * count);
+    int *sb = (int *)malloc(sizeof(int) * count), *sk = (int *)malloc(sizeof(int) * count);
+    int *sg = (int *)malloc(sizeof(int) * count);
+    int rc = -1;
+    if (!bm || !km || !ord || !cost || !b || !sN || !sL || !sb || !sk || !sg) goto out;
+    b->gid = (int *)malloc(sizeof(int) * count);
+    b->lid = (int *)malloc(sizeof(int) * count);
+    if (!b->gid || !b->lid) goto out;
     for (int i = 0; i < count; i++) {
-        if (N[i] < 4 || L[i] < 2 || L[i] > N[i] - 1 || k[i] < 1) { free(bm); free(km); return NULL; }
+        if (N[i] < 4 || L[i] < 2 || L[i] > N[i] - 1 || k[i] < 1) goto out;
         int r = imin(L[i], N[i] - L[i] + 1);
         int kp = imin(k[i] + p, r);
         bm[i] = imax(kp, 2);
         km[i] = imin(k[i], kp);
-        if (bm[i] > SSA_BMAX) { free(bm); free(km); return NULL; }
+        if (bm[i] > SSA_BMAX) goto out;
+        ord[i] = i;
+        cost[i] = (double)N[i] * bm[i];
     }
-    ssa_amd_batch *b = (ssa_amd_batch *)calloc(1, sizeof(ssa_amd_batch));
-    int err = 0;
-    if (b) b->eng = ssa_engine_create(precision, count, N, L, NULL, bm, km, q, &err);
-    if (!b || !b->eng) { free(bm); free(km); free(b); return NULL; }
+    g_sort_cost = cost;
+    qsort(ord, count, sizeof(int), by_cost_desc);
     b->count = count;
     b->q = q;
-    int rc = ssa_engine_prepare(b->eng, 1, h);
-    for (int i = 0; rc == 0 && i < count; i++) {
-        rc = ssa_engine_set_rng(b->eng, i, seed ^ (0x9E3779B97F4A7C15ULL * (unsigned long long)(i + 1)), 0);
-        if (rc == 0 && g) rc = ssa_engine_set_step(b->eng, i, imin(imax(g[i], 1), km[i]));
+    b->G = batch_branches(count);
+    for (int j = 0; j < count; j++) b->gid[ord[j]] = j % b->G;
+    rc = 0;
+    for (int br = 0; rc == 0 && br < b->G; br++) {
+        int n = 0;
+        for (int i = 0; i < count; i++) {
+            if (b->gid[i] != br) continue;
+            b->lid[i] = n;
+            sN[n] = N[i]; sL[n] = L[i]; sb[n] = bm[i]; sk[n] = km[i];
+            sg[n] = g ? imin(imax(g[i], 1), km[i]) : 0;
+            n++;
+        }
+        int err = 0;
+        b->eng[br] = ssa_engine_create(precision, n, sN, sL, NULL, sb, sk, q, &err);
+        if (!b->eng[br]) { rc = -1; break; }
+        rc = ssa_engine_prepare(b->eng[br], 1, h);
+        for (int i = 0; rc == 0 && i < count; i++) {
+            if (b->gid[i] != br) continue;
+            rc = ssa_engine_set_rng(b->eng[br], b->lid[i], seed ^ (0x9E3779B97F4A7C15ULL * (unsigned long long)(i + 1)), 0);
+            if (rc == 0 && g) rc = ssa_engine_set_step(b->eng[br], b->lid[i], sg[b->lid[i]]);
+        }
+        if (rc == 0 && h > 0) rc = ssa_engine_set_step_h(b->eng[br], h);
     }
-    if (rc == 0 && h > 0) rc = ssa_engine_set_step_h(b->eng, h);
-    free(bm);
-    free(km);
+    if (rc == 0 && b->G > 1) {
+        b->multi = ssa_multi_create(b->eng, b->G);
+        if (!b->multi) rc = -1;
+    }
+out:
+    free(bm); free(km); free(ord); free(cost);
+    free(sN); free(sL); free(sb); free(sk); free(sg);
     if (rc) { ssa_amd_batch_free(b); return NULL; }
     return b;
 }
@@ -795,48 +853,88 @@ ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const
     return ssa_amd_batch_create_ex(count, N, L, k, NULL, 0, p, q, precision, seed);
 }
 
-int ssa_amd_batch_query(ssa_amd_batch *b) { return b ? ssa_engine_query(b->eng) : -1; }
+int ssa_amd_batch_query(ssa_amd_batch *b)
+{
+    if (!b) return -1;
+    return b->multi ? ssa_multi_query(b->multi) : ssa_engine_query(b->eng[0]);
+}
 
-int ssa_amd_batch_status(ssa_amd_batch *b, int i) { return b ? ssa_engine_status(b->eng, i) : -1; }
+int ssa_amd_batch_status(ssa_amd_batch *b, int i)
+{
+    if (!b || i < 0 || i >= b->count) return -1;
+    return ssa_engine_status(b->eng[b->gid[i]], b->lid[i]);
+}
 
 int ssa_amd_batch_set_signals_f32(ssa_amd_batch *b, const float *cat, const long long *offsets)
 {
     if (!b || !cat || !offsets) return -1;
-    return ssa_engine_upload_signals_f32(b->eng, cat, offsets);
+    long long *off = (long long *)malloc(sizeof(long long) * b->count);
+    if (!off) return -1;
+    int rc = 0;
+    for (int br = 0; rc == 0 && br < b->G; br++) {
+        for (int i = 0; i < b->count; i++)
+            if (b->gid[i] == br) off[b->lid[i]] = offsets[i];
+        rc = ssa_engine_upload_signals_f32(b->eng[br], cat, off);
+    }
+    free(off);
+    return rc;
+}
+
+int ssa_amd_batch_set_key(ssa_amd_batch *b, int i, unsigned long long key)
+{
+    if (!b || i < 0 || i >= b->count) return -1;
+    return ssa_engine_set_rng(b->eng[b->gid[i]], b->lid[i], key, 0);
 }
 
 int ssa_amd_batch_run(ssa_amd_batch *b, int use_graph)
 {
     if (!b) return -1;
-    return ssa_engine_step(b->eng, b->q, use_graph);
+    return b->multi ? ssa_multi_step(b->multi, b->q, use_graph) : ssa_engine_step(b->eng[0], b->q, use_graph);
 }
 
-int ssa_amd_batch_sync(ssa_amd_batch *b) { return b ? ssa_engine_sync(b->eng) : -1; }
+int ssa_amd_batch_sync(ssa_amd_batch *b)
+{
+    if (!b) return -1;
+    return b->multi ? ssa_multi_sync(b->multi) : ssa_engine_sync(b->eng[0]);
+}
 
 int ssa_amd_batch_sigma(ssa_amd_batch *b, int i, ssa_real *sigma)
 {
     if (!b || i < 0 || i >= b->count || !sigma) return -1;
-    return ssa_engine_results(b->eng, i, sigma, NULL, NULL, NULL, NULL, NULL);
+    return ssa_engine_results(b->eng[b->gid[i]], b->lid[i], sigma, NULL, NULL, NULL, NULL, NULL);
 }
 
 int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out)
 {
     if (!b || i < 0 || i >= b->count || !out) return -1;
-    return ssa_engine_step_outputs(b->eng, i, out);
+    return ssa_engine_step_outputs(b->eng[b->gid[i]], b->lid[i], out);
 }
 
-long long ssa_amd_batch_bytes(const ssa_amd_batch *b) { return b ? ssa_engine_bytes(b->eng) : 0; }
+long long ssa_amd_batch_bytes(const ssa_amd_batch *b)
+{
+    if (!b) return 0;
+    long long t = 0;
+    for (int br = 0; br < b->G; br++) t += ssa_engine_bytes(b->eng[br]);
+    return t;
+}
 
+/* per-launch timings of one eager step, branch after branch */
 int ssa_amd_batch_profile(ssa_amd_batch *b, int max_rec, int *phase, int *lgn, float *ms)
 {
     if (!b || !phase || !lgn || !ms) return -1;
-    return ssa_engine_profile_step(b->eng, b->q, max_rec, phase, lgn, ms);
+    int n = 0;
+    for (int br = 0; br < b->G; br++) {
+        int r = ssa_engine_profile_step(b->eng[br], b->q, max_rec - n, phase + n, lgn + n, ms + n);
+        if (r < 0) return -1;
+        n += r;
+    }
+    return n;
 }
 
 int ssa_amd_batch_plan(const ssa_amd_batch *b, int i, int *out8)
 {
     if (!b || i < 0 || i >= b->count || !out8) return -1;
-    const ssa_desc *d = ssa_engine_desc(b->eng, i);
+    const ssa_desc *d = ssa_engine_desc(b->eng[b->gid[i]], b->lid[i]);
     out8[0] = d->n; out8[1] = d->CL; out8[2] = d->CK; out8[3] = d->AL; out8[4] = d->SK;
     out8[5] = d->b; out8[6] = d->k; out8[7] = d->lgn;
     return 0;
@@ -845,6 +943,9 @@ int ssa_amd_batch_plan(const ssa_amd_batch *b, int i, int *out8)
 void ssa_amd_batch_free(ssa_amd_batch *b)
 {
     if (!b) return;
-    ssa_engine_destroy(b->eng);
+    ssa_multi_destroy(b->multi);
+    for (int br = 0; br < BATCH_MAX_BRANCHES; br++) ssa_engine_destroy(b->eng[br]);
+    free(b->gid);
+    free(b->lid);
     free(b);
 }

@@ -1,0 +1,124 @@
+"""Continuous-batching SSA serving loop on one GPU (no counterpart in the
+reference, which serves one context per call).
+
+A request is one series x with its window L, components k, a reconstruction
+group {0..g-1} and a forecast horizon h: decompose (randomized, q power
+iterations) + reconstruct the group + R-forecast h steps (`ssa_opt_forecast`,
+MKL/ssa_opt.h:2843-2867, on the group's reconstruction).
+
+The server gathers the requests that have arrived into a wave (at most
+`max_wave` series), runs the wave as one batched step on the device (one
+`ssa_amd_batch`, mixed or plain form) and hands every request its output.
+Up to `inflight` waves are outstanding: while one runs on the GPU the next is
+assembled and uploaded.  Each request carries its own Philox key (its id), so
+its test matrix, and hence its result, does not depend on the wave it lands
+in.
+"""
+import heapq
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import ssa_amd
+
+
+@dataclass(order=True)
+class Request:
+    arrival: float
+    rid: int
+    x: np.ndarray = field(compare=False)
+    L: int = field(compare=False)
+    k: int = field(compare=False)
+    g: int = field(compare=False)
+    h: int = field(compare=False)
+    done: float = field(default=-1.0, compare=False)
+    out: np.ndarray = field(default=None, compare=False)
+    status: int = field(default=0, compare=False)
+
+
+def request_key(rid, seed=6):
+    return (seed * 0x100000001B3 + 0x9E3779B97F4A7C15 * (rid + 1)) & ((1 << 64) - 1)
+
+
+class Server:
+    """Waves of requests through `ssa_amd.Batch` (all requests of a wave share
+    h, p, q and the precision)."""
+
+    def __init__(self, p=10, q=2, h=50, precision=1, max_wave=512, inflight=2, seed=6):
+        self.p, self.q, self.h, self.precision = p, q, h, precision
+        self.max_wave, self.inflight, self.seed = max_wave, inflight, seed
+        self.waves = []          # (batch, requests) in flight
+        self.wave_sizes = []
+
+    def _launch(self, reqs):
+        b = ssa_amd.Batch([r.x for r in reqs], [r.L for r in reqs], [r.k for r in reqs], p=self.p, q=self.q,
+                          precision=self.precision, groups=[r.g for r in reqs], h=self.h,
+                          keys=[request_key(r.rid, self.seed) for r in reqs])
+        b.run(use_graph=False)
+        self.waves.append((b, reqs))
+        self.wave_sizes.append(len(reqs))
+
+    def _retire(self, block):
+        b, reqs = self.waves[0]
+        if not block and not b.query():
+            return False
+        b.sync()
+        now = time.perf_counter()
+        for i, r in enumerate(reqs):
+            r.out = b.output(i)
+            r.status = b.status(i)
+            r.done = now
+        self.waves.pop(0)
+        return True
+
+    def run(self, requests, t0=None):
+        """Serve `requests` (arrival times relative to t0, seconds) in real
+        time; returns them with completion times set."""
+        pending = list(requests)
+        heapq.heapify(pending)
+        t0 = time.perf_counter() if t0 is None else t0
+        waiting = []
+        finished = []
+        while pending or waiting or self.waves:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(heapq.heappop(pending))
+            if waiting and len(self.waves) < self.inflight:
+                wave, waiting = waiting[:self.max_wave], waiting[self.max_wave:]
+                self._launch(wave)
+                finished.extend(wave)
+                continue
+            if self.waves and self._retire(block=False):
+                continue
+            # nothing ready: sleep until the next arrival or poll the oldest wave
+            nap = 2e-4
+            if pending and not self.waves:
+                nap = max(0.0, min(0.01, pending[0].arrival - now))
+            time.sleep(nap)
+        for r in finished:
+            r.done -= t0
+        return finished
+
+
+def synthetic_requests(count, rate, n_lo=500, n_hi=50000, k_lo=10, k_hi=50, h=50, seed=6, t_start=0.0):
+    """Open-loop Poisson arrivals at `rate` per second: N log-uniform in
+    [n_lo, n_hi], L = N // 4, k ~ U{k_lo..k_hi}, group {0..floor(2k/5)-1},
+    signals of the benchmark family (FP32).  rate = inf: all at t_start."""
+    import workloads
+    xs, Ns, Ls, ks = workloads.ragged_batch(count=count, n_lo=n_lo, n_hi=n_hi, k_lo=k_lo, k_hi=k_hi, seed=seed,
+                                            dtype=np.float32)
+    rng = np.random.default_rng(seed + 1000)
+    if np.isinf(rate):
+        arrivals = np.full(count, t_start)
+    else:
+        arrivals = t_start + np.cumsum(rng.exponential(1.0 / rate, size=count))
+    return [Request(float(arrivals[i]), i, xs[i], int(Ls[i]), int(ks[i]), max(1, (2 * int(ks[i])) // 5), h)
+            for i in range(count)]
+
+
+def latency_stats(reqs):
+    lat = np.array([r.done - r.arrival for r in reqs])
+    return {"p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(1e3 * float(lat.mean()), 3), "requests": len(reqs)}

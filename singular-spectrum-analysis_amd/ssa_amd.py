@@ -85,6 +85,7 @@ SIGNATURES = [
     ("ssa_amd_plan_info", _I, [_P, c_int_p]),
     ("ssa_amd_batch_create", _VP, [_I, c_int_p, c_int_p, c_int_p, _I, _I, _I, _U64]),
     ("ssa_amd_batch_set_signals_f32", _I, [_VP, c_float_p, c_ll_p]),
+    ("ssa_amd_batch_set_key", _I, [_VP, _I, _U64]),
     ("ssa_amd_batch_run", _I, [_VP, _I]),
     ("ssa_amd_batch_sync", _I, [_VP]),
     ("ssa_amd_batch_sigma", _I, [_VP, _I, c_double_p]),
@@ -389,9 +390,10 @@ class MSSA:
 class Batch:
     """Many uneven series in one device engine (no counterpart in the reference)."""
 
-    def __init__(self, xs, Ls, ks, p=10, q=2, precision=1, seed=4, groups=None, h=0):
+    def __init__(self, xs, Ls, ks, p=10, q=2, precision=1, seed=4, groups=None, h=0, keys=None):
         """groups[i]: reconstruct (and forecast) components 0..groups[i]-1
-        of series i (default all k); h > 0 appends an h-step LRF forecast."""
+        of series i (default all k); h > 0 appends an h-step LRF forecast;
+        keys[i]: Philox key of series i (default derived from seed and i)."""
         self._l = lib()
         self.count = len(xs)
         self.horizon = int(h)
@@ -409,6 +411,10 @@ class Batch:
         if self._l.ssa_amd_batch_set_signals_f32(self._h, cat.ctypes.data_as(c_float_p),
                                                  offs.ctypes.data_as(c_ll_p)) != 0:
             raise RuntimeError("batch upload failed")
+        if keys is not None:
+            for i, key in enumerate(keys):
+                if self._l.ssa_amd_batch_set_key(self._h, i, int(key) & ((1 << 64) - 1)) != 0:
+                    raise RuntimeError("batch key failed")
 
     def __del__(self):
         if getattr(self, "_h", None):
