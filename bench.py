@@ -188,6 +188,48 @@ def serving(seconds):
             "p50_in_mix_over_alone": round(mix["p50_ms"] / solo["p50_ms"], 3)}
 
 
+def single_configs(reps=3):
+    """cfg2: one series N = 20000, L = 5000, k = 50, p = 10, q = 2:
+    decompose + the 50 single-component reconstructions + W-correlation
+    (median of `reps`), plain and mixed, and the singular-value error of both
+    forms against a dense SVD of the explicit Hankel matrix at N = 4000,
+    L = 1000.  cfg4: MSSA M = 16, N = 10000, L = 2500, k = 64, p = 10:
+    decompose + per-channel reconstruction of components 0..8."""
+    out = {}
+    x = workloads.single(20000, seed=2)
+    groups = [[i] for i in range(50)]
+    for prec, name in ((0, "plain"), (1, "mixed")):
+        s = ssa_amd.SSA(x, 5000, q=Q_POWER, precision=prec, seed=12)
+        t = []
+        for r in range(reps + 1):
+            a = time.perf_counter()
+            s.decompose(50, oversampling=P_OVERSAMPLE)
+            s.reconstruct_many(groups)
+            s.wcorr_matrix()
+            if r:
+                t.append(time.perf_counter() - a)
+        out[f"cfg2_{name}_ms"] = round(1e3 * float(np.median(t)), 2)
+    xd = workloads.single(4000, seed=3)
+    H = np.lib.stride_tricks.sliding_window_view(xd, 3001)[:1000]
+    exact = np.linalg.svd(H, compute_uv=False)[:50]
+    for prec, name in ((0, "plain"), (1, "mixed")):
+        s = ssa_amd.SSA(xd, 1000, q=Q_POWER, precision=prec, seed=13).decompose(50, oversampling=P_OVERSAMPLE)
+        out[f"dense_sigma_err_{name}"] = float(np.max(np.abs(s.singular_values - exact) / exact))
+    X = workloads.mssa_channels(M=16, N=10000, seed=5)
+    for prec, name in ((0, "plain"), (1, "mixed")):
+        m = ssa_amd.MSSA(X, 2500, q=Q_POWER, precision=prec)
+        t = []
+        for r in range(reps + 1):
+            a = time.perf_counter()
+            m.decompose(64, oversampling=P_OVERSAMPLE)
+            for ch in range(16):
+                m.reconstruct(ch, list(range(9)))
+            if r:
+                t.append(time.perf_counter() - a)
+        out[f"cfg4_mssa_{name}_ms"] = round(1e3 * float(np.median(t)), 2)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +242,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this SSA path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -244,6 +287,11 @@ def main():
               f"p50 {serving_out['alone']['p50_ms']} ms", flush=True)
 
 
+    configs_out = None
+    if not args.no_configs:
+        configs_out = single_configs()
+        print(f"[bench] single series: {configs_out}", flush=True)
+
     ms_step = 1e3 * total / args.steps
     out = {
         "metric": "SSA/sec batched (N=500-20k, k=50) + p50/p99 latency; mixed vs FP64 plain",
@@ -271,6 +319,7 @@ def main():
     }
     out.update(roof)
     out["serving"] = serving_out
+    out["single_series"] = configs_out
     out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 

@@ -185,6 +185,10 @@ ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const
  * k) and, when h > 0, an h-step LRF forecast of that group appended */
 ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, const int *k, const int *g, int h,
                                        int p, int q, int precision, unsigned long long seed);
+/* re-target an idle batch to a new set of series (same p, q, h, precision,
+ * seed); device buffers are reused when large enough, so a serving loop that
+ * rebinds a few batches allocates nothing per wave */
+int ssa_amd_batch_rebind(ssa_amd_batch *b, int count, const int *N, const int *L, const int *k, const int *g);
 /* 1 when the batch's queued work has finished, 0 if still running, -1 on error */
 int ssa_amd_batch_query(ssa_amd_batch *b);
 /* device status of series i after a step (0 ok, 2 forecast verticality >= 1) */
@@ -200,6 +204,9 @@ int ssa_amd_batch_run(ssa_amd_batch *b, int use_graph);
 int ssa_amd_batch_sync(ssa_amd_batch *b);
 int ssa_amd_batch_sigma(ssa_amd_batch *b, int i, ssa_real *sigma);       /* returns k_i */
 int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out);         /* N_i (+ h) values */
+/* every series' output at out + offsets[i] (N_i (+ h) values each) and, when
+ * status is not NULL, its device status word; one synchronisation per branch */
+int ssa_amd_batch_outputs(ssa_amd_batch *b, ssa_real *out, const long long *offsets, int *status);
 long long ssa_amd_batch_bytes(const ssa_amd_batch *b);
 /* one eager step with HIP events around every kernel launch: per launch the
  * phase (0 spectra, 1 matvec H, 2 matvec H^T, 3 Gram, 4 Cholesky, 5 apply,

@@ -41,6 +41,9 @@ struct ssa_engine {
     void *arena_a;             /* signals + spectra (base pointer of all offsets) */
     void *arena_b;             /* panels, staging, outputs */
     long long a_elems, b_elems;
+    long long a_cap, b_cap, small_cap;   /* allocated elements (rebind reuses them) */
+    int dd_cap, host_cap;                /* allocated descriptors (device, host) */
+    int q_hint;
     double *small;
     long long small_elems;
     void *tw;
@@ -307,6 +310,65 @@ static int sync_desc(ssa_engine *e)
 /* lifecycle                                                                 */
 /* ------------------------------------------------------------------------ */
 
+/* descriptors, chunk plans and the signal/spectra arena for a set of series;
+ * device buffers are reused when large enough */
+static int engine_layout(ssa_engine *e, int count, const int *N, const int *L, const int *M, const int *bmax,
+                         const int *kmax)
+{
+    if (count < 1) return -1;
+    if (count > e->host_cap) {
+        free(e->hd); free(e->bmax); free(e->kmax);
+        e->hd = (ssa_desc *)calloc(count, sizeof(ssa_desc));
+        e->bmax = (int *)calloc(count, sizeof(int));
+        e->kmax = (int *)calloc(count, sizeof(int));
+        if (!e->hd || !e->bmax || !e->kmax) { e->host_cap = 0; return -1; }
+        e->host_cap = count;
+    } else {
+        memset(e->hd, 0, sizeof(ssa_desc) * count);
+    }
+    e->count = count;
+    long long off = 0;
+    for (int s = 0; s < count; s++) {
+        ssa_desc *d = &e->hd[s];
+        int m = M ? M[s] : 1;
+        if (N[s] < 4 || L[s] < 2 || L[s] > N[s] - 1 || m < 1) return -1;
+        d->N = N[s]; d->L = L[s]; d->K = N[s] - L[s] + 1; d->M = m; d->ML = m * L[s];
+        e->bmax[s] = bmax ? bmax[s] : 0;
+        e->kmax[s] = kmax ? kmax[s] : 0;
+        if (e->bmax[s] > SSA_BMAX) return -1;
+        ssa_plan pl;
+        int bb = e->bmax[s] > 0 ? e->bmax[s] : 60, kk = e->kmax[s] > 0 ? e->kmax[s] : 50;
+        if (ssa_plan_series(e->prec, d->N, d->L, m, bb, kk, e->q_hint, -1, &pl)) return -1;
+        d->n = pl.n; d->lgn = pl.lgn; d->CL = pl.CL; d->CK = pl.CK; d->AL = pl.AL; d->SK = pl.SK;
+        d->tilesY = ceil_div(d->ML, SSA_GRAM_ROWS);
+        d->tilesZ = ceil_div(d->K, SSA_GRAM_ROWS);
+        d->seed = 42;
+        d->off_x = off; off = align64(off + (long long)m * d->N);
+        d->off_spec = off; off = align64(off + (long long)m * d->AL * d->SK * 2 * d->n);
+    }
+    e->a_elems = off;
+    if (off > e->a_cap || !e->arena_a) {
+        if (e->arena_a) { hipFree(e->arena_a); e->arena_a = NULL; e->a_cap = 0; }
+        const long long cap = off > 64 ? off + off / 4 : 64;
+        HIPCHK(hipMalloc(&e->arena_a, e->esz * (size_t)cap));
+        e->a_cap = cap;
+    }
+    if (count > e->dd_cap) {
+        if (e->dd) { hipFree(e->dd); e->dd = NULL; e->dd_cap = 0; }
+        HIPCHK(hipMalloc((void **)&e->dd, sizeof(ssa_desc) * count));
+        e->dd_cap = count;
+    }
+    e->prepared = 0;
+    e->max_groups = 0;
+    e->max_h = 0;
+    e->step_h = 0;
+    e->lists_dirty = 1;
+    e->desc_dirty = 1;
+    e->step_groups_ready = 0;
+    if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+    return 0;
+}
+
 ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, const int *M, const int *bmax,
                               const int *kmax, int q_hint, int *err)
 {
@@ -317,41 +379,14 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
     ssa_engine *e = (ssa_engine *)calloc(1, sizeof(ssa_engine));
     if (!e) return NULL;
     e->prec = prec;
-    e->count = count;
     e->esz = prec == SSA_PREC_F64 ? 8 : 4;
-    e->hd = (ssa_desc *)calloc(count, sizeof(ssa_desc));
-    e->bmax = (int *)calloc(count, sizeof(int));
-    e->kmax = (int *)calloc(count, sizeof(int));
-    if (!e->hd || !e->bmax || !e->kmax) goto fail;
+    e->q_hint = q_hint;
     if (hipStreamCreate(&e->st) != hipSuccess) goto fail;
     if (!g_tw[prec]) {
         if (ssa_dev_twiddles(prec, &g_tw[prec], e->st) != 0) goto fail;
     }
     e->tw = g_tw[prec];
-    long long off = 0;
-    for (int s = 0; s < count; s++) {
-        ssa_desc *d = &e->hd[s];
-        int m = M ? M[s] : 1;
-        if (N[s] < 4 || L[s] < 2 || L[s] > N[s] - 1 || m < 1) goto fail;
-        d->N = N[s]; d->L = L[s]; d->K = N[s] - L[s] + 1; d->M = m; d->ML = m * L[s];
-        e->bmax[s] = bmax ? bmax[s] : 0;
-        e->kmax[s] = kmax ? kmax[s] : 0;
-        if (e->bmax[s] > SSA_BMAX) goto fail;
-        ssa_plan pl;
-        int bb = e->bmax[s] > 0 ? e->bmax[s] : 60, kk = e->kmax[s] > 0 ? e->kmax[s] : 50;
-        if (ssa_plan_series(prec, d->N, d->L, m, bb, kk, q_hint, -1, &pl)) goto fail;
-        d->n = pl.n; d->lgn = pl.lgn; d->CL = pl.CL; d->CK = pl.CK; d->AL = pl.AL; d->SK = pl.SK;
-        d->tilesY = ceil_div(d->ML, SSA_GRAM_ROWS);
-        d->tilesZ = ceil_div(d->K, SSA_GRAM_ROWS);
-        d->seed = 42;
-        d->off_x = off; off = align64(off + (long long)m * d->N);
-        d->off_spec = off; off = align64(off + (long long)m * d->AL * d->SK * 2 * d->n);
-    }
-    e->a_elems = off;
-    if (hipMalloc(&e->arena_a, e->esz * (size_t)(off > 0 ? off : 64)) != hipSuccess) goto fail;
-    if (hipMalloc((void **)&e->dd, sizeof(ssa_desc) * count) != hipSuccess) goto fail;
-    e->lists_dirty = 1;
-    e->desc_dirty = 1;
+    if (engine_layout(e, count, N, L, M, bmax, kmax)) goto fail;
     {
         const char *v = getenv("SSA_AMD_CHOLQR2_ALL");
         e->cholqr2_all = (v && *v && *v != '0') ? 1 : 0;
@@ -368,6 +403,14 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
 fail:
     ssa_engine_destroy(e);
     return NULL;
+}
+
+int ssa_engine_rebind(ssa_engine *e, int count, const int *N, const int *L, const int *M, const int *bmax,
+                      const int *kmax)
+{
+    if (!e || !N || !L) return -1;
+    HIPCHK(hipStreamSynchronize(e->st));
+    return engine_layout(e, count, N, L, M, bmax, kmax);
 }
 
 void ssa_engine_destroy(ssa_engine *e)
@@ -472,8 +515,6 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
     if (max_groups < e->max_groups) max_groups = e->max_groups;
     if (max_h < e->max_h) max_h = e->max_h;
     HIPCHK(hipStreamSynchronize(e->st));
-    if (e->arena_b) { hipFree(e->arena_b); e->arena_b = NULL; }
-    if (e->small) { hipFree(e->small); e->small = NULL; }
     long long off = 0, soff = 0;
     for (int s = 0; s < e->count; s++) {
         ssa_desc *d = &e->hd[s];
@@ -497,8 +538,18 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
         d->off_part = soff; soff = align64(soff + part);
         d->off_small = soff; soff = align64(soff + (long long)SSA_NSLOTS * SSA_BMAX * SSA_BMAX);
     }
-    HIPCHK(hipMalloc(&e->arena_b, e->esz * (size_t)(off > 0 ? off : 64)));
-    HIPCHK(hipMalloc((void **)&e->small, 8 * (size_t)soff));
+    if (off > e->b_cap || !e->arena_b) {
+        if (e->arena_b) { hipFree(e->arena_b); e->arena_b = NULL; e->b_cap = 0; }
+        const long long cap = off > 64 ? off + off / 4 : 64;
+        HIPCHK(hipMalloc(&e->arena_b, e->esz * (size_t)cap));
+        e->b_cap = cap;
+    }
+    if (soff > e->small_cap || !e->small) {
+        if (e->small) { hipFree(e->small); e->small = NULL; e->small_cap = 0; }
+        const long long cap = soff > 64 ? soff + soff / 4 : 64;
+        HIPCHK(hipMalloc((void **)&e->small, 8 * (size_t)cap));
+        e->small_cap = cap;
+    }
     e->b_elems = off;
     e->small_elems = soff;
     /* rebase panel offsets on arena_a */
@@ -1197,6 +1248,38 @@ int ssa_engine_profile_step(ssa_engine *e, int q, int max_rec, int *phase, int *
         HIPCHK(hipEventElapsedTime(&ms[i], e->prof_ev[2 * i], e->prof_ev[2 * i + 1]));
     }
     return e->prof_n;
+}
+
+/* every series' step output (N + step h values each, FP64) into out at
+ * offsets[s], and the device status words, with one synchronisation */
+int ssa_engine_step_outputs_all(ssa_engine *e, double *out, const long long *offsets, int *status)
+{
+    if (!e->prepared || !out || !offsets) return -1;
+    long long total = 0;
+    for (int s = 0; s < e->count; s++) total += e->hd[s].N + e->step_h;
+    const size_t dbytes = status ? sizeof(ssa_desc) * (size_t)e->count : 0;
+    DEVCHK(pin_reserve(e, (size_t)total * e->esz + dbytes));
+    long long at = 0;
+    for (int s = 0; s < e->count; s++) {
+        const ssa_desc *d = &e->hd[s];
+        const long long n = d->N + e->step_h;
+        HIPCHK(hipMemcpyAsync((char *)e->hpin + at * e->esz, (char *)e->arena_a + e->esz * d->off_out, n * e->esz,
+                              hipMemcpyDeviceToHost, e->st));
+        at += n;
+    }
+    ssa_desc *hd = (ssa_desc *)((char *)e->hpin + (size_t)total * e->esz);
+    if (status) HIPCHK(hipMemcpyAsync(hd, e->dd, dbytes, hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    at = 0;
+    for (int s = 0; s < e->count; s++) {
+        const long long n = e->hd[s].N + e->step_h;
+        double *o = out + offsets[s];
+        if (e->prec == SSA_PREC_F64) memcpy(o, (double *)e->hpin + at, n * 8);
+        else for (long long i = 0; i < n; i++) o[i] = ((float *)e->hpin)[at + i];
+        if (status) status[s] = hd[s].status;
+        at += n;
+    }
+    return 0;
 }
 
 int ssa_engine_step_outputs(ssa_engine *e, int s, double *out)

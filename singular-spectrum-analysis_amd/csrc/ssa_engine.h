@@ -30,6 +30,12 @@ int ssa_plan_series(int prec, int N, int L, int M, int b, int k, int q, int lgn_
 ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, const int *M,
                               const int *bmax, const int *kmax, int q_hint, int *err);
 void ssa_engine_destroy(ssa_engine *e);
+/* re-target an idle engine to a new set of series (as create); device
+ * buffers are kept when large enough, so a serving loop allocates nothing
+ * per wave once its engines have grown to the largest wave.  Call
+ * ssa_engine_prepare (and the per-series setters) again afterwards. */
+int ssa_engine_rebind(ssa_engine *e, int count, const int *N, const int *L, const int *M, const int *bmax,
+                      const int *kmax);
 int ssa_engine_count(const ssa_engine *e);
 int ssa_engine_precision(const ssa_engine *e);
 const ssa_desc *ssa_engine_desc(const ssa_engine *e, int s);
@@ -81,6 +87,8 @@ int ssa_engine_matvec(ssa_engine *e, int s, int dir, const double *in, double *o
  * graph on first use when use_graph is set */
 int ssa_engine_step(ssa_engine *e, int q, int use_graph);
 int ssa_engine_step_outputs(ssa_engine *e, int s, double *out);   /* N (+ step h) values */
+/* all series: outputs at out + offsets[s], device status words (may be NULL), one sync */
+int ssa_engine_step_outputs_all(ssa_engine *e, double *out, const long long *offsets, int *status);
 /* group {0..gsize-1} reconstructed (and forecast) by the batched step for series s */
 int ssa_engine_set_step(ssa_engine *e, int s, int gsize);
 /* forecast horizon of the batched step (0: none; <= max_h of prepare) */

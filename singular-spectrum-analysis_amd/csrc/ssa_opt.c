@@ -760,17 +760,17 @@ ssa_real mssa_opt_variance_explained(const MSSA_Opt *mssa, int start, int end)
 struct ssa_amd_batch {
     ssa_engine *eng[BATCH_MAX_BRANCHES];
     ssa_multi *multi;
-    int G, count, q;
+    int Gmax, G, count, cap, q, p, h, precision;
+    unsigned long long seed;
     int *gid, *lid;            /* series i -> branch, index within that branch */
 };
 
-static int batch_branches(int count)
+static int batch_branches(void)
 {
     const char *v = getenv("SSA_AMD_BRANCHES");
     int G = (v && *v) ? atoi(v) : 2;
     if (G < 1) G = 1;
     if (G > BATCH_MAX_BRANCHES) G = BATCH_MAX_BRANCHES;
-    if (G > count) G = count;
     return G;
 }
 
@@ -782,23 +782,28 @@ static int by_cost_desc(const void *a, const void *b)
     return *(const int *)a - *(const int *)b;
 }
 
-ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, const int *k, const int *g, int h,
-                                       int p, int q, int precision, unsigned long long seed)
+/* deal the series to the branches (round-robin in order of decreasing cost)
+ * and (re)build each branch's engine; engines and their device buffers are
+ * reused across calls */
+static int batch_bind(ssa_amd_batch *b, int count, const int *N, const int *L, const int *k, const int *g)
 {
-    if (count < 1 || !N || !L || !k || q < 0 || h < 0 || (precision != 0 && precision != 1)) return NULL;
-    if (p <= 0) p = 8;
+    if (count < 1 || !N || !L || !k) return -1;
+    const int p = b->p;
+    int rc = -1;
     int *bm = (int *)malloc(sizeof(int) * count), *km = (int *)malloc(sizeof(int) * count);
     int *ord = (int *)malloc(sizeof(int) * count);
     double *cost = (double *)malloc(sizeof(double) * count);
-    ssa_amd_batch *b = (ssa_amd_batch *)calloc(1, sizeof(ssa_amd_batch));
     int *sN = (int *)malloc(sizeof(int) * count), *sL = (int *)malloc(sizeof(int) * count);
     int *sb = (int *)malloc(sizeof(int) * count), *sk = (int *)malloc(sizeof(int) * count);
     int *sg = (int *)malloc(sizeof(int) * count);
-    int rc = -1;
-    if (!bm || !km || !ord || !cost || !b || !sN || !sL || !sb || !sk || !sg) goto out;
-    b->gid = (int *)malloc(sizeof(int) * count);
-    b->lid = (int *)malloc(sizeof(int) * count);
-    if (!b->gid || !b->lid) goto out;
+    if (!bm || !km || !ord || !cost || !sN || !sL || !sb || !sk || !sg) goto out;
+    if (count > b->cap) {
+        free(b->gid); free(b->lid);
+        b->gid = (int *)malloc(sizeof(int) * count);
+        b->lid = (int *)malloc(sizeof(int) * count);
+        b->cap = (b->gid && b->lid) ? count : 0;
+        if (!b->cap) goto out;
+    }
     for (int i = 0; i < count; i++) {
         if (N[i] < 4 || L[i] < 2 || L[i] > N[i] - 1 || k[i] < 1) goto out;
         int r = imin(L[i], N[i] - L[i] + 1);
@@ -812,8 +817,7 @@ ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, co
     g_sort_cost = cost;
     qsort(ord, count, sizeof(int), by_cost_desc);
     b->count = count;
-    b->q = q;
-    b->G = batch_branches(count);
+    b->G = imin(b->Gmax, count);
     for (int j = 0; j < count; j++) b->gid[ord[j]] = j % b->G;
     rc = 0;
     for (int br = 0; rc == 0 && br < b->G; br++) {
@@ -825,17 +829,25 @@ ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, co
             sg[n] = g ? imin(imax(g[i], 1), km[i]) : 0;
             n++;
         }
-        int err = 0;
-        b->eng[br] = ssa_engine_create(precision, n, sN, sL, NULL, sb, sk, q, &err);
-        if (!b->eng[br]) { rc = -1; break; }
-        rc = ssa_engine_prepare(b->eng[br], 1, h);
+        if (!b->eng[br]) {
+            int err = 0;
+            b->eng[br] = ssa_engine_create(b->precision, n, sN, sL, NULL, sb, sk, b->q, &err);
+            if (!b->eng[br]) { rc = -1; break; }
+        } else if (ssa_engine_rebind(b->eng[br], n, sN, sL, NULL, sb, sk)) {
+            rc = -1;
+            break;
+        }
+        rc = ssa_engine_prepare(b->eng[br], 1, b->h);
         for (int i = 0; rc == 0 && i < count; i++) {
             if (b->gid[i] != br) continue;
-            rc = ssa_engine_set_rng(b->eng[br], b->lid[i], seed ^ (0x9E3779B97F4A7C15ULL * (unsigned long long)(i + 1)), 0);
+            rc = ssa_engine_set_rng(b->eng[br], b->lid[i],
+                                    b->seed ^ (0x9E3779B97F4A7C15ULL * (unsigned long long)(i + 1)), 0);
             if (rc == 0 && g) rc = ssa_engine_set_step(b->eng[br], b->lid[i], sg[b->lid[i]]);
         }
-        if (rc == 0 && h > 0) rc = ssa_engine_set_step_h(b->eng[br], h);
+        if (rc == 0 && b->h > 0) rc = ssa_engine_set_step_h(b->eng[br], b->h);
     }
+    ssa_multi_destroy(b->multi);
+    b->multi = NULL;
     if (rc == 0 && b->G > 1) {
         b->multi = ssa_multi_create(b->eng, b->G);
         if (!b->multi) rc = -1;
@@ -843,8 +855,31 @@ ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, co
 out:
     free(bm); free(km); free(ord); free(cost);
     free(sN); free(sL); free(sb); free(sk); free(sg);
-    if (rc) { ssa_amd_batch_free(b); return NULL; }
+    return rc;
+}
+
+ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, const int *k, const int *g, int h,
+                                       int p, int q, int precision, unsigned long long seed)
+{
+    if (count < 1 || !N || !L || !k || q < 0 || h < 0 || (precision != 0 && precision != 1)) return NULL;
+    ssa_amd_batch *b = (ssa_amd_batch *)calloc(1, sizeof(ssa_amd_batch));
+    if (!b) return NULL;
+    b->p = p <= 0 ? 8 : p;
+    b->q = q;
+    b->h = h;
+    b->precision = precision;
+    b->seed = seed;
+    b->Gmax = batch_branches();
+    if (batch_bind(b, count, N, L, k, g)) { ssa_amd_batch_free(b); return NULL; }
     return b;
+}
+
+int ssa_amd_batch_rebind(ssa_amd_batch *b, int count, const int *N, const int *L, const int *k, const int *g)
+{
+    if (!b) return -1;
+    if (b->multi) ssa_multi_sync(b->multi);
+    for (int br = 0; br < b->G; br++) ssa_engine_sync(b->eng[br]);
+    return batch_bind(b, count, N, L, k, g);
 }
 
 ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const int *k, int p, int q,
@@ -908,6 +943,26 @@ int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out)
 {
     if (!b || i < 0 || i >= b->count || !out) return -1;
     return ssa_engine_step_outputs(b->eng[b->gid[i]], b->lid[i], out);
+}
+
+int ssa_amd_batch_outputs(ssa_amd_batch *b, ssa_real *out, const long long *offsets, int *status)
+{
+    if (!b || !out || !offsets) return -1;
+    long long *off = (long long *)malloc(sizeof(long long) * b->count);
+    int *st = status ? (int *)malloc(sizeof(int) * b->count) : NULL;
+    if (!off || (status && !st)) { free(off); free(st); return -1; }
+    int rc = 0;
+    for (int br = 0; rc == 0 && br < b->G; br++) {
+        for (int i = 0; i < b->count; i++)
+            if (b->gid[i] == br) off[b->lid[i]] = offsets[i];
+        rc = ssa_engine_step_outputs_all(b->eng[br], out, off, st);
+        if (rc == 0 && status)
+            for (int i = 0; i < b->count; i++)
+                if (b->gid[i] == br) status[i] = st[b->lid[i]];
+    }
+    free(off);
+    free(st);
+    return rc;
 }
 
 long long ssa_amd_batch_bytes(const ssa_amd_batch *b)

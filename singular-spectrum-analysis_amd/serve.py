@@ -10,7 +10,9 @@ The server gathers the requests that have arrived into a wave (at most
 `max_wave` series), runs the wave as one batched step on the device (one
 `ssa_amd_batch`, mixed or plain form) and hands every request its output.
 Up to `inflight` waves are outstanding: while one runs on the GPU the next is
-assembled and uploaded.  Each request carries its own Philox key (its id), so
+assembled and uploaded.  The server keeps one batch per in-flight slot and
+re-targets it to each new wave (`ssa_amd_batch_rebind`), so once the slots
+have grown to the largest wave no device memory is allocated or freed.  Each request carries its own Philox key (its id), so
 its test matrix, and hence its result, does not depend on the wave it lands
 in.
 """
@@ -49,12 +51,18 @@ class Server:
         self.p, self.q, self.h, self.precision = p, q, h, precision
         self.max_wave, self.inflight, self.seed = max_wave, inflight, seed
         self.waves = []          # (batch, requests) in flight
+        self.idle = []           # batches ready for the next wave
         self.wave_sizes = []
 
     def _launch(self, reqs):
-        b = ssa_amd.Batch([r.x for r in reqs], [r.L for r in reqs], [r.k for r in reqs], p=self.p, q=self.q,
-                          precision=self.precision, groups=[r.g for r in reqs], h=self.h,
-                          keys=[request_key(r.rid, self.seed) for r in reqs])
+        xs, Ls, ks = [r.x for r in reqs], [r.L for r in reqs], [r.k for r in reqs]
+        gs, keys = [r.g for r in reqs], [request_key(r.rid, self.seed) for r in reqs]
+        if self.idle:
+            b = self.idle.pop()
+            b.rebind(xs, Ls, ks, groups=gs, keys=keys)
+        else:
+            b = ssa_amd.Batch(xs, Ls, ks, p=self.p, q=self.q, precision=self.precision, groups=gs, h=self.h,
+                              keys=keys)
         b.run(use_graph=False)
         self.waves.append((b, reqs))
         self.wave_sizes.append(len(reqs))
@@ -64,12 +72,14 @@ class Server:
         if not block and not b.query():
             return False
         b.sync()
+        outs, status = b.outputs()
         now = time.perf_counter()
         for i, r in enumerate(reqs):
-            r.out = b.output(i)
-            r.status = b.status(i)
+            r.out = outs[i]
+            r.status = int(status[i])
             r.done = now
         self.waves.pop(0)
+        self.idle.append(b)
         return True
 
     def run(self, requests, t0=None):

@@ -86,6 +86,8 @@ SIGNATURES = [
     ("ssa_amd_batch_create", _VP, [_I, c_int_p, c_int_p, c_int_p, _I, _I, _I, _U64]),
     ("ssa_amd_batch_set_signals_f32", _I, [_VP, c_float_p, c_ll_p]),
     ("ssa_amd_batch_set_key", _I, [_VP, _I, _U64]),
+    ("ssa_amd_batch_rebind", _I, [_VP, _I, c_int_p, c_int_p, c_int_p, c_int_p]),
+    ("ssa_amd_batch_outputs", _I, [_VP, c_double_p, c_ll_p, c_int_p]),
     ("ssa_amd_batch_run", _I, [_VP, _I]),
     ("ssa_amd_batch_sync", _I, [_VP]),
     ("ssa_amd_batch_sigma", _I, [_VP, _I, c_double_p]),
@@ -395,16 +397,22 @@ class Batch:
         of series i (default all k); h > 0 appends an h-step LRF forecast;
         keys[i]: Philox key of series i (default derived from seed and i)."""
         self._l = lib()
-        self.count = len(xs)
         self.horizon = int(h)
-        self.N = np.ascontiguousarray([len(x) for x in xs], dtype=np.int32)
-        self.L = np.ascontiguousarray(Ls, dtype=np.int32)
-        self.k = np.ascontiguousarray(ks, dtype=np.int32)
+        self._dims(xs, Ls, ks)
         g = None if groups is None else np.ascontiguousarray(groups, dtype=np.int32)
         self._h = self._l.ssa_amd_batch_create_ex(self.count, _ip(self.N), _ip(self.L), _ip(self.k),
                                                  None if g is None else _ip(g), self.horizon, p, q, precision, seed)
         if not self._h:
             raise RuntimeError("batch creation failed")
+        self._upload(xs, keys)
+
+    def _dims(self, xs, Ls, ks):
+        self.count = len(xs)
+        self.N = np.ascontiguousarray([len(x) for x in xs], dtype=np.int32)
+        self.L = np.ascontiguousarray(Ls, dtype=np.int32)
+        self.k = np.ascontiguousarray(ks, dtype=np.int32)
+
+    def _upload(self, xs, keys):
         cat = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float32) for x in xs]))
         offs = np.zeros(self.count, dtype=np.int64)
         offs[1:] = np.cumsum(self.N[:-1])
@@ -415,6 +423,16 @@ class Batch:
             for i, key in enumerate(keys):
                 if self._l.ssa_amd_batch_set_key(self._h, i, int(key) & ((1 << 64) - 1)) != 0:
                     raise RuntimeError("batch key failed")
+
+    def rebind(self, xs, Ls, ks, groups=None, keys=None):
+        """Re-target this (idle) batch to new series with the same p, q, h,
+        precision and seed; device buffers are reused when large enough."""
+        self._dims(xs, Ls, ks)
+        g = None if groups is None else np.ascontiguousarray(groups, dtype=np.int32)
+        if self._l.ssa_amd_batch_rebind(self._h, self.count, _ip(self.N), _ip(self.L), _ip(self.k),
+                                        None if g is None else _ip(g)) != 0:
+            raise RuntimeError("batch rebind failed")
+        self._upload(xs, keys)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -450,6 +468,18 @@ class Batch:
         if self._l.ssa_amd_batch_output(self._h, i, _dp(out)) != 0:
             raise RuntimeError("batch output failed")
         return out
+
+    def outputs(self):
+        """(list of every series' output, status words) with one device
+        synchronisation per branch."""
+        lens = self.N.astype(np.int64) + self.horizon
+        offs = np.zeros(self.count, dtype=np.int64)
+        offs[1:] = np.cumsum(lens[:-1])
+        cat = np.zeros(int(lens.sum()))
+        st = np.zeros(self.count, dtype=np.int32)
+        if self._l.ssa_amd_batch_outputs(self._h, _dp(cat), offs.ctypes.data_as(c_ll_p), _ip(st)) != 0:
+            raise RuntimeError("batch outputs failed")
+        return [cat[offs[i]:offs[i] + lens[i]] for i in range(self.count)], st
 
     def bytes(self):
         return int(self._l.ssa_amd_batch_bytes(self._h))
