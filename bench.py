@@ -170,16 +170,19 @@ def serving(seconds):
     open-loop Poisson stream at 70 % of it for `seconds`; then the same kind
     of request served alone."""
     import serve
-    serve.Server(max_wave=64).run(serve.synthetic_requests(64, rate=np.inf, seed=60))   # warm-up
-    backlog = serve.synthetic_requests(2048, rate=np.inf, seed=6)
-    done = serve.Server(max_wave=512, inflight=2).run(backlog)
+    srv = serve.Server(max_wave=512, inflight=2)
+    # grow the in-flight slots to the largest wave before anything is timed
+    srv.warm(sorted(serve.synthetic_requests(4096, rate=np.inf, seed=63), key=lambda r: -len(r.x))[:512])
+    done = srv.run(serve.synthetic_requests(2048, rate=np.inf, seed=6))
     capacity = len(done) / max(r.done for r in done)
     rate = 0.7 * capacity
     count = max(64, int(rate * seconds))
-    stream = serve.Server(max_wave=512, inflight=2).run(serve.synthetic_requests(count, rate=rate, seed=61))
+    stream = srv.run(serve.synthetic_requests(count, rate=rate, seed=61))
     alone = []
+    one = serve.Server(max_wave=1, inflight=1)
+    one.warm(sorted(serve.synthetic_requests(512, rate=np.inf, seed=64), key=lambda r: -len(r.x))[:1])
     for r in serve.synthetic_requests(48, rate=np.inf, seed=62):
-        alone.extend(serve.Server(max_wave=1, inflight=1).run([r]))
+        alone.extend(one.run([r]))
     mix, solo = serve.latency_stats(stream), serve.latency_stats(alone)
     return {"workload": "N log-uniform in [500, 50000], L = N // 4, k ~ U{10..50}, group 0..floor(2k/5)-1, "
                         "h = 50, Poisson arrivals, mixed form",
@@ -243,6 +246,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=5.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="plan with the cost model instead of measured costs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this SSA path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -261,6 +265,12 @@ def main():
     global ssa_amd
     import ssa_amd as _ssa_amd
     ssa_amd = _ssa_amd
+    tuned = None
+    if not args.no_tune:
+        # measured transform costs for the planner (both forms)
+        tuned = {"mixed_ns": ssa_amd.autotune(1), "plain_ns": ssa_amd.autotune(0)}
+        print(f"[bench] autotune (ns per transform, 2^6..2^13): {[round(v, 2) for v in tuned['mixed_ns'].values()]}",
+              flush=True)
 
     mixed = ssa_amd.Batch(xs, Ls, ks, p=P_OVERSAMPLE, q=Q_POWER, precision=1, seed=4)
     plans = [mixed.plan(i) for i in range(len(xs))]
@@ -318,6 +328,7 @@ def main():
         "step_achieved_GBs": round(alg / (ms_step * 1e-3) / 1e9, 1),
     }
     out.update(roof)
+    out["autotune"] = tuned
     out["serving"] = serving_out
     out["single_series"] = configs_out
     out["cpu_baseline"] = cpu

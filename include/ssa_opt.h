@@ -176,6 +176,12 @@ int ssa_amd_reconstruct_many(const SSA_Opt *ssa, const int *glist, const int *gc
 /* the device correlation plan of the context: out[0..5] = n, CL, CK, AL, SK, Jacobi sweeps */
 int ssa_amd_plan_info(const SSA_Opt *ssa, int *out6);
 
+/* autotuner: time the block matvec at every FFT length 2^6..2^13 on this
+ * device for the precision (0 plain FP64, 1 mixed) and plan every context
+ * and batch created afterwards with the measured transform costs instead of
+ * the n log n model; ns8 (may be NULL) receives the 8 per-transform costs */
+int ssa_amd_autotune(int precision, double *ns8);
+
 /* batched path: many uneven series in device engines (parallel graph
  * branches, SSA_AMD_BRANCHES, default 2) */
 typedef struct ssa_amd_batch ssa_amd_batch;

@@ -103,19 +103,28 @@ static void prof_end(struct ssa_engine *e)
 
 static int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
-static double plan_cost(int n, int lgn, int M, int AL, int SK, int b, int k, int q)
+/* Measured device time of one transform of length 2^lgn inside a full block
+ * matvec launch (ns, whole-device throughput), per precision, set by
+ * ssa_engine_autotune; all zero: the n (log2 n + 3) model. */
+static double g_tcost[2][NLG];
+static int g_tuned[2];
+static int g_force_lgn = -1;   /* autotune: every series planned at this length */
+
+static double plan_cost(int prec, int n, int lgn, int M, int AL, int SK, int b, int k, int q)
 {
     int nLc = M * AL;
     double fwd = (SK == 1) ? 1.0 + nLc : (double)nLc * (SK + 1);
     double adj = (nLc == 1) ? 1.0 + SK : (double)SK * (nLc + 1);
     double pairs = ceil_div(b > 0 ? b : 2, 2);
     double transforms = (q + 1) * pairs * (fwd + adj) + (double)AL * SK * (k + 1);
-    return transforms * (double)n * (lgn + 3);
+    const double per = g_tuned[prec] ? g_tcost[prec][lgn] : (double)n * (lgn + 3);
+    return transforms * per;
 }
 
 int ssa_plan_series(int prec, int N, int L, int M, int b, int k, int q, int lgn_force, ssa_plan *out)
 {
-    (void)prec;
+    if (prec != SSA_PREC_F64) prec = SSA_PREC_F32;
+    if (g_force_lgn >= 6) lgn_force = g_force_lgn;
     int K = N - L + 1;
     ssa_plan best;
     best.cost = -1;
@@ -132,7 +141,7 @@ int ssa_plan_series(int prec, int N, int L, int M, int b, int k, int q, int lgn_
             int CK = cand[c][1] < K ? cand[c][1] : K;
             if (CL < 1 || CK < 1 || CL + CK - 1 > n) continue;
             int AL = ceil_div(L, CL), SK = ceil_div(K, CK);
-            double cost = plan_cost(n, lgn, M, AL, SK, b, k, q);
+            double cost = plan_cost(prec, n, lgn, M, AL, SK, b, k, q);
             if (best.cost < 0 || cost < best.cost) {
                 best.n = n; best.lgn = lgn; best.CL = CL; best.CK = CK; best.AL = AL; best.SK = SK;
                 best.cost = cost;
@@ -1133,6 +1142,65 @@ int ssa_engine_step(ssa_engine *e, int q, int use_graph)
         e->graph_q = q;
     }
     HIPCHK(hipGraphLaunch(e->gexec, e->st));
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* autotuner: measured transform cost per FFT length                         */
+/* ------------------------------------------------------------------------ */
+
+/* For every length 2^lgn, lgn = 6..13: a throw-away engine of series with
+ * N = n, L = n/4 (one chunk each way, so a column pair costs exactly two
+ * transforms per block matvec), enough of them to fill the device; the
+ * block matvec is timed with HIP events (best of 3) and divided by the
+ * transforms it performs.  The planner then compares candidate chunkings by
+ * measured device time instead of the n (log2 n + 3) model. */
+int ssa_engine_autotune(int prec, double *ns_out)
+{
+    if (prec != SSA_PREC_F64 && prec != SSA_PREC_F32) return -1;
+    double t[NLG] = {0};
+    for (int lg = 6; lg <= 13; lg++) {
+        const int n = 1 << lg, L = n / 4, K = n - L + 1;
+        const int b = L < 60 ? L : 60, pairs = (b + 1) / 2;
+        const int count = (int)((48000 + 2 * pairs - 1) / (2 * pairs));
+        int *N = (int *)malloc(sizeof(int) * count), *Lv = (int *)malloc(sizeof(int) * count);
+        int *bv = (int *)malloc(sizeof(int) * count), *kv = (int *)malloc(sizeof(int) * count);
+        if (!N || !Lv || !bv || !kv) { free(N); free(Lv); free(bv); free(kv); return -1; }
+        for (int i = 0; i < count; i++) { N[i] = n; Lv[i] = L; bv[i] = b; kv[i] = b / 2 > 0 ? b / 2 : 1; }
+        (void)K;
+        int err = 0;
+        g_force_lgn = lg;
+        ssa_engine *e = ssa_engine_create(prec, count, N, Lv, NULL, bv, kv, 2, &err);
+        g_force_lgn = -1;
+        free(N); free(Lv); free(bv); free(kv);
+        if (!e) return -1;
+        int rc = ssa_engine_prepare(e, 1, 0);
+        if (rc == 0) rc = sync_desc(e);
+        /* defined (zero) signals, spectra and panels: timing only */
+        if (rc == 0 && (hipMemsetAsync(e->arena_a, 0, e->esz * (size_t)e->a_elems, e->st) != hipSuccess ||
+                        hipMemsetAsync(e->arena_b, 0, e->esz * (size_t)e->b_elems, e->st) != hipSuccess))
+            rc = -1;
+        hipEvent_t e0 = NULL, e1 = NULL;
+        if (rc == 0 && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = -1;
+        float best = -1.0f;
+        for (int rep = 0; rc == 0 && rep < 4; rep++) {
+            if (hipEventRecord(e0, e->st) != hipSuccess) { rc = -1; break; }
+            rc = ssa_dev_matvec(e->prec, lg, 0, SSA_PANEL_Z, SSA_PANEL_Y, e->dd, it_ptr(e, e->mvf[lg]), e->mvf[lg].n,
+                                e->arena_a, e->tw, e->st);
+            if (rc || hipEventRecord(e1, e->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0 && (best < 0 || ms < best)) best = ms;   /* the first launch warms up */
+        }
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        ssa_engine_destroy(e);
+        if (rc || best <= 0) return -1;
+        t[lg] = 1e6 * (double)best / (2.0 * pairs * count);
+    }
+    for (int lg = 0; lg < NLG; lg++) g_tcost[prec][lg] = t[lg];
+    g_tuned[prec] = 1;
+    if (ns_out) for (int lg = 6; lg <= 13; lg++) ns_out[lg - 6] = t[lg];
     return 0;
 }
 

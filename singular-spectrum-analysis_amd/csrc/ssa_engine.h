@@ -26,6 +26,11 @@ typedef struct { int n, lgn, CL, CK, AL, SK; double cost; } ssa_plan;
  * is in [6, 13] only that length is considered. */
 int ssa_plan_series(int prec, int N, int L, int M, int b, int k, int q, int lgn_force, ssa_plan *out);
 
+/* measure the device cost of one transform at every length 2^6..2^13 for
+ * precision prec and plan every later engine of that precision with it;
+ * ns_out (may be NULL) receives the 8 costs in ns */
+int ssa_engine_autotune(int prec, double *ns_out);
+
 /* create: count series with lengths N[i], windows L[i], channels M[i] (NULL -> 1) */
 ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, const int *M,
                               const int *bmax, const int *kmax, int q_hint, int *err);

@@ -915,6 +915,12 @@ int ssa_amd_batch_set_signals_f32(ssa_amd_batch *b, const float *cat, const long
     return rc;
 }
 
+int ssa_amd_autotune(int precision, double *ns8)
+{
+    if (precision != 0 && precision != 1) return -1;
+    return ssa_engine_autotune(precision == 1 ? SSA_PREC_F32 : SSA_PREC_F64, ns8);
+}
+
 int ssa_amd_batch_set_key(ssa_amd_batch *b, int i, unsigned long long key)
 {
     if (!b || i < 0 || i >= b->count) return -1;

@@ -54,6 +54,14 @@ class Server:
         self.idle = []           # batches ready for the next wave
         self.wave_sizes = []
 
+    def warm(self, requests):
+        """Grow every in-flight slot to hold `requests` as one wave (device
+        arenas, pinned staging), outside any timed run."""
+        for _ in range(self.inflight):
+            self._launch(list(requests))
+        while self.waves:
+            self._retire(block=True)
+
     def _launch(self, reqs):
         xs, Ls, ks = [r.x for r in reqs], [r.L for r in reqs], [r.k for r in reqs]
         gs, keys = [r.g for r in reqs], [request_key(r.rid, self.seed) for r in reqs]

@@ -86,6 +86,7 @@ SIGNATURES = [
     ("ssa_amd_batch_create", _VP, [_I, c_int_p, c_int_p, c_int_p, _I, _I, _I, _U64]),
     ("ssa_amd_batch_set_signals_f32", _I, [_VP, c_float_p, c_ll_p]),
     ("ssa_amd_batch_set_key", _I, [_VP, _I, _U64]),
+    ("ssa_amd_autotune", _I, [_I, c_double_p]),
     ("ssa_amd_batch_rebind", _I, [_VP, _I, c_int_p, c_int_p, c_int_p, c_int_p]),
     ("ssa_amd_batch_outputs", _I, [_VP, c_double_p, c_ll_p, c_int_p]),
     ("ssa_amd_batch_run", _I, [_VP, _I]),
@@ -129,6 +130,16 @@ def _dp(a):
 
 def _ip(a):
     return a.ctypes.data_as(c_int_p)
+
+
+def autotune(precision=1):
+    """Measure the per-transform device cost at every FFT length (ns, for
+    2^6..2^13) and plan every later context and batch of this precision with
+    it (no counterpart in the reference)."""
+    out = np.zeros(8)
+    if lib().ssa_amd_autotune(int(precision), _dp(out)) != 0:
+        raise RuntimeError("autotune failed")
+    return {6 + i: float(v) for i, v in enumerate(out)}
 
 
 class SSA:
