@@ -13,6 +13,7 @@
 //               CholeskyQR2 replacement of dgeqrf/dorgqr (MKL/ssa_opt.h:1723-1725)
 //   k_chol      FP64 Cholesky of the summed Gram, triangular inverse, products
 //   k_apply     panel * small matrix (Q = Y R^-1, U = Q W, V = B W S^-1;
+//               k_apply_f32: the FP32-MFMA form for FP32 panels;
 //               MKL/ssa_opt.h:1736-1738)
 //   k_svd       FP64 one-sided Jacobi SVD of the b x b R factor of B = H^T Q,
 //               sign convention sum(U[:,i]) >= 0 (MKL/ssa_opt.h:1731, :1749-1759)
@@ -877,6 +878,92 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply(const ssa_desc *D, c
 }
 
 // ---------------------------------------------------------------------------
+// FP32 panels, FP32 MFMA (v_mfma_f32_16x16x4f32) with FP32 accumulation: the
+// mixed form's apply.  Same item, wave and row-in-lane structure as k_apply.
+// M (FP64 in the small area) is rounded to FP32 when staged; the panel is
+// FP32 already, so each product is exact in the FP32 MFMA and the only new
+// rounding is the accumulation over b terms, of the same order as the FP32
+// rounding the input panel already carries.  Operand layout: lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; accumulator value v of lane l is
+// D[4*(l/16) + v][l%16].
+// ---------------------------------------------------------------------------
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply_f32(const ssa_desc *D, const ssa_item *items,
+                                                                float *bulk, const double *small, int src,
+                                                                int dst, int mslot, int ncols_k)
+{
+    extern __shared__ float smf[];
+    constexpr int VR = 2, SL = 16 * VR, KK = NB * 4, MP = small_ld<NB>(), B16 = NB * 16;
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const Panel<float> Pi = panel_of<float>(bulk, d, src);
+    const Panel<float> Po = panel_of<float>(bulk, d, dst);
+    const int b = d.b, nc = ncols_k ? d.k : d.b;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, cl = l & 15, kq = l >> 4;
+    float *Ms = smf;
+    {
+        const double *M = small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX;
+        for (int idx = threadIdx.x; idx < B16 * B16; idx += 64 * APPLY_WAVES) {
+            const int c = idx / B16, k = idx - c * B16;
+            Ms[k * MP + c] = (k < b && c < nc) ? (float)M[(long long)c * SSA_BMAX + k] : 0.0f;
+        }
+    }
+    __syncthreads();
+    const int r_begin = it.x * SSA_GRAM_ROWS;
+    const int r_end = min(Pi.rows, r_begin + SSA_GRAM_ROWS);
+    float cur[KK][VR], nxt[KK][VR];
+    auto load = [&](int r0, float (&a)[KK][VR]) {
+#pragma unroll
+        for (int kk = 0; kk < KK; kk++) {
+            const int c = 4 * kk + kq, r = r0 + VR * cl;
+            if (c < b && r < r_end) load_vr(Pi.p + (long long)c * Pi.ld + r, r_end - r, a[kk]);
+            else {
+#pragma unroll
+                for (int v = 0; v < VR; v++) a[kk][v] = 0.0f;
+            }
+        }
+    };
+    int r0 = r_begin + SL * w;
+    load(r0, cur);
+    for (; r0 < r_end; r0 += APPLY_WAVES * SL) {
+        load(r0 + APPLY_WAVES * SL, nxt);
+        const int r = r0 + VR * cl;
+#pragma unroll
+        for (int cb = 0; cb < NB; cb++) {
+            if (cb * 16 >= nc) break;
+            f4v t[VR];
+#pragma unroll
+            for (int rb = 0; rb < VR; rb++) t[rb] = f4v{0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < KK; kk++) {
+                // an upper-triangular M has no rows below the column block's last column
+                if (!ncols_k && kk >= 4 * (cb + 1)) break;
+                const float mv = Ms[(4 * kk + kq) * MP + cb * 16 + cl];
+#pragma unroll
+                for (int rb = 0; rb < VR; rb++)
+                    t[rb] = __builtin_amdgcn_mfma_f32_16x16x4f32(mv, cur[kk][rb], t[rb], 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int c = cb * 16 + 4 * kq + v;
+                if (c < nc && r < r_end) {
+                    float o[VR];
+#pragma unroll
+                    for (int rb = 0; rb < VR; rb++) o[rb] = t[rb][v];
+                    store_vr(Po.p + (long long)c * Po.ld + r, r_end - r, o);
+                }
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < KK; kk++)
+#pragma unroll
+            for (int v = 0; v < VR; v++) cur[kk][v] = nxt[kk][v];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // SVD of R (b x b, FP64) by one-sided Jacobi on R^T with a round-robin
 // parallel order: R^T V = W S, so the converged columns of R^T, normalised,
 // are the right singular vectors W of R (R W = U S) and no rotation has to be
@@ -1373,6 +1460,17 @@ template <typename T, int NB>
 static void launch_apply(int src, int dst, int slot, int ncols_k, const ssa_desc *D, const ssa_item *items,
                          int n_items, void *bulk, const double *small, hipStream_t st)
 {
+    // FP32 panels take the FP32 MFMA unless SSA_AMD_APPLY_F64=1 asks for FP64 accumulation
+    static const int apply_f64 = [] {
+        const char *v = getenv("SSA_AMD_APPLY_F64");
+        return v && atoi(v) != 0;
+    }();
+    if (sizeof(T) == 4 && !apply_f64) {
+        const size_t lds = (size_t)NB * 16 * small_ld<NB>() * sizeof(float);
+        hipLaunchKernelGGL((k_apply_f32<NB>), dim3(n_items), dim3(64 * APPLY_WAVES), lds, st, D, items,
+                           (float *)bulk, small, src, dst, slot, ncols_k);
+        return;
+    }
     const size_t lds = (size_t)NB * 16 * small_ld<NB>() * sizeof(double);
     hipLaunchKernelGGL((k_apply<T, NB>), dim3(n_items), dim3(64 * APPLY_WAVES), lds, st, D, items, (T *)bulk, small,
                        src, dst, slot, ncols_k);
