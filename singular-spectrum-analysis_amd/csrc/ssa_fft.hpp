@@ -97,7 +97,8 @@ template <typename T, int S> __device__ __forceinline__ void dft16(C2<T> *v)
     dft8<T, S>(o);
 #pragma unroll
     for (int m = 0; m < 8; m++) {
-        C2<T> t = (m == 0) ? o[0] : cmul(o[m], w16<T, S>(m));
+        // m = 4 is S*i: a swap and a negation, the same values as the product
+        C2<T> t = (m == 0) ? o[0] : (m == 4) ? muli<T, S>(o[4]) : cmul(o[m], w16<T, S>(m));
         v[m] = cadd(e[m], t);
         v[m + 8] = csub(e[m], t);
     }

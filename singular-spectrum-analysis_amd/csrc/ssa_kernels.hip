@@ -532,14 +532,17 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
 }
 
 // ---------------------------------------------------------------------------
-// First-pass Cholesky in one wave per series, the b x b matrix in registers
-// (b <= NB*16 <= 64): lane j holds column j, slot i = A[i][j].  Row k of the
-// factor is broadcast with uniform readlanes, so no LDS and no barriers.
+// Cholesky in one wave per series, the b x b matrix in registers
+// (b <= NB*16 <= 64): lane j holds column j, slot i = A[i][j].  The matrix is
+// padded to NB*16 with an identity block, so the inner loops run over the
+// full width with static slot indices and no per-element branches; steps
+// k >= b are skipped by one uniform branch each.  Each step writes its row of R into an LDS
+// triangle (column j of R at Rl[tri_off(j) + i], i <= j); the trailing
+// update and the inverse read R from there as uniform (broadcast) LDS reads.
 // Same definition as k_chol pass 1: the Gram partials summed in tile order,
 // pivots below tol * (original diagonal) mark dead columns (zero row of R,
-// zero row and column of R^-1).  R^-1 is formed in place by rows (X R = I):
-// lane i computes row i of X while lane j's slots below its diagonal still
-// hold column j of R until step j has read them.
+// zero row and column of R^-1).  R^-1 is formed by rows (X R = I): lane i
+// computes row i of X.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double rdlane(double v, int lane)
 {
@@ -548,81 +551,103 @@ __device__ __forceinline__ double rdlane(double v, int lane)
     return __hiloint2double(hi, lo);
 }
 
-// Sum the Gram partials of a series in tile order: lane j gets column j
-// (slots i < b), its diagonal dg and its column sum cy.
+__host__ __device__ constexpr int tri_off(int j) { return j * (j + 1) / 2; }
+
+// Sum the Gram partials of a series in tile order: lane j < b gets column j
+// (slots i < b, zero above), its diagonal dg and its column sum cy; lanes
+// j >= b get column j of the identity (dg = 1).  Reads are clamped into the
+// b x b partial and masked, so no slot index depends on b.
 template <int NB>
 __device__ __forceinline__ void gram_sum_regs(const double *part, int tiles, int b, int ln, double (&a)[NB * 16],
                                               double &dg, double &cy)
 {
     constexpr int B16 = NB * 16;
     const int stride = b * b + b;
+    const bool live = ln < b;
+    const int lc = live ? ln : b - 1;
 #pragma unroll
     for (int i = 0; i < B16; i++) a[i] = 0.0;
     dg = 0.0;
     cy = 0.0;
-    if (ln >= b) return;
     // 16 slots per sweep over the tiles (bounded loads in flight)
 #pragma unroll
     for (int h = 0; h < NB; h++) {
+        if (16 * h >= b) break;
         for (int t = 0; t < tiles; t++) {
             const double *pt = part + (long long)t * stride;
 #pragma unroll
-            for (int i = 16 * h; i < 16 * h + 16; i++)
-                if (i < b) a[i] += pt[i * b + ln];
+            for (int i = 16 * h; i < 16 * h + 16; i++) {
+                const double v = pt[(i < b ? i : b - 1) * b + lc];
+                a[i] += (i < b) ? v : 0.0;
+            }
             if (h == 0) {
-                dg += pt[ln * b + ln];
-                cy += pt[b * b + ln];
+                dg += pt[lc * b + lc];
+                cy += pt[b * b + lc];
             }
         }
     }
+#pragma unroll
+    for (int i = 0; i < B16; i++)
+        if (!live) a[i] = (i == ln) ? 1.0 : 0.0;
+    if (!live) {
+        dg = 1.0;
+        cy = 0.0;
+    }
 }
 
-// Cholesky A = R^T R in place (lane j: column j of R in slots 0..j).  Loops
-// have constant trip counts with uniform guards so that they unroll fully
-// and every slot index is static (a[] stays in registers).
+// Cholesky A = R^T R in place (lane j: column j of R in slots 0..j), row k of
+// R also stored to Rl.  Constant trip counts, static slot indices.
 template <int NB>
-__device__ __forceinline__ void chol_regs(double (&a)[NB * 16], double dg, int b, int ln, double tol, int &ndead,
-                                          bool &mydead)
+__device__ __forceinline__ void chol_regs(double (&a)[NB * 16], double dg, int b, int ln, double tol, double *Rl,
+                                          int &ndead, bool &mydead)
 {
     constexpr int B16 = NB * 16;
     ndead = 0;
     mydead = true;
 #pragma unroll
     for (int k = 0; k < B16; k++) {
-        if (k < b) {
-            const double p = rdlane(a[k], k), d0 = rdlane(dg, k);
-            const bool dead = !(p > tol * d0) || !(d0 > 0.0);
-            const double piv = dead ? 0.0 : sqrt(p);
-            ndead += dead;
-            if (ln == k) mydead = dead;
-            if (ln >= k) a[k] = dead ? 0.0 : (ln == k ? piv : a[k] / piv);
+        if (k >= b) continue;   // one uniform branch per step (it also bounds the scheduling region)
+        const double p = rdlane(a[k], k), d0 = rdlane(dg, k);
+        const bool dead = !(p > tol * d0) || !(d0 > 0.0);
+        const double piv = dead ? 0.0 : sqrt(p);
+        ndead += dead;
+        if (ln == k) mydead = dead;
+        if (ln >= k) a[k] = dead ? 0.0 : (ln == k ? piv : a[k] / piv);
+        if (ln >= k && ln < B16) Rl[tri_off(ln) + k] = a[k];   // lanes >= B16 hold no column
+        __syncthreads();
 #pragma unroll
-            for (int i = k + 1; i < B16; i++)
-                if (i < b) a[i] = fma(-rdlane(a[k], i), a[k], a[i]);
+        for (int i = k + 1; i < B16; i++) {
+            a[i] = fma(-Rl[tri_off(i) + k], a[k], a[i]);
+            // at most 16 broadcast values in flight (register pressure)
+            if ((i & 15) == 15) asm volatile("" ::: "memory");
         }
     }
 }
 
-// X = R^-1 in place, by rows (X R = I): afterwards lane i holds row i of X
-// (slots j >= i, zero below).  Lane j's slots below its diagonal keep column
-// j of R until step j has read them.
+// X = R^-1 by rows (X R = I) from the R in Rl: afterwards lane i holds row i
+// of X (slots j >= i, zero below).
 template <int NB>
-__device__ __forceinline__ void inv_rows(double (&a)[NB * 16], int b, int ln, bool mydead)
+__device__ __forceinline__ void inv_rows(double (&a)[NB * 16], int b, int ln, bool mydead, const double *Rl)
 {
     constexpr int B16 = NB * 16;
 #pragma unroll
     for (int j = 0; j < B16; j++) {
         if (j >= b) continue;
-        const double rjj = rdlane(a[j], j);
+        const double rjj = Rl[tri_off(j) + j];
         double s = 0.0;
 #pragma unroll
-        for (int l = 0; l < j; l++) s = fma(-a[l], rdlane(a[l], j), s);
+        for (int l = 0; l < j; l++) {
+            s = fma(-a[l], Rl[tri_off(j) + l], s);
+            if ((l & 15) == 15) asm volatile("" ::: "memory");
+        }
         const bool deadj = !(rjj > 0.0);
         if (ln <= j) a[j] = (deadj || mydead) ? 0.0 : (ln == j ? 1.0 / rjj : s / rjj);
         if (ln == j) {
 #pragma unroll
             for (int l = 0; l < j; l++) a[l] = 0.0;
         }
+        // R is read-only here: keep the scheduler from hoisting later steps' loads
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -637,12 +662,14 @@ template <int NB> __device__ __forceinline__ void stage_rows(const double *M, in
 
 // One wave per series: R = chol(G) and R^-1 of the summed Gram into the
 // pass's slots (pass 1: R1, R1INV, colsum(Y) for Y, the dead-column count;
-// pass 2: R2, R2INV).  Dead pivots as in k_chol.
+// pass 2: R2, R2INV).  Dead pivots as in k_chol.  Rows and columns b..NB*16-1
+// of both slots are written as zeros.
 template <int NB>
 __global__ __launch_bounds__(64) void k_chol_w(const ssa_desc *D, const ssa_item *items, double *small, int which,
                                                int pass, double tol)
 {
     constexpr int B16 = NB * 16;
+    __shared__ double Rl[tri_off(B16)];
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const int b = d.b, ln = threadIdx.x;
@@ -654,20 +681,21 @@ __global__ __launch_bounds__(64) void k_chol_w(const ssa_desc *D, const ssa_item
     if (pass == 1 && which == SSA_PANEL_Y && ln < b) misc[SSA_MISC_COLSUMY + ln] = cy;
     int ndead;
     bool mydead;
-    chol_regs<NB>(a, dg, b, ln, tol, ndead, mydead);
+    chol_regs<NB>(a, dg, b, ln, tol, Rl, ndead, mydead);
+    __syncthreads();
     double *Ro = slot(small, d, pass == 1 ? SSA_SLOT_R1 : SSA_SLOT_R2);
     double *Rio = slot(small, d, pass == 1 ? SSA_SLOT_R1INV : SSA_SLOT_R2INV);
-    if (ln < b) {
-#pragma unroll
-        for (int i = 0; i < B16; i++)
-            if (i < b) Ro[(long long)ln * SSA_BMAX + i] = (i <= ln) ? a[i] : 0.0;
-    }
-    inv_rows<NB>(a, b, ln, mydead);
-    if (ln < b) {
-#pragma unroll
+    // column j of R from the triangle: lanes over rows, coalesced
+    if (ln < B16) {
         for (int j = 0; j < B16; j++)
-            if (j < b) Rio[(long long)j * SSA_BMAX + ln] = a[j];
+            Ro[(long long)j * SSA_BMAX + ln] = (ln <= j && j < b) ? Rl[tri_off(j) + ln] : 0.0;
     }
+    inv_rows<NB>(a, b, ln, mydead, Rl);
+    if (ln < B16) {
+#pragma unroll
+        for (int j = 0; j < B16; j++) Rio[(long long)j * SSA_BMAX + ln] = (ln < b && j < b) ? a[j] : 0.0;
+    }
+    // dead padding columns cannot occur (identity pivots), so ndead counts real columns only
     if (pass == 1 && ln == 0) misc[SSA_MISC_DEAD] = ndead;
 }
 
