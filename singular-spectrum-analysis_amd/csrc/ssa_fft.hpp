@@ -55,6 +55,26 @@ template <typename T, int S> __device__ __forceinline__ C2<T> w16(int m)
     return {c, (T)S * s};
 }
 
+// exp(S * 2 pi i * m / 32) for m = 0..15 (even m from w16)
+template <typename T, int S> __device__ __forceinline__ C2<T> w32(int m)
+{
+    if (m % 2 == 0) return w16<T, S>(m / 2);
+    const T a1 = (T)0.98078528040323044913, b1 = (T)0.19509032201612826785;   // cos, sin of pi/16
+    const T a3 = (T)0.83146961230254523708, b3 = (T)0.55557023301960222474;   // cos, sin of 3 pi/16
+    T c = 0, s = 0;
+    switch (m) {
+    case 1: c = a1; s = b1; break;
+    case 3: c = a3; s = b3; break;
+    case 5: c = b3; s = a3; break;
+    case 7: c = b1; s = a1; break;
+    case 9: c = -b1; s = a1; break;
+    case 11: c = -b3; s = a3; break;
+    case 13: c = -a3; s = b3; break;
+    default: c = -a1; s = b1; break;
+    }
+    return {c, (T)S * s};
+}
+
 // in-register DFTs, natural order in and out
 template <typename T, int S> __device__ __forceinline__ void dft2(C2<T> *v)
 {
@@ -104,12 +124,28 @@ template <typename T, int S> __device__ __forceinline__ void dft16(C2<T> *v)
     }
 }
 
+template <typename T, int S> __device__ __forceinline__ void dft32(C2<T> *v)
+{
+    C2<T> e[16], o[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) { e[i] = v[2 * i]; o[i] = v[2 * i + 1]; }
+    dft16<T, S>(e);
+    dft16<T, S>(o);
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+        C2<T> t = (m == 0) ? o[0] : (m == 8) ? muli<T, S>(o[8]) : cmul(o[m], w32<T, S>(m));
+        v[m] = cadd(e[m], t);
+        v[m + 16] = csub(e[m], t);
+    }
+}
+
 template <typename T, int S, int R> __device__ __forceinline__ void dftR(C2<T> *v)
 {
     if constexpr (R == 2) dft2<T, S>(v);
     else if constexpr (R == 4) dft4<T, S>(v[0], v[1], v[2], v[3]);
     else if constexpr (R == 8) dft8<T, S>(v);
     else if constexpr (R == 16) dft16<T, S>(v);
+    else if constexpr (R == 32) dft32<T, S>(v);
 }
 
 // LDS index of element p in the exchange that follows a pass of stride
@@ -146,24 +182,29 @@ __host__ __device__ constexpr long long tw_offset(int nslog, int rlog)
 }
 constexpr long long TW_ENTRIES = tw_offset(0, 5);
 
-template <typename T, int LOGN> struct FftGeom {
+template <typename T, int LOGN, bool WIDE = false> struct FftGeom {
     static constexpr int N = 1 << LOGN;
-    static constexpr int E = (sizeof(T) == 4 && LOGN >= 10) ? 16 : N >= 16 ? 8 : N / 2;
+    // WIDE, FP32 at N = 8192: 32 values per thread, a radix-32 pass then two
+    // radix-16 passes (two LDS exchanges instead of three), at half the waves
+    static constexpr int E = (WIDE && sizeof(T) == 4 && LOGN == 13) ? 32 : (sizeof(T) == 4 && LOGN >= 10) ? 16 : N >= 16 ? 8 : N / 2;
     static constexpr int NT = N / E;
-    static constexpr int LOGE = (E == 16) ? 4 : (E == 8) ? 3 : (E == 4) ? 2 : 1;
-    static constexpr int FIRST = (LOGN % LOGE) ? (LOGN % LOGE) : LOGE;   // radix log of pass 0
+    static constexpr int LOGE = (E == 32) ? 5 : (E == 16) ? 4 : (E == 8) ? 3 : (E == 4) ? 2 : 1;
+    static constexpr int NEXT = (E == 32) ? 4 : LOGE;                     // radix log of passes 1..
+    static constexpr int FIRST = (E == 32) ? LOGN - 8 : (LOGN % LOGE) ? (LOGN % LOGE) : LOGE;   // radix log of pass 0
+    // waves per SIMD the transform kernels are built for (register budget)
+    static constexpr int OCC = (E == 32) ? 2 : (E == 8 && sizeof(T) == 4) ? 8 : 4;
     static constexpr int LDS = N;                                        // elements per plane
     // exchange one plane at a time when both planes would keep a second
     // workgroup off the CU (160 KB LDS)
     static constexpr bool SPLIT = 2 * LDS * sizeof(T) > 80000;
 };
 
-template <typename T, int LOGN> using FftRegs = C2<T>[FftGeom<T, LOGN>::E];
+template <typename T, int LOGN, bool WIDE = false> using FftRegs = C2<T>[FftGeom<T, LOGN, WIDE>::E];
 
-template <typename T, int LOGN, int SIGN, bool SPLIT, int NSLOG, int RLOG>
-__device__ __forceinline__ void fft_pass(FftRegs<T, LOGN> &c, T *lre, T *lim, const C2<T> *tw, int tid)
+template <typename T, int LOGN, int SIGN, bool SPLIT, int NSLOG, int RLOG, bool WIDE = false>
+__device__ __forceinline__ void fft_pass(FftRegs<T, LOGN, WIDE> &c, T *lre, T *lim, const C2<T> *tw, int tid)
 {
-    using G = FftGeom<T, LOGN>;
+    using G = FftGeom<T, LOGN, WIDE>;
     constexpr int N = G::N, NT = G::NT, E = G::E;
     constexpr int R = 1 << RLOG, NS = 1 << NSLOG, BPT = E / R;
 #pragma unroll
@@ -186,7 +227,7 @@ __device__ __forceinline__ void fft_pass(FftRegs<T, LOGN> &c, T *lre, T *lim, co
             // w^r for r < R from the base-table entries w^1, w^2, w^4, w^8
             const int kk = j & (NS - 1);
             const int step = kk << (TW_BASE_LOG - NSLOG - RLOG);
-            C2<T> wp[4];
+            C2<T> wp[5];
 #pragma unroll
             for (int b = 0; (1 << b) < R; b++) {
                 wp[b] = tw[(1 << b) * step];
@@ -244,14 +285,14 @@ __device__ __forceinline__ void fft_pass(FftRegs<T, LOGN> &c, T *lre, T *lim, co
             }
             __syncthreads();
         }
-        constexpr int NEXT = G::LOGE;
-        fft_pass<T, LOGN, SIGN, SPLIT, NSLOG + RLOG, NEXT>(c, lre, lim, tw, tid);
+        constexpr int NEXT = G::NEXT;
+        fft_pass<T, LOGN, SIGN, SPLIT, NSLOG + RLOG, NEXT, WIDE>(c, lre, lim, tw, tid);
     }
 }
 
 // Full transform of the natural-layout registers c (in place).
-template <typename T, int LOGN, int SIGN, bool SPLIT>
-__device__ __forceinline__ void fft(FftRegs<T, LOGN> &c, T *lre, T *lim, const C2<T> *tw)
+template <typename T, int LOGN, int SIGN, bool SPLIT, bool WIDE = false>
+__device__ __forceinline__ void fft(FftRegs<T, LOGN, WIDE> &c, T *lre, T *lim, const C2<T> *tw)
 {
     // The twiddles and LDS addresses depend only on the thread index, so the
     // compiler would hoist those of every pass out of the callers' loops and
@@ -261,7 +302,7 @@ __device__ __forceinline__ void fft(FftRegs<T, LOGN> &c, T *lre, T *lim, const C
     const C2<T> *t = tw;
     asm volatile("" : "+v"(tid));
     asm volatile("" : "+s"(t));
-    fft_pass<T, LOGN, SIGN, SPLIT, 0, FftGeom<T, LOGN>::FIRST>(c, lre, lim, t, tid);
+    fft_pass<T, LOGN, SIGN, SPLIT, 0, FftGeom<T, LOGN, WIDE>::FIRST, WIDE>(c, lre, lim, t, tid);
 }
 
 }  // namespace ssa

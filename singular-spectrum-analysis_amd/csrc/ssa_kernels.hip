@@ -207,12 +207,12 @@ __global__ __launch_bounds__(256) void k_omega(const ssa_desc *__restrict__ D, c
 // out[o][i] = sum_p sum_j seg_(o,p)[i + j] in_p[j]  via  T+( X_(o,p) . T+(in_p) ).
 // ---------------------------------------------------------------------------
 template <typename T, int LOGN>
-__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::E == 8 && sizeof(T) == 4 ? 8 : 4)) void k_matvec(const ssa_desc *__restrict__ D,
+__global__ __launch_bounds__((FftGeom<T, LOGN, true>::NT), (FftGeom<T, LOGN, true>::OCC)) void k_matvec(const ssa_desc *__restrict__ D,
                                                                      const ssa_item *__restrict__ items, T *bulk,
                                                                      const C2<T> *__restrict__ tw, int dir,
                                                                      int in_panel, int out_panel, int n_items)
 {
-    using G = FftGeom<T, LOGN>;
+    using G = FftGeom<T, LOGN, true>;
     constexpr int E = G::E, NT = G::NT;
     constexpr bool SPLIT = G::SPLIT;
     __shared__ T lre[G::LDS];
@@ -254,7 +254,7 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::E == 8 &
                         z[e].y = (two && q < len) ? a1[q] : (T)0;
                     }
                 }
-                fft<T, LOGN, 1, SPLIT>(z, lre, lim, tw);
+                fft<T, LOGN, 1, SPLIT, true>(z, lre, lim, tw);
             }
             // spectra are indexed [L chunk][K chunk]
             const int lo = dir == 0 ? o : p, ko = dir == 0 ? p : o;
@@ -266,7 +266,7 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::E == 8 &
                 acc[e].y += xv.x * z[e].y + xv.y * z[e].x;
             }
         }
-        fft<T, LOGN, 1, SPLIT>(acc, lre, lim, tw);
+        fft<T, LOGN, 1, SPLIT, true>(acc, lre, lim, tw);
         int row0, len;
         if (dir == 0) {
             const int m = o / d.AL, a = o - m * d.AL;
@@ -1152,7 +1152,7 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
 // (the imaginary part of the self-convolution of a + ib is 2 a (*) b).
 // ---------------------------------------------------------------------------
 template <typename T, int LOGN>
-__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::E == 8 && sizeof(T) == 4 ? 8 : 4)) void k_recon(const ssa_desc *__restrict__ D,
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) void k_recon(const ssa_desc *__restrict__ D,
                                                               const ssa_item *__restrict__ items, T *bulk,
                                                               const double *__restrict__ small,
                                                               const int4 *__restrict__ gdesc,
@@ -1398,7 +1398,7 @@ int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel, cons
     if (!lgn_ok(prec, lgn)) return -2;
     hipStream_t st = (hipStream_t)stream;
 #define LAUNCH(T, L)                                                                                          \
-    hipLaunchKernelGGL((k_matvec<T, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L>::NT), 0, st, D, items,   \
+    hipLaunchKernelGGL((k_matvec<T, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L, true>::NT), 0, st, D, items,   \
                        (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel, n_items)
 #define CASE(L)                                                                                               \
     case L:                                                                                                   \
