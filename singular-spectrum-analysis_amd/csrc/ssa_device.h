@@ -102,8 +102,8 @@ int ssa_dev_trmm(int nb, int slot_a, int slot_b, int slot_c, const ssa_desc *D, 
                  double *small, void *stream);
 /* SVD of the R factor of B in slot r_slot; colsum(Q) = colsum(Y) * (slot rinv_slot), the R^-1 of the final Q.
  * Every series of the launch has b <= bmax; instantiated per column-block count SSA_NB(bmax). */
-int ssa_dev_svd(int bmax, int r_slot, int rinv_slot, const ssa_desc *D, const ssa_item *items, int n_items,
-                double *small, void *stream);
+int ssa_dev_svd(int bmax, int r_slot, int rinv_slot, double early, const ssa_desc *D, const ssa_item *items,
+                int n_items, double *small, void *stream);
 /* reconstruction: items (s, gid, row chunk o, col chunk sk); groups: gdesc[gid] = {start, count},
  * glist = component indices; scale_sigma 1: sum_g sigma_g u_g (*) v_g, 0: u (*) v */
 int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, const ssa_desc *D, const ssa_item *items,

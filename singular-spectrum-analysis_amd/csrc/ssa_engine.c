@@ -833,11 +833,14 @@ static int launch_decompose(ssa_engine *e, int q, int injected)
         r_slot = SSA_SLOT_R2;
         rinv_slot = SSA_SLOT_R1INV;
     }
+    /* Jacobi stops after a sweep whose largest column cosine was below this
+     * (the residual is then of its square): 1e-16 plain, 1e-12 mixed */
+    const double early = e->prec == SSA_PREC_F64 ? 1e-8 : 1e-6;
     for (int nb = 1; nb < NBW; nb++) {
         if (!e->ser_nb[nb].n) continue;
         prof_begin(e, PH_SVD, nb);
-        DEVCHK(ssa_dev_svd(16 * nb < SSA_BMAX ? 16 * nb : SSA_BMAX, r_slot, rinv_slot, e->dd, it_ptr(e, e->ser_nb[nb]),
-                           e->ser_nb[nb].n, e->small, e->st));
+        DEVCHK(ssa_dev_svd(16 * nb < SSA_BMAX ? 16 * nb : SSA_BMAX, r_slot, rinv_slot, early, e->dd,
+                           it_ptr(e, e->ser_nb[nb]), e->ser_nb[nb].n, e->small, e->st));
         prof_end(e);
     }
     DEVCHK(apply_all(e, PH_UV, SSA_PANEL_Y, SSA_PANEL_U, SSA_SLOT_WK, 1));

@@ -1001,7 +1001,7 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply_f32(const ssa_desc *
 // ---------------------------------------------------------------------------
 template <int NB>
 __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *items, double *small, int r_slot,
-                                             int rinv_slot)
+                                             int rinv_slot, double early)
 {
     extern __shared__ double sm[];
     const int bq = D[items[blockIdx.x].s].b;
@@ -1009,7 +1009,7 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     double *A = sm;                        // column j at A + j*LDP (rows 0..b-1), bp columns
     __shared__ int perm[SSA_BMAX + 1];
     __shared__ double S[SSA_BMAX + 1];
-    __shared__ int rotated;
+    __shared__ int rotated, big;
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const int b = d.b, bp = (b + 1) & ~1;
@@ -1025,11 +1025,15 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     // 32 groups of 8 lanes; group g rotates pair g of the round (bp/2 <= 40 pairs:
     // groups take pairs g and g + 32), lanes stride the b rows
     const int grp = threadIdx.x >> 3, gl = threadIdx.x & 7;
+    // A sweep rotates every pair whose cosine exceeds 1e-15.  Convergence is
+    // quadratic: after a sweep in which no pair's cosine exceeded `early`, the
+    // remaining cosines are of order early^2, so the sweep that would only
+    // confirm convergence is skipped.
     int sweeps = 0;
-    for (; sweeps < 40; sweeps++) {
-        if (threadIdx.x == 0) rotated = 0;
+    while (sweeps < 40) {
+        if (threadIdx.x == 0) rotated = big = 0;
         __syncthreads();
-        int any_local = 0;
+        int any_local = 0, big_local = 0;
         for (int round = 0; round < bp - 1; round++) {
             for (int pi = grp; pi < bp / 2; pi += 32) {
                 // circle-method pairing without integer division
@@ -1064,6 +1068,7 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                     be += __shfl_xor(be, o);
                     ga += __shfl_xor(ga, o);
                 }
+                if (ga != 0.0 && fabs(ga) > early * sqrt(al * be)) big_local = 1;
                 if (ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
                     const double zeta = (be - al) / (2.0 * ga);
                     const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
@@ -1082,10 +1087,12 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
             __syncthreads();
         }
         if (any_local) rotated = 1;
+        if (big_local) big = 1;
         __syncthreads();
-        const int any = rotated;
+        const int any = rotated, any_big = big;
         __syncthreads();
-        if (!any) break;
+        sweeps++;
+        if (!any || !any_big) break;
     }
     // singular values = column norms
     for (int j = wave; j < b; j += nwave) {
@@ -1539,18 +1546,18 @@ int ssa_dev_trmm(int nb, int slot_a, int slot_b, int slot_c, const ssa_desc *D, 
     return (int)hipGetLastError();
 }
 
-int ssa_dev_svd(int bmax, int r_slot, int rinv_slot, const ssa_desc *D, const ssa_item *items, int n_items,
-                double *small, void *stream)
+int ssa_dev_svd(int bmax, int r_slot, int rinv_slot, double early, const ssa_desc *D, const ssa_item *items,
+                int n_items, double *small, void *stream)
 {
     if (n_items <= 0) return 0;
     const size_t lds = (size_t)((bmax + 1) & ~1) * (bmax + 1) * sizeof(double);
     hipStream_t st = (hipStream_t)stream;
     switch (SSA_NB(bmax)) {
-    case 1: hipLaunchKernelGGL(k_svd<1>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
-    case 2: hipLaunchKernelGGL(k_svd<2>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
-    case 3: hipLaunchKernelGGL(k_svd<3>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
-    case 4: hipLaunchKernelGGL(k_svd<4>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
-    case 5: hipLaunchKernelGGL(k_svd<5>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot); break;
+    case 1: hipLaunchKernelGGL(k_svd<1>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot, early); break;
+    case 2: hipLaunchKernelGGL(k_svd<2>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot, early); break;
+    case 3: hipLaunchKernelGGL(k_svd<3>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot, early); break;
+    case 4: hipLaunchKernelGGL(k_svd<4>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot, early); break;
+    case 5: hipLaunchKernelGGL(k_svd<5>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot, early); break;
     default: return -1;
     }
     return (int)hipGetLastError();
