@@ -821,14 +821,31 @@ static int batch_bind(ssa_amd_batch *b, int count, const int *N, const int *L, c
     for (int j = 0; j < count; j++) b->gid[ord[j]] = j % b->G;
     rc = 0;
     for (int br = 0; rc == 0 && br < b->G; br++) {
+        /* Engine order of the branch's series: in decreasing cost, dealt in a
+         * snake over N_XCD_BINS bins which are then concatenated.  Launches
+         * give each XCD a contiguous range of work items, so every range holds
+         * a similar mix of sizes, largest first (the launch tails are short
+         * items). */
+        enum { N_XCD_BINS = 8 };
+        int nbr = 0;
+        for (int j = 0; j < count; j++) if (b->gid[ord[j]] == br) nbr++;
         int n = 0;
-        for (int i = 0; i < count; i++) {
-            if (b->gid[i] != br) continue;
-            b->lid[i] = n;
-            sN[n] = N[i]; sL[n] = L[i]; sb[n] = bm[i]; sk[n] = km[i];
-            sg[n] = g ? imin(imax(g[i], 1), km[i]) : 0;
-            n++;
+        for (int bin = 0; bin < N_XCD_BINS; bin++) {
+            int pos = 0;   /* position of the series in the branch's cost order */
+            for (int j = 0; j < count; j++) {
+                const int i = ord[j];
+                if (b->gid[i] != br) continue;
+                const int rnd = pos / N_XCD_BINS, slot = pos % N_XCD_BINS;
+                const int bi = (rnd & 1) ? N_XCD_BINS - 1 - slot : slot;
+                pos++;
+                if (bi != bin) continue;
+                b->lid[i] = n;
+                sN[n] = N[i]; sL[n] = L[i]; sb[n] = bm[i]; sk[n] = km[i];
+                sg[n] = g ? imin(imax(g[i], 1), km[i]) : 0;
+                n++;
+            }
         }
+        if (n != nbr) { rc = -1; break; }
         if (!b->eng[br]) {
             int err = 0;
             b->eng[br] = ssa_engine_create(b->precision, n, sN, sL, NULL, sb, sk, b->q, &err);
