@@ -151,6 +151,15 @@ int ssa_opt_forecast_with_lrf(const SSA_LRF *lrf, const ssa_real *base_signal, i
 int ssa_opt_forecast(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output);
 int ssa_opt_forecast_full(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output);
 
+/* ---- V-forecast (MKL/ssa_opt.h:2887-3084, declared at :352-355) ----
+ * One projection per eigenvector and step, x = sum_g pi_g (U_g[0:L-1] . z) / (1 - nu^2),
+ * on the device.  -1 for an index outside the computed components, nu^2 >= 1 - 1e-10,
+ * n_forecast outside 1..65536, or (fast) base_len < L - 1. */
+int ssa_opt_vforecast(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output);
+int ssa_opt_vforecast_full(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output);
+int ssa_opt_vforecast_fast(const SSA_Opt *ssa, const int *group, int n_group, const ssa_real *base_signal,
+                           int base_len, int n_forecast, ssa_real *output);
+
 /* ---- MSSA (MKL/ssa_opt.h:3881-4283) ---- */
 int mssa_opt_init(MSSA_Opt *mssa, const ssa_real *X, int M, int N, int L);
 int mssa_opt_decompose(MSSA_Opt *mssa, int k, int oversampling);

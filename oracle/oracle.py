@@ -39,6 +39,7 @@ def load():
     lib.orc_lrf.argtypes = [vp, _ip, ctypes.c_int, _dp, _dp]
     lib.orc_forecast.argtypes = [vp, _ip, ctypes.c_int, ctypes.c_int, _dp]
     lib.orc_forecast_full.argtypes = [vp, _ip, ctypes.c_int, ctypes.c_int, _dp]
+    lib.orc_vforecast.argtypes = [vp, _ip, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp]
     lib.orc_variance_explained.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.orc_variance_explained.restype = ctypes.c_double
     lib.orc_dims.argtypes = [vp, _ip]
@@ -184,6 +185,20 @@ class OracleSSA:
         out = np.zeros(self.N + h)
         if self._l.orc_forecast_full(self._p, _i(g), len(g), h, _d(out)) != 0:
             raise RuntimeError("oracle forecast_full failed")
+        return out
+
+    def vforecast(self, group, h, base=None):
+        """V-forecast from the group's reconstruction, or from the last L-1
+        values of `base` (MKL/ssa_opt.h:2906-2965, :3016-3078)."""
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        out = np.zeros(h)
+        if base is None:
+            rc = self._l.orc_vforecast(self._p, _i(g), len(g), None, 0, h, _d(out))
+        else:
+            b = np.ascontiguousarray(base, dtype=np.float64)
+            rc = self._l.orc_vforecast(self._p, _i(g), len(g), _d(b), len(b), h, _d(out))
+        if rc != 0:
+            raise RuntimeError("oracle vforecast failed")
         return out
 
     def variance_explained(self, start, end):

@@ -564,6 +564,51 @@ int orc_forecast_full(const orc_ssa *s, const int *group, int n_group, int h, do
     return rc;
 }
 
+/* V-forecast, MKL/ssa_opt.h:2906-2965 (plain), :3016-3078 (fast: base given):
+ * each step projects the last L-1 values on every eigenvector of the group,
+ * x = (sum_g pi_g (U_g[0:L-1] . z)) / (1 - nu^2), in the reference's order
+ * (one sequential dot per component, then the weighted sum). */
+int orc_vforecast(const orc_ssa *s, const int *group, int n_group, const double *base, int base_len, int h,
+                  double *out)
+{
+    if (!s || !s->decomposed || !group || !out || n_group < 1 || h < 1) return -1;
+    int L = s->L, w = L - 1;
+    double nu = 0.0;
+    for (int g = 0; g < n_group; g++) {
+        int idx = group[g];
+        if (idx < 0 || idx >= s->k) return -1;
+        double pi = s->U[(size_t)idx * L + L - 1];
+        nu += pi * pi;
+    }
+    if (nu >= 1.0 - 1e-10) return -1;
+    double sc = 1.0 / (1.0 - nu);
+    double *rec = NULL;
+    if (!base) {
+        rec = (double *)xcalloc(s->N, sizeof(double));
+        if (orc_reconstruct(s, group, n_group, rec) != 0) { free(rec); return -1; }
+        base = rec;
+        base_len = s->N;
+    }
+    if (base_len < w) { free(rec); return -1; }
+    double *buf = (double *)xcalloc((size_t)w + h, sizeof(double));
+    for (int i = 0; i < w; i++) buf[i] = base[base_len - w + i];
+    for (int t = 0; t < h; t++) {
+        double x = 0.0;
+        for (int g = 0; g < n_group; g++) {
+            const double *u = s->U + (size_t)group[g] * L;
+            double dot = 0.0;
+            for (int j = 0; j < w; j++) dot += u[j] * buf[t + j];
+            x += u[L - 1] * dot;
+        }
+        x *= sc;
+        buf[w + t] = x;
+        out[t] = x;
+    }
+    free(buf);
+    free(rec);
+    return 0;
+}
+
 double orc_variance_explained(const orc_ssa *s, int start, int end)   /* :3118-3128 */
 {
     if (!s || !s->decomposed || start < 0 || s->total_variance <= 0) return 0.0;

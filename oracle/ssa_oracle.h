@@ -74,6 +74,8 @@ int  orc_wcorr(const orc_ssa *s, double *W);
 int  orc_lrf(const orc_ssa *s, const int *group, int n_group, double *R, double *nu2);
 int  orc_forecast(const orc_ssa *s, const int *group, int n_group, int h, double *out);
 int  orc_forecast_full(const orc_ssa *s, const int *group, int n_group, int h, double *out);
+int  orc_vforecast(const orc_ssa *s, const int *group, int n_group, const double *base, int base_len, int h,
+                   double *out);
 double orc_variance_explained(const orc_ssa *s, int start, int end);
 
 /* ---- accessors for ctypes ---- */

@@ -117,10 +117,10 @@ int ssa_dev_olap(int prec, int weight, const ssa_desc *D, const ssa_item *items,
  * then normalisation; W written row-major k x k to small + off_small + SLOT_W */
 int ssa_dev_wcorr(int prec, int nb, const ssa_desc *D, const ssa_item *tile_items, int n_tiles,
                   const ssa_item *series_items, int n_series, void *bulk, double *small, void *stream);
-/* LRF + forecast: per series, group gdesc[0]; reads the reconstruction at off_out, writes
- * h forecasts after it (off_out + N) */
-int ssa_dev_forecast(int prec, int h, int limit_gsize, const ssa_desc *D, const ssa_item *items, int n_items,
-                     const int *gdesc, const int *glist, void *bulk, double *small, void *stream);
+/* LRF + forecast (vmode 0) or V-forecast (vmode 1): per series, group gdesc[0]; reads the
+ * last L-1 values before off_out + N, writes h forecasts after them (off_out + N) */
+int ssa_dev_forecast(int prec, int h, int limit_gsize, int vmode, const ssa_desc *D, const ssa_item *items,
+                     int n_items, const int *gdesc, const int *glist, void *bulk, double *small, void *stream);
 
 #ifdef __cplusplus
 }

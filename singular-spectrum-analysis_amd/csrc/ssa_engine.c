@@ -1004,21 +1004,43 @@ int ssa_engine_wcorr(ssa_engine *e, int s, double *W)
     return 0;
 }
 
-int ssa_engine_forecast(ssa_engine *e, int s, const int *group, int n_group, int h, double *out, int full)
+int ssa_engine_forecast(ssa_engine *e, int s, const int *group, int n_group, int h, double *out, int full,
+                        int vmode, const double *base, int base_len)
 {
     if (!e->prepared || s < 0 || s >= e->count || n_group < 1 || h < 1 || h > e->max_h) return -1;
     const ssa_desc *d = &e->hd[s];
     if (n_group > SSA_BMAX) return -1;
     for (int g = 0; g < n_group; g++) if (group[g] < 0 || group[g] >= d->k) return -1;  /* MKL/ssa_opt.h:2782-2787 */
+    const int w = d->L - 1;
+    if (base && (base_len < w || full)) return -1;
     int gc = n_group;
-    DEVCHK(ssa_engine_reconstruct(e, s, 0, group, &gc, 1, 1, 0, NULL));
+    if (!base) {
+        DEVCHK(ssa_engine_reconstruct(e, s, 0, group, &gc, 1, 1, 0, NULL));
+    } else {
+        /* the caller's base signal stands in for the reconstruction: its last
+         * L-1 values go where the recurrence reads them (off_out + N - (L-1)) */
+        int gd[4] = {0, n_group, 0, 0};
+        DEVCHK(sync_desc(e));
+        DEVCHK(groups_upload(e, gd, 1, group, n_group));
+        DEVCHK(pin_reserve(e, (size_t)(w > 0 ? w : 1) * e->esz));
+        for (int j = 0; j < w; j++) {
+            const double v = base[base_len - w + j];
+            if (e->prec == SSA_PREC_F64) ((double *)e->hpin)[j] = v;
+            else ((float *)e->hpin)[j] = (float)v;
+        }
+        if (w > 0) {
+            HIPCHK(hipMemcpyAsync((char *)e->arena_a + e->esz * (d->off_out + d->N - w), e->hpin, (size_t)w * e->esz,
+                                  hipMemcpyHostToDevice, e->st));
+            HIPCHK(hipStreamSynchronize(e->st));   /* the staging buffer is reused below */
+        }
+    }
     e->hitems_n = e->dyn_base;
     span one;
     one.off = e->hitems_n;
     DEVCHK(items_push(e, s, 0, 0, 0));
     one.n = 1;
     DEVCHK(items_upload(e, e->dyn_base));
-    DEVCHK(ssa_dev_forecast(e->prec, h, 0, e->dd, it_ptr(e, one), 1, e->dgroups, e->dgroups + 4, e->arena_a,
+    DEVCHK(ssa_dev_forecast(e->prec, h, 0, vmode, e->dd, it_ptr(e, one), 1, e->dgroups, e->dgroups + 4, e->arena_a,
                             e->small, e->st));
     /* status of the series: 2 = verticality >= 1 */
     int status = 0;
@@ -1119,7 +1141,7 @@ static int launch_step(ssa_engine *e, int q)
     prof_end(e);
     if (e->step_h > 0) {
         prof_begin(e, PH_FORECAST, 0);
-        DEVCHK(ssa_dev_forecast(e->prec, e->step_h, 1, e->dd, it_ptr(e, e->ser), e->ser.n, e->dgroups, e->dgroups + 4,
+        DEVCHK(ssa_dev_forecast(e->prec, e->step_h, 1, 0, e->dd, it_ptr(e, e->ser), e->ser.n, e->dgroups, e->dgroups + 4,
                                 e->arena_a, e->small, e->st));
         prof_end(e);
     }

@@ -82,7 +82,10 @@ int ssa_engine_reconstruct(ssa_engine *e, int s, int ch, const int *glist, const
 int ssa_engine_wcorr(ssa_engine *e, int s, double *W);
 /* reconstruction of group + LRF forecast of h steps.  out: N + h values
  * (full) or the h forecasts; returns -1 when the verticality is >= 1 */
-int ssa_engine_forecast(ssa_engine *e, int s, const int *group, int n_group, int h, double *out, int full);
+/* R-forecast (vmode 0) or V-forecast (vmode 1) of group from the group's
+ * reconstruction, or (base != NULL) from the last L-1 values of base */
+int ssa_engine_forecast(ssa_engine *e, int s, const int *group, int n_group, int h, double *out, int full,
+                        int vmode, const double *base, int base_len);
 
 /* single-vector Hankel products of series s (dir 0: y = H v, 1: z = H^T u) */
 int ssa_engine_matvec(ssa_engine *e, int s, int dir, const double *in, double *out);

@@ -20,7 +20,8 @@
 //   k_recon     frequency-domain diagonal averaging (MKL/ssa_opt.h:2325-2400)
 //   k_olap      overlap-add of the chunked convolutions and 1/count weights
 //   k_wnorm     W-correlation normalisation (MKL/ssa_opt.h:2407-2477)
-//   k_forecast  LRF coefficients and recurrence (MKL/ssa_opt.h:2766-2885)
+//   k_forecast  LRF coefficients and recurrence (MKL/ssa_opt.h:2766-2885), or the
+//               V-forecast recurrence (MKL/ssa_opt.h:2906-3084)
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -1273,7 +1274,8 @@ __global__ __launch_bounds__(256) void k_wnorm(const ssa_desc *D, const ssa_item
 // reconstruction (at off_out, gid 0); forecasts written after it.
 template <typename T>
 __global__ __launch_bounds__(256) void k_forecast(ssa_desc *D, const ssa_item *items, const int4 *gdesc,
-                                                  const int *glist, T *bulk, double *small, int h, int limit_gsize)
+                                                  const int *glist, T *bulk, double *small, int h, int limit_gsize,
+                                                  int vmode)
 {
     __shared__ double red[32];
     __shared__ double pis[SSA_BMAX];
@@ -1305,13 +1307,40 @@ __global__ __launch_bounds__(256) void k_forecast(ssa_desc *D, const ssa_item *i
     double *buf = R + L;
     const T *rec = bulk + d.off_out;
     for (int j = threadIdx.x; j < L - 1; j += 256) {
-        double s = 0;
-        for (int g = 0; g < n; g++) s += pis[g] * (double)U[(long long)ids[g] * d.ML + j];
-        R[j] = s * sc;
+        if (!vmode) {
+            double s = 0;
+            for (int g = 0; g < n; g++) s += pis[g] * (double)U[(long long)ids[g] * d.ML + j];
+            R[j] = s * sc;
+        }
         buf[j] = (double)rec[d.N - (L - 1) + j];
     }
     __syncthreads();
     T *out = bulk + d.off_out + d.N;
+    if (vmode) {
+        // V-forecast (MKL/ssa_opt.h:2906-2965): each step projects the last
+        // L-1 values on every eigenvector of the group, x = sc * sum_g pi_g
+        // (U_g[0:L-1] . z); wave w takes components w, w+4, ...
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int t = 0; t < h; t++) {
+            double x = 0;
+            for (int g = w; g < n; g += 4) {
+                const T *u = U + (long long)ids[g] * d.ML;
+                double dot = 0;
+                for (int j = lane; j < L - 1; j += 64) dot += (double)u[j] * buf[t + j];
+                for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o);
+                x += pis[g] * dot;
+            }
+            if (lane == 0) red[w] = x;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const double v = (red[0] + red[1] + red[2] + red[3]) * sc;
+                buf[L - 1 + t] = v;
+                out[t] = (T)v;
+            }
+            __syncthreads();
+        }
+        return;
+    }
     for (int t = 0; t < h; t++) {
         double s = 0, dummy = 0;
         for (int j = threadIdx.x; j < L - 1; j += 256) s += R[j] * buf[t + j];
@@ -1609,16 +1638,18 @@ int ssa_dev_wcorr(int prec, int nb, const ssa_desc *D, const ssa_item *tile_item
     return (int)hipGetLastError();
 }
 
-int ssa_dev_forecast(int prec, int h, int limit_gsize, const ssa_desc *D, const ssa_item *items, int n_items,
-                     const int *gdesc, const int *glist, void *bulk, double *small, void *stream)
+int ssa_dev_forecast(int prec, int h, int limit_gsize, int vmode, const ssa_desc *D, const ssa_item *items,
+                     int n_items, const int *gdesc, const int *glist, void *bulk, double *small, void *stream)
 {
     if (n_items <= 0) return 0;
     if (prec == SSA_PREC_F64)
         hipLaunchKernelGGL(k_forecast<double>, dim3(n_items), dim3(256), 0, (hipStream_t)stream,
-                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (double *)bulk, small, h, limit_gsize);
+                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (double *)bulk, small, h, limit_gsize,
+                           vmode);
     else
         hipLaunchKernelGGL(k_forecast<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream,
-                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (float *)bulk, small, h, limit_gsize);
+                           (ssa_desc *)D, items, (const int4 *)gdesc, glist, (float *)bulk, small, h, limit_gsize,
+                           vmode);
     return (int)hipGetLastError();
 }
 

@@ -545,7 +545,7 @@ int ssa_opt_forecast(const SSA_Opt *ssa, const int *group, int n_group, int n_fo
 {
     if (!ssa || !ssa->decomposed || !group || !output || n_group < 1 || n_forecast < 1) return -1;
     if (n_forecast > SSA_MAX_H) return -1;
-    return ssa_engine_forecast(((ssa_ctx *)ssa->dev)->eng, 0, group, n_group, n_forecast, output, 0);
+    return ssa_engine_forecast(((ssa_ctx *)ssa->dev)->eng, 0, group, n_group, n_forecast, output, 0, 0, NULL, 0);
 }
 
 /* MKL/ssa_opt.h:2869-2885 */
@@ -553,7 +553,38 @@ int ssa_opt_forecast_full(const SSA_Opt *ssa, const int *group, int n_group, int
 {
     if (!ssa || !ssa->decomposed || !group || !output || n_group < 1 || n_forecast < 1) return -1;
     if (n_forecast > SSA_MAX_H) return -1;
-    return ssa_engine_forecast(((ssa_ctx *)ssa->dev)->eng, 0, group, n_group, n_forecast, output, 1);
+    return ssa_engine_forecast(((ssa_ctx *)ssa->dev)->eng, 0, group, n_group, n_forecast, output, 1, 0, NULL, 0);
+}
+
+/* V-forecast (MKL/ssa_opt.h:2906-2965): the same recurrence as the R-forecast,
+ * computed as it is defined there, one projection per eigenvector and step,
+ * x = (1/(1 - nu^2)) sum_g pi_g (U_g[0:L-1] . z).  Every index of the group must
+ * be a computed component and nu^2 < 1 - 1e-10, else -1. */
+int ssa_opt_vforecast(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output)
+{
+    if (!ssa || !ssa->decomposed || !group || !output || n_group < 1 || n_forecast < 1) return -1;
+    if (n_forecast > SSA_MAX_H) return -1;
+    return ssa_engine_forecast(((ssa_ctx *)ssa->dev)->eng, 0, group, n_group, n_forecast, output, 0, 1, NULL, 0);
+}
+
+/* MKL/ssa_opt.h:2967-3014: reconstruction followed by the V-forecast, N + h values */
+int ssa_opt_vforecast_full(const SSA_Opt *ssa, const int *group, int n_group, int n_forecast, ssa_real *output)
+{
+    if (!ssa || !ssa->decomposed || !group || !output || n_group < 1 || n_forecast < 1) return -1;
+    if (n_forecast > SSA_MAX_H) return -1;
+    return ssa_engine_forecast(((ssa_ctx *)ssa->dev)->eng, 0, group, n_group, n_forecast, output, 1, 1, NULL, 0);
+}
+
+/* MKL/ssa_opt.h:3016-3078: V-forecast continuing the caller's base signal (its
+ * last L-1 values; base_len >= L-1) instead of the group's reconstruction */
+int ssa_opt_vforecast_fast(const SSA_Opt *ssa, const int *group, int n_group, const ssa_real *base_signal,
+                           int base_len, int n_forecast, ssa_real *output)
+{
+    if (!ssa || !ssa->decomposed || !group || !base_signal || !output) return -1;
+    if (n_group < 1 || n_forecast < 1 || base_len < ssa->L - 1) return -1;
+    if (n_forecast > SSA_MAX_H) return -1;
+    return ssa_engine_forecast(((ssa_ctx *)ssa->dev)->eng, 0, group, n_group, n_forecast, output, 0, 1, base_signal,
+                               base_len);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -71,6 +71,9 @@ SIGNATURES = [
     ("ssa_opt_forecast_with_lrf", _I, [ctypes.POINTER(_SSA_LRF), c_double_p, _I, _I, c_double_p]),
     ("ssa_opt_forecast", _I, [_P, c_int_p, _I, _I, c_double_p]),
     ("ssa_opt_forecast_full", _I, [_P, c_int_p, _I, _I, c_double_p]),
+    ("ssa_opt_vforecast", _I, [_P, c_int_p, _I, _I, c_double_p]),
+    ("ssa_opt_vforecast_full", _I, [_P, c_int_p, _I, _I, c_double_p]),
+    ("ssa_opt_vforecast_fast", _I, [_P, c_int_p, _I, c_double_p, _I, _I, c_double_p]),
     ("mssa_opt_init", _I, [_M, c_double_p, _I, _I, _I]),
     ("mssa_opt_decompose", _I, [_M, _I, _I]),
     ("mssa_opt_reconstruct", _I, [_M, _I, c_int_p, _I, c_double_p]),
@@ -289,6 +292,34 @@ class SSA:
         g = np.ascontiguousarray(group, dtype=np.int32)
         if self._l.ssa_opt_forecast_full(ctypes.byref(self._ctx), _ip(g), len(group), n_forecast, _dp(out)) != 0:
             raise RuntimeError("Forecast failed")
+        return out
+
+    def vforecast(self, group: List[int], n_forecast: int) -> np.ndarray:
+        """V-forecast (MKL/py/ssa_wrapper.py:880-900, MKL/ssa_opt.h:2906-2965)."""
+        self._check()
+        out = np.zeros(n_forecast)
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        if self._l.ssa_opt_vforecast(ctypes.byref(self._ctx), _ip(g), len(group), n_forecast, _dp(out)) != 0:
+            raise RuntimeError("V-forecast failed")
+        return out
+
+    def vforecast_full(self, group: List[int], n_forecast: int) -> np.ndarray:
+        self._check()
+        out = np.zeros(self.N + n_forecast)
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        if self._l.ssa_opt_vforecast_full(ctypes.byref(self._ctx), _ip(g), len(group), n_forecast, _dp(out)) != 0:
+            raise RuntimeError("V-forecast failed")
+        return out
+
+    def vforecast_fast(self, group: List[int], base_signal, n_forecast: int) -> np.ndarray:
+        """V-forecast continuing base_signal (its last L-1 values)."""
+        self._check()
+        out = np.zeros(n_forecast)
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        base = np.ascontiguousarray(base_signal, dtype=np.float64)
+        if self._l.ssa_opt_vforecast_fast(ctypes.byref(self._ctx), _ip(g), len(group), _dp(base), len(base),
+                                          n_forecast, _dp(out)) != 0:
+            raise RuntimeError("V-forecast failed")
         return out
 
     def lrf(self, group: List[int]):

@@ -105,6 +105,28 @@ def test_forecast_and_lrf():
     assert np.abs(R - R2).max() < 1e-10
 
 
+def test_vforecast_against_oracle():
+    """ssa_opt_vforecast / _full / _fast (MKL/ssa_opt.h:2906-3078) against the
+    oracle's V-forecast, 1e-9 of RMS(x); the same recurrence as the R-forecast."""
+    x, dev, orc = _pair(1000, 250, 20, 10, 2, seed=1)
+    g = [0, 1, 2, 3, 4]
+    r = rms(x)
+    v = dev.vforecast(g, 50)
+    assert np.abs(v - orc.vforecast(g, 50)).max() / r < 1e-9
+    assert np.abs(v - dev.forecast(g, 50)).max() / r < 1e-9
+    ff = dev.vforecast_full(g, 30)
+    assert np.abs(ff[:1000] - orc.reconstruct(g)).max() / r < 1e-9
+    assert np.abs(ff[1000:] - orc.vforecast(g, 30)).max() / r < 1e-9
+    base = x[:900]
+    assert np.abs(dev.vforecast_fast(g, base, 20) - orc.vforecast(g, 20, base=base)).max() / r < 1e-9
+    with pytest.raises(RuntimeError):
+        dev.vforecast([0, 20], 5)               # index beyond the computed components
+    with pytest.raises(RuntimeError):
+        dev.vforecast_fast(g, x[:100], 5)       # base shorter than L - 1
+    # a failed call leaves the context usable
+    assert np.abs(dev.vforecast(g, 10) - v[:10]).max() == 0.0
+
+
 def test_reconstruct_many_matches_single():
     x, dev, orc = _pair(2000, 500, 12, 10, 2, seed=3)
     groups = [[i] for i in range(12)] + [[1, 2], [0, 3, 5]]

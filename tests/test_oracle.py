@@ -11,6 +11,7 @@ a wrong sign, a transposed operand, an off-by-one extraction) fails it:
     full-rank identity sum_i X_i = x;
   * W-correlation against its definition on explicit reconstructions;
   * the LRF against the sinusoid recurrence x[n] = 2 cos(w) x[n-1] - x[n-2];
+  * the V-forecast against the same sinusoid and the LRF recurrence;
   * the reference's own ssa_opt_ref.h, compiled from source (oracle/_ref).
 """
 import ctypes
@@ -223,6 +224,48 @@ def test_lrf_verticality_failure():
     s = O.OracleSSA(x, 10).decompose(1, 4, 1, O.philox_gaussian(5, 0, 41, 5))
     rc, _, nu = s.lrf([0])
     assert rc == -1 and nu >= 1 - 1e-10
+
+
+def test_vforecast_exact_on_sinusoid_and_equal_to_lrf():
+    """V-forecast (one projection per eigenvector and step) is the same
+    recurrence as the LRF forecast: on a sinusoid it continues the series
+    exactly, and on a noisy series it agrees with the R-forecast to rounding."""
+    N, L = 200, 40
+    omega = 2 * np.pi / 13.0
+    t = np.arange(N + 30)
+    full = np.sin(omega * t + 0.4)
+    s = O.OracleSSA(full[:N], L).decompose(2, 6, 2, O.philox_gaussian(4, 0, N - L + 1, 8))
+    v = s.vforecast([0, 1], 30)
+    assert np.abs(v - full[N:]).max() < 1e-9
+    x = workloads.single(600, seed=3)
+    s2 = O.OracleSSA(x, 150).decompose(12, 8, 2, O.philox_gaussian(7, 0, 451, 20))
+    g = [0, 1, 2, 3, 4]
+    v2, r2 = s2.vforecast(g, 40), s2.forecast(g, 40)
+    assert np.abs(v2 - r2).max() < 1e-10 * np.abs(r2).max()
+    # the fast form continues the caller's base: given the reconstruction it is the plain form
+    rec = s2.reconstruct(g)
+    assert np.array_equal(s2.vforecast(g, 40, base=rec), v2)
+    # and given another base it is the R-recurrence on that base
+    base = x[:500]
+    rc, R, _ = s2.lrf(g)
+    buf = list(base[-149:])
+    for _ in range(10):
+        buf.append(float(np.dot(R, buf[-149:])))
+    assert np.abs(s2.vforecast(g, 10, base=base) - np.array(buf[149:])).max() < 1e-10 * np.abs(base).max()
+
+
+def test_vforecast_rejects_bad_input():
+    x = workloads.single(300, seed=4)
+    s = O.OracleSSA(x, 60).decompose(6, 6, 1, O.philox_gaussian(8, 0, 241, 12))
+    with pytest.raises(RuntimeError):
+        s.vforecast([0, 6], 5)             # index beyond the computed components
+    with pytest.raises(RuntimeError):
+        s.vforecast([0], 5, base=x[:58])   # base shorter than L - 1
+    z = np.zeros(50)
+    z[-1] = 1.0
+    sv = O.OracleSSA(z, 10).decompose(1, 4, 1, O.philox_gaussian(5, 0, 41, 5))
+    with pytest.raises(RuntimeError):
+        sv.vforecast([0], 3)               # verticality >= 1
 
 
 def test_mssa_matvec_and_single_channel():
