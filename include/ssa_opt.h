@@ -160,6 +160,56 @@ int ssa_opt_vforecast_full(const SSA_Opt *ssa, const int *group, int n_group, in
 int ssa_opt_vforecast_fast(const SSA_Opt *ssa, const int *group, int n_group, const ssa_real *base_signal,
                            int base_len, int n_forecast, ssa_real *output);
 
+/* ---- Cadzow iterations and gap filling (MKL/ssa_opt.h:3147-3878, declared at :372-430) ----
+ * Host loops over this API: one device context is re-targeted with update_signal
+ * instead of a new context per iteration, and its test-matrix counter is reset
+ * so that every iteration draws the matrix a fresh context would (the reference
+ * re-initialises, restarting its stream). */
+typedef struct {
+    int iterations;       /* iterations performed */
+    ssa_real final_diff;  /* final relative difference */
+    ssa_real converged;   /* 1.0 if converged, 0.0 if max_iter was reached */
+} SSA_CadzowResult;
+
+typedef struct {
+    int iterations;
+    ssa_real final_diff;  /* final relative change of the gap values */
+    int converged;
+    int n_gaps;           /* NaN positions filled */
+} SSA_GapFillResult;
+
+/* :3165-3292: returns the iterations performed, -1 on bad arguments or a failed step */
+int ssa_opt_cadzow(const ssa_real *x, int N, int L, int rank, int max_iter, ssa_real tol, ssa_real *output,
+                   SSA_CadzowResult *result);
+/* :3294-3315: output = alpha * cadzow + (1 - alpha) * x, 0 < alpha <= 1 */
+int ssa_opt_cadzow_weighted(const ssa_real *x, int N, int L, int rank, int max_iter, ssa_real tol, ssa_real alpha,
+                            ssa_real *output, SSA_CadzowResult *result);
+/* :3319-3440: iterates on the context's own signal (update_signal between steps) */
+int ssa_opt_cadzow_inplace(SSA_Opt *ssa, int rank, int max_iter, ssa_real tol, SSA_CadzowResult *result);
+/* :3498-3683: NaN positions of x filled in place by iterated rank-`rank` reconstruction */
+int ssa_opt_gapfill(ssa_real *x, int N, int L, int rank, int max_iter, ssa_real tol, SSA_GapFillResult *result);
+/* :3685-3877: each gap filled from a forecast of the data before it and a backcast of the data after it */
+int ssa_opt_gapfill_simple(ssa_real *x, int N, int L, int rank, SSA_GapFillResult *result);
+
+/* ---- ESPRIT parameter estimation (MKL/ssa_opt.h:4289-4558, declared at :387-403) ----
+ * Host FP64 on the mirrored U: least squares U[0:L-1] Z = U[1:L] by Householder QR,
+ * eigenvalues of Z by Francis double-shift QR.  The poles come in this solver's
+ * order (the reference's follows dgeev). */
+typedef struct {
+    ssa_real *periods;      /* samples per cycle, infinite for |frequency| <= 1e-12 */
+    ssa_real *frequencies;  /* cycles per sample, arg(pole) / (2 pi) */
+    ssa_real *moduli;       /* |pole| */
+    ssa_real *rates;        /* log |pole|, -30 below 1e-12 */
+    int n_components;
+} SSA_ParEstimate;
+
+int ssa_opt_parestimate(const SSA_Opt *ssa, const int *group, int n_group, SSA_ParEstimate *result);
+void ssa_opt_parestimate_free(SSA_ParEstimate *result);
+/* No counterpart in the reference: the ESPRIT core of ssa_opt_parestimate on k
+ * columns U (L x k, column-major): poles wr + i wi.  For callers that hold
+ * eigenvectors themselves, and for host-only checks. */
+int ssa_amd_esprit_poles(const ssa_real *U, int L, int k, ssa_real *wr, ssa_real *wi);
+
 /* ---- MSSA (MKL/ssa_opt.h:3881-4283) ---- */
 int mssa_opt_init(MSSA_Opt *mssa, const ssa_real *X, int M, int N, int L);
 int mssa_opt_decompose(MSSA_Opt *mssa, int k, int oversampling);

@@ -40,6 +40,10 @@ def load():
     lib.orc_forecast.argtypes = [vp, _ip, ctypes.c_int, ctypes.c_int, _dp]
     lib.orc_forecast_full.argtypes = [vp, _ip, ctypes.c_int, ctypes.c_int, _dp]
     lib.orc_vforecast.argtypes = [vp, _ip, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp]
+    lib.orc_cadzow.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _dp]
+    lib.orc_cadzow_inplace.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint64, _dp]
+    lib.orc_gapfill.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp]
+    lib.orc_gapfill_simple.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
     lib.orc_variance_explained.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.orc_variance_explained.restype = ctypes.c_double
     lib.orc_dims.argtypes = [vp, _ip]
@@ -201,8 +205,47 @@ class OracleSSA:
             raise RuntimeError("oracle vforecast failed")
         return out
 
+    def cadzow_inplace(self, rank, max_iter, tol, counter0=0):
+        """Cadzow on this context's signal (MKL/ssa_opt.h:3319-3440); the
+        decompositions use Philox counters counter0, counter0 + 1, ..."""
+        r = np.zeros(3)
+        it = self._l.orc_cadzow_inplace(self._p, rank, max_iter, tol, counter0, _d(r))
+        if it < 0:
+            raise RuntimeError("oracle cadzow_inplace failed")
+        return it, float(r[1]), bool(r[2])
+
     def variance_explained(self, start, end):
         return self._l.orc_variance_explained(self._p, start, end)
+
+
+def cadzow(x, L, rank, max_iter=20, tol=1e-9):
+    """Cadzow iterations (MKL/ssa_opt.h:3165-3292): (signal, iterations, final_diff, converged)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(len(x))
+    r = np.zeros(3)
+    it = lib().orc_cadzow(_d(x), len(x), L, rank, max_iter, tol, _d(out), _d(r))
+    if it < 0:
+        raise RuntimeError("oracle cadzow failed")
+    return out, it, float(r[1]), bool(r[2])
+
+
+def gapfill(x, L, rank, max_iter=50, tol=1e-6):
+    """Iterative gap filling (MKL/ssa_opt.h:3498-3683) of the NaNs of a copy of x:
+    (filled, iterations, final_diff, converged, n_gaps)."""
+    y = np.array(x, dtype=np.float64)
+    r = np.zeros(4)
+    if lib().orc_gapfill(_d(y), len(y), L, rank, max_iter, tol, _d(r)) != 0:
+        raise RuntimeError("oracle gapfill failed")
+    return y, int(r[0]), float(r[1]), bool(r[2]), int(r[3])
+
+
+def gapfill_simple(x, L, rank):
+    """Forecast/backcast gap filling (MKL/ssa_opt.h:3685-3877) of a copy of x."""
+    y = np.array(x, dtype=np.float64)
+    r = np.zeros(4)
+    if lib().orc_gapfill_simple(_d(y), len(y), L, rank, _d(r)) != 0:
+        raise RuntimeError("oracle gapfill_simple failed")
+    return y, int(r[3])
 
 
 class OracleMSSA:

@@ -609,6 +609,203 @@ int orc_vforecast(const orc_ssa *s, const int *group, int n_group, const double 
     return 0;
 }
 
+/* ---- Cadzow iterations and gap filling (MKL/ssa_opt.h:3147-3878) ----
+ * The reference builds a fresh context (test-matrix stream restarted) for each
+ * iteration; every rank-r decomposition here is the one the device path
+ * makes: b = min(r + 8, L, K) columns, q = 2, the Philox matrix of seed 42 at
+ * the given counter. */
+static int cz_decompose(orc_ssa *s, int rank, uint64_t counter)
+{
+    int b = imin(rank + 8, imin(s->L, s->K));
+    double *om = (double *)xcalloc((size_t)s->K * b, sizeof(double));
+    orc_philox_gaussian(42, counter, s->K, b, om);
+    int rc = orc_decompose(s, rank, 8, 2, om);
+    free(om);
+    return rc;
+}
+
+static double cz_norm(const double *x, int n)
+{
+    double a = 0.0;
+    for (int i = 0; i < n; i++) a += x[i] * x[i];
+    return sqrt(a);
+}
+
+/* :3165-3292; out3 = {iterations, final_diff, converged}; returns iterations or -1 */
+int orc_cadzow(const double *x, int N, int L, int rank, int max_iter, double tol, double *output, double *out3)
+{
+    if (!x || !output || N < 4 || L < 2 || L > N - 1 || rank < 1 || max_iter < 1) return -1;
+    double *y = (double *)xcalloc(N, sizeof(double)), *yn = (double *)xcalloc(N, sizeof(double));
+    int *group = (int *)xcalloc(rank, sizeof(int));
+    for (int i = 0; i < rank; i++) group[i] = i;
+    memcpy(y, x, sizeof(double) * N);
+    double ynorm = cz_norm(y, N);
+    if (ynorm < 1e-15) ynorm = 1.0;
+    double diff = 0.0;
+    int conv = 0, it, rc = 0;
+    for (it = 0; it < max_iter; it++) {
+        orc_ssa s;
+        memset(&s, 0, sizeof(s));
+        rc = orc_init(&s, y, N, L);
+        if (rc == 0) rc = cz_decompose(&s, rank, 0);
+        if (rc == 0) rc = orc_reconstruct(&s, group, rank, yn);
+        orc_free(&s);
+        if (rc) break;
+        double d2 = 0.0;
+        for (int i = 0; i < N; i++) d2 += (yn[i] - y[i]) * (yn[i] - y[i]);
+        diff = sqrt(d2) / ynorm;
+        double *t = y; y = yn; yn = t;
+        if (diff < tol) { conv = 1; it++; break; }
+    }
+    if (rc == 0) {
+        memcpy(output, y, sizeof(double) * N);
+        if (out3) { out3[0] = it; out3[1] = diff; out3[2] = conv; }
+    }
+    free(y); free(yn); free(group);
+    return rc ? -1 : it;
+}
+
+/* :3319-3440 on a context whose test-matrix counter starts at `counter0`
+ * (one increment per decomposition) */
+int orc_cadzow_inplace(orc_ssa *s, int rank, int max_iter, double tol, uint64_t counter0, double *out3)
+{
+    if (!s || !s->X || rank < 1 || max_iter < 1) return -1;
+    int N = s->N;
+    double *yn = (double *)xcalloc(N, sizeof(double)), *yc = (double *)xcalloc(N, sizeof(double));
+    int *group = (int *)xcalloc(rank, sizeof(int));
+    for (int i = 0; i < rank; i++) group[i] = i;
+    double ynorm = 1.0, diff = 0.0;
+    int conv = 0, it, rc = 0;
+    uint64_t ctr = counter0;
+    for (it = 0; it < max_iter; it++) {
+        rc = cz_decompose(s, rank, ctr++);
+        if (rc) break;
+        if (it > 0) memcpy(yc, yn, sizeof(double) * N);
+        rc = orc_reconstruct(s, group, rank, yn);
+        if (rc) break;
+        if (it == 0) {
+            ynorm = cz_norm(yn, N);
+            if (ynorm < 1e-15) ynorm = 1.0;
+        } else {
+            double d2 = 0.0;
+            for (int i = 0; i < N; i++) d2 += (yn[i] - yc[i]) * (yn[i] - yc[i]);
+            diff = sqrt(d2) / ynorm;
+            if (diff < tol) { conv = 1; it++; break; }
+        }
+        if (it < max_iter - 1) orc_update_signal(s, yn);
+    }
+    if (rc == 0 && out3) { out3[0] = it; out3[1] = diff; out3[2] = conv; }
+    free(yn); free(yc); free(group);
+    return rc ? -1 : it;
+}
+
+/* :3455-3496 */
+static void cz_interp(double *x, const unsigned char *gap, int N)
+{
+    for (int i = 0; i < N;) {
+        if (!gap[i]) { i++; continue; }
+        int a = i;
+        while (i < N && gap[i]) i++;
+        int b = i, len = b - a;
+        double lv = a > 0 ? x[a - 1] : (b < N ? x[b] : 0.0);
+        double rv = b < N ? x[b] : lv;
+        for (int j = 0; j < len; j++) {
+            double t = len > 1 ? (double)(j + 1) / (len + 1) : 0.5;
+            x[a + j] = lv + t * (rv - lv);
+        }
+    }
+}
+
+/* :3498-3683; out4 = {iterations, final_diff, converged, n_gaps} */
+int orc_gapfill(double *x, int N, int L, int rank, int max_iter, double tol, double *out4)
+{
+    if (!x || N < 4 || L < 2 || L > N - 1 || rank < 1) return -1;
+    unsigned char *gap = (unsigned char *)xcalloc(N, 1);
+    int ng = 0;
+    for (int i = 0; i < N; i++) if (isnan(x[i])) { gap[i] = 1; ng++; }
+    if (ng == 0) {
+        free(gap);
+        if (out4) { out4[0] = 0; out4[1] = 0; out4[2] = 1; out4[3] = 0; }
+        return 0;
+    }
+    if (N - ng < L + rank) { free(gap); return -1; }
+    cz_interp(x, gap, N);
+    double *ov = (double *)xcalloc(ng, sizeof(double)), *nv = (double *)xcalloc(ng, sizeof(double));
+    double *rec = (double *)xcalloc(N, sizeof(double));
+    int *group = (int *)xcalloc(rank, sizeof(int));
+    for (int i = 0; i < rank; i++) group[i] = i;
+    for (int i = 0, g = 0; i < N; i++) if (gap[i]) ov[g++] = x[i];
+    double diff = 1e10;
+    int conv = 0, it, rc = 0;
+    for (it = 0; it < max_iter; it++) {
+        orc_ssa s;
+        memset(&s, 0, sizeof(s));
+        rc = orc_init(&s, x, N, L);
+        if (rc == 0) rc = cz_decompose(&s, rank, 0);
+        if (rc == 0) rc = orc_reconstruct(&s, group, rank, rec);
+        orc_free(&s);
+        if (rc) break;
+        for (int i = 0, g = 0; i < N; i++) if (gap[i]) { nv[g++] = rec[i]; x[i] = rec[i]; }
+        double sd = 0.0, so = 0.0;
+        for (int g = 0; g < ng; g++) { sd += (nv[g] - ov[g]) * (nv[g] - ov[g]); so += ov[g] * ov[g]; }
+        diff = so > 1e-12 ? sqrt(sd / so) : sqrt(sd);
+        if (diff < tol) { conv = 1; it++; break; }
+        memcpy(ov, nv, sizeof(double) * ng);
+    }
+    if (rc == 0 && out4) { out4[0] = it; out4[1] = diff; out4[2] = conv; out4[3] = ng; }
+    free(gap); free(ov); free(nv); free(rec); free(group);
+    return rc ? -1 : 0;
+}
+
+static int cz_seg_forecast(const double *seg, int len, int L, int rank, const int *group, int h, double *out)
+{
+    orc_ssa s;
+    memset(&s, 0, sizeof(s));
+    int rc = orc_init(&s, seg, len, L);
+    if (rc == 0) rc = cz_decompose(&s, rank, 0);
+    if (rc == 0) rc = orc_forecast(&s, group, rank, h, out);
+    orc_free(&s);
+    return rc;
+}
+
+/* :3685-3877 (a side whose forecast fails counts as absent) */
+int orc_gapfill_simple(double *x, int N, int L, int rank, double *out4)
+{
+    if (!x || N < 4 || L < 2 || L > N - 1 || rank < 1) return -1;
+    unsigned char *gap = (unsigned char *)xcalloc(N, 1);
+    int ng = 0;
+    for (int i = 0; i < N; i++) if (isnan(x[i])) { gap[i] = 1; ng++; }
+    int *group = (int *)xcalloc(rank, sizeof(int));
+    for (int i = 0; i < rank; i++) group[i] = i;
+    double *fl = (double *)xcalloc(N, sizeof(double)), *fr = (double *)xcalloc(N, sizeof(double));
+    double *rev = (double *)xcalloc(N, sizeof(double)), *tmp = (double *)xcalloc(N, sizeof(double));
+    for (int i = 0; i < N;) {
+        if (!gap[i]) { i++; continue; }
+        int a = i;
+        while (i < N && gap[i]) i++;
+        int b = i, len = b - a, hl = 0, hr = 0;
+        if (a >= L + rank) hl = cz_seg_forecast(x, a, L, rank, group, len, fl) == 0;
+        if (N - b >= L + rank) {
+            for (int j = 0; j < N - b; j++) rev[j] = x[N - 1 - j];
+            hr = cz_seg_forecast(rev, N - b, L, rank, group, len, tmp) == 0;
+            if (hr) for (int j = 0; j < len; j++) fr[j] = tmp[len - 1 - j];
+        }
+        for (int j = 0; j < len; j++) {
+            double v = 0.0, w = 0.0;
+            if (hl) { double wl = (double)(len - j) / len; v += wl * fl[j]; w += wl; }
+            if (hr) { double wr = (double)(j + 1) / len; v += wr * fr[j]; w += wr; }
+            if (w > 0) x[a + j] = v / w;
+            else {
+                double lv = a > 0 ? x[a - 1] : 0.0, rv = b < N ? x[b] : lv;
+                x[a + j] = (lv + rv) / 2.0;
+            }
+        }
+    }
+    if (out4) { out4[0] = 1; out4[1] = 0; out4[2] = 1; out4[3] = ng; }
+    free(gap); free(group); free(fl); free(fr); free(rev); free(tmp);
+    return 0;
+}
+
 double orc_variance_explained(const orc_ssa *s, int start, int end)   /* :3118-3128 */
 {
     if (!s || !s->decomposed || start < 0 || s->total_variance <= 0) return 0.0;

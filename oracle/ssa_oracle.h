@@ -77,6 +77,12 @@ int  orc_forecast_full(const orc_ssa *s, const int *group, int n_group, int h, d
 int  orc_vforecast(const orc_ssa *s, const int *group, int n_group, const double *base, int base_len, int h,
                    double *out);
 double orc_variance_explained(const orc_ssa *s, int start, int end);
+/* Cadzow / gap filling (MKL/ssa_opt.h:3147-3878); out3 = iterations, final_diff, converged;
+ * out4 adds n_gaps */
+int  orc_cadzow(const double *x, int N, int L, int rank, int max_iter, double tol, double *output, double *out3);
+int  orc_cadzow_inplace(orc_ssa *s, int rank, int max_iter, double tol, uint64_t counter0, double *out3);
+int  orc_gapfill(double *x, int N, int L, int rank, int max_iter, double tol, double *out4);
+int  orc_gapfill_simple(double *x, int N, int L, int rank, double *out4);
 
 /* ---- accessors for ctypes ---- */
 int  orc_dims(const orc_ssa *s, int *out4);   /* N, L, K, n */

@@ -215,23 +215,31 @@ static int items_upload(ssa_engine *e, long long from)
 static const ssa_item *it_ptr(const ssa_engine *e, span sp) { return e->ditems + sp.off; }
 
 /* matvec work items: (s, pair, o0, o1).  When a single input chunk feeds
- * several output chunks and the engine has plenty of pairs, one item
- * covers all output chunks so the input transform is reused. */
+ * several output chunks, an item covers a range of G output chunks and its
+ * input transform serves them all (1 + G transforms instead of 2G).  G is as
+ * large as keeps about MV_MIN_ITEMS items in the launch: the whole range for a
+ * large batch, short ranges for one series with many output chunks (MSSA),
+ * where a few long items would leave most CUs idle. */
+enum { MV_MIN_ITEMS = 1024 };
 static int build_matvec(ssa_engine *e, int dir, int lgn, span *sp)
 {
-    long long pairs_total = 0;
-    for (int s = 0; s < e->count; s++) pairs_total += (e->hd[s].b + 1) / 2;
+    long long units = 0;   /* (pair, output chunk) units of the launch */
+    for (int s = 0; s < e->count; s++) {
+        const ssa_desc *d = &e->hd[s];
+        if (d->lgn != lgn || d->b <= 0) continue;
+        units += (long long)((d->b + 1) / 2) * (dir == 0 ? d->M * d->AL : d->SK);
+    }
     sp->off = e->hitems_n;
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
         if (d->lgn != lgn || d->b <= 0) continue;
         int nLc = d->M * d->AL;
         int n_in = dir == 0 ? d->SK : nLc, n_out = dir == 0 ? nLc : d->SK;
-        int reuse = (n_in == 1) && (pairs_total * n_out >= 2048);
-        for (int p = 0; p < (d->b + 1) / 2; p++) {
-            if (reuse) { if (items_push(e, s, p, 0, n_out)) return -1; }
-            else for (int o = 0; o < n_out; o++) if (items_push(e, s, p, o, o + 1)) return -1;
-        }
+        long long g = n_in == 1 ? units / MV_MIN_ITEMS : 1;
+        int G = (int)(g < 1 ? 1 : g > n_out ? n_out : g);
+        for (int p = 0; p < (d->b + 1) / 2; p++)
+            for (int o = 0; o < n_out; o += G)
+                if (items_push(e, s, p, o, o + G < n_out ? o + G : n_out)) return -1;
     }
     sp->n = (int)(e->hitems_n - sp->off);
     return 0;

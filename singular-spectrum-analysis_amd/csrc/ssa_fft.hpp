@@ -185,14 +185,15 @@ constexpr long long TW_ENTRIES = tw_offset(0, 5);
 template <typename T, int LOGN, bool WIDE = false> struct FftGeom {
     static constexpr int N = 1 << LOGN;
     // WIDE, FP32 at N = 8192: 32 values per thread, a radix-32 pass then two
-    // radix-16 passes (two LDS exchanges instead of three), at half the waves
-    static constexpr int E = (WIDE && sizeof(T) == 4 && LOGN == 13) ? 32 : (sizeof(T) == 4 && LOGN >= 10) ? 16 : N >= 16 ? 8 : N / 2;
+    // radix-16 passes (two LDS exchanges instead of three), at half the waves;
+    // WIDE, FP64 at 1024 <= N <= 4096: radix-16 passes at 2 waves per SIMD
+    static constexpr int E = (WIDE && sizeof(T) == 4 && LOGN == 13) ? 32 : (sizeof(T) == 4 && LOGN >= 10) || (WIDE && LOGN >= 10 && LOGN <= 12) ? 16 : N >= 16 ? 8 : N / 2;
     static constexpr int NT = N / E;
     static constexpr int LOGE = (E == 32) ? 5 : (E == 16) ? 4 : (E == 8) ? 3 : (E == 4) ? 2 : 1;
     static constexpr int NEXT = (E == 32) ? 4 : LOGE;                     // radix log of passes 1..
     static constexpr int FIRST = (E == 32) ? LOGN - 8 : (LOGN % LOGE) ? (LOGN % LOGE) : LOGE;   // radix log of pass 0
     // waves per SIMD the transform kernels are built for (register budget)
-    static constexpr int OCC = (E == 32) ? 2 : (E == 8 && sizeof(T) == 4) ? 8 : 4;
+    static constexpr int OCC = (E == 32 || (E == 16 && sizeof(T) == 8)) ? 2 : (E == 8 && sizeof(T) == 4) ? 8 : 4;
     static constexpr int LDS = N;                                        // elements per plane
     // exchange one plane at a time when both planes would keep a second
     // workgroup off the CU (160 KB LDS)

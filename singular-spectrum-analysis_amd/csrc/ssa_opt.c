@@ -588,6 +588,342 @@ int ssa_opt_vforecast_fast(const SSA_Opt *ssa, const int *group, int n_group, co
 }
 
 /* ------------------------------------------------------------------------ */
+/* Cadzow iterations and gap filling (MKL/ssa_opt.h:3147-3878)               */
+/* ------------------------------------------------------------------------ */
+
+/* A rank-`rank` decomposition as the reference's iterated callers make it
+ * (MKL/ssa_opt.h:3206-3232): randomized with oversampling 8 when
+ * rank + 8 < min(L, K) / 2, else the block method.  `fresh` restarts the
+ * test-matrix counter: the reference builds a new context (and so restarts its
+ * random stream) for every iteration.  `may_prepare` = 0 is the in-place
+ * variant, which takes the randomized method only on an already prepared
+ * context wide enough for it (:3370-3377). */
+static int iter_decompose(SSA_Opt *ssa, int rank, int fresh, int may_prepare)
+{
+    int r = imin(ssa->L, ssa->K);
+    int use_rand = rank + 8 < r / 2;
+    if (use_rand && !may_prepare && !(ssa->prepared && rank + 8 <= ssa->prepared_kp)) use_rand = 0;
+    if (fresh) ssa->calls = 0;
+    if (use_rand && ((may_prepare && ssa_opt_prepare(ssa, rank, 8) != 0) ||
+                     ssa_opt_decompose_randomized(ssa, rank, 8) != 0))
+        use_rand = 0;
+    if (!use_rand) {
+        if (fresh) ssa->calls = 0;
+        if (ssa_opt_decompose_block(ssa, rank, imin(rank, 16), 3) != 0) return -1;
+    }
+    return 0;
+}
+
+static double norm2(const double *x, int n)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s += x[i] * x[i];
+    return sqrt(s);
+}
+
+/* MKL/ssa_opt.h:3165-3292 */
+int ssa_opt_cadzow(const ssa_real *x, int N, int L, int rank, int max_iter, ssa_real tol, ssa_real *output,
+                   SSA_CadzowResult *result)
+{
+    if (!x || !output || N < 4 || L < 2 || L > N - 1 || rank < 1 || max_iter < 1) return -1;
+    double *y = (double *)malloc(sizeof(double) * N), *y_new = (double *)malloc(sizeof(double) * N);
+    int *group = (int *)malloc(sizeof(int) * rank);
+    SSA_Opt ssa;
+    memset(&ssa, 0, sizeof(ssa));
+    int iter = -1;
+    if (!y || !y_new || !group) goto out;
+    for (int i = 0; i < rank; i++) group[i] = i;
+    memcpy(y, x, sizeof(double) * N);
+    double y_norm = norm2(y, N);
+    if (y_norm < 1e-15) y_norm = 1.0;
+    if (ssa_opt_init(&ssa, y, N, L) != 0) goto out;
+    double diff = 0.0;
+    bool converged = false;
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        if (it > 0 && ssa_opt_update_signal(&ssa, y) != 0) goto out;
+        if (iter_decompose(&ssa, rank, 1, 1) != 0 || ssa_opt_reconstruct(&ssa, group, rank, y_new) != 0) goto out;
+        double d2 = 0.0;
+        for (int i = 0; i < N; i++) {
+            double d = y_new[i] - y[i];
+            d2 += d * d;
+        }
+        diff = sqrt(d2) / y_norm;
+        double *t = y;
+        y = y_new;
+        y_new = t;
+        if (diff < tol) {
+            converged = true;
+            it++;
+            break;
+        }
+    }
+    memcpy(output, y, sizeof(double) * N);
+    if (result) {
+        result->iterations = it;
+        result->final_diff = diff;
+        result->converged = converged ? 1.0 : 0.0;
+    }
+    iter = it;
+out:
+    ssa_opt_free(&ssa);
+    free(y); free(y_new); free(group);
+    return iter;
+}
+
+/* MKL/ssa_opt.h:3294-3315 */
+int ssa_opt_cadzow_weighted(const ssa_real *x, int N, int L, int rank, int max_iter, ssa_real tol, ssa_real alpha,
+                            ssa_real *output, SSA_CadzowResult *result)
+{
+    if (!(alpha > 0.0) || alpha > 1.0) return -1;
+    int ret = ssa_opt_cadzow(x, N, L, rank, max_iter, tol, output, result);
+    if (ret < 0) return ret;
+    if (alpha < 1.0) {
+        double beta = 1.0 - alpha;
+        for (int i = 0; i < N; i++) output[i] = alpha * output[i] + beta * x[i];
+    }
+    return ret;
+}
+
+/* MKL/ssa_opt.h:3319-3440: the context's own signal, re-decomposed in place.
+ * The first reconstruction sets the scale; later ones are compared with the
+ * previous reconstruction and become the signal for the next step. */
+int ssa_opt_cadzow_inplace(SSA_Opt *ssa, int rank, int max_iter, ssa_real tol, SSA_CadzowResult *result)
+{
+    if (!ssa || !ssa->initialized || rank < 1 || max_iter < 1) return -1;
+    int N = ssa->N;
+    double *y_new = (double *)malloc(sizeof(double) * N), *y_cur = (double *)malloc(sizeof(double) * N);
+    int *group = (int *)malloc(sizeof(int) * rank);
+    int iter = -1;
+    if (!y_new || !y_cur || !group) goto out;
+    for (int i = 0; i < rank; i++) group[i] = i;
+    double y_norm = 1.0, diff = 0.0;
+    bool converged = false;
+    int it;
+    for (it = 0; it < max_iter; it++) {
+        ssa->decomposed = false;
+        ssa->n_components = 0;
+        ssa->total_variance = 0.0;
+        ssa_opt_free_cached_ffts(ssa);
+        if (iter_decompose(ssa, rank, 0, 0) != 0) goto out;
+        if (it > 0) memcpy(y_cur, y_new, sizeof(double) * N);
+        if (ssa_opt_reconstruct(ssa, group, rank, y_new) != 0) goto out;
+        if (it == 0) {
+            y_norm = norm2(y_new, N);
+            if (y_norm < 1e-15) y_norm = 1.0;
+        } else {
+            double d2 = 0.0;
+            for (int i = 0; i < N; i++) {
+                double d = y_new[i] - y_cur[i];
+                d2 += d * d;
+            }
+            diff = sqrt(d2) / y_norm;
+            if (diff < tol) {
+                converged = true;
+                it++;
+                break;
+            }
+        }
+        if (it < max_iter - 1 && ssa_opt_update_signal(ssa, y_new) != 0) goto out;
+    }
+    if (result) {
+        result->iterations = it;
+        result->final_diff = diff;
+        result->converged = converged ? 1.0 : 0.0;
+    }
+    iter = it;
+out:
+    free(y_new); free(y_cur); free(group);
+    return iter;
+}
+
+/* MKL/ssa_opt.h:3455-3496: gaps start as the straight line between their
+ * neighbours (a gap at an end takes the value of its one neighbour) */
+static void interp_gaps(double *x, const unsigned char *gap, int N)
+{
+    int i = 0;
+    while (i < N) {
+        if (!gap[i]) {
+            i++;
+            continue;
+        }
+        int a = i;
+        while (i < N && gap[i]) i++;
+        int b = i, len = b - a;
+        double lv, rv;
+        if (a > 0) lv = x[a - 1];
+        else lv = (b < N) ? x[b] : 0.0;
+        rv = (b < N) ? x[b] : lv;
+        for (int j = 0; j < len; j++) {
+            double t = (len > 1) ? (double)(j + 1) / (len + 1) : 0.5;
+            x[a + j] = lv + t * (rv - lv);
+        }
+    }
+}
+
+/* MKL/ssa_opt.h:3498-3683 */
+int ssa_opt_gapfill(ssa_real *x, int N, int L, int rank, int max_iter, ssa_real tol, SSA_GapFillResult *result)
+{
+    if (!x || N < 4 || L < 2 || L > N - 1 || rank < 1) return -1;
+    unsigned char *gap = (unsigned char *)calloc(N, 1);
+    if (!gap) return -1;
+    int n_gaps = 0;
+    for (int i = 0; i < N; i++)
+        if (isnan(x[i])) {
+            gap[i] = 1;
+            n_gaps++;
+        }
+    if (n_gaps == 0) {
+        free(gap);
+        if (result) {
+            result->iterations = 0;
+            result->final_diff = 0.0;
+            result->converged = 1;
+            result->n_gaps = 0;
+        }
+        return 0;
+    }
+    if (N - n_gaps < L + rank) {
+        free(gap);
+        return -1;
+    }
+    interp_gaps(x, gap, N);
+    double *old_v = (double *)malloc(sizeof(double) * n_gaps), *new_v = (double *)malloc(sizeof(double) * n_gaps);
+    double *rec = (double *)malloc(sizeof(double) * N);
+    int *group = (int *)malloc(sizeof(int) * rank);
+    SSA_Opt ssa;
+    memset(&ssa, 0, sizeof(ssa));
+    int rc = -1;
+    if (!old_v || !new_v || !rec || !group) goto out;
+    for (int i = 0; i < rank; i++) group[i] = i;
+    for (int i = 0, g = 0; i < N; i++) if (gap[i]) old_v[g++] = x[i];
+    double diff = 1e10;
+    int converged = 0, it;
+    for (it = 0; it < max_iter; it++) {
+        /* a fresh decomposition of the current signal (:3591-3603) */
+        if (it == 0 ? ssa_opt_init(&ssa, x, N, L) != 0 : ssa_opt_update_signal(&ssa, x) != 0) goto out;
+        ssa.calls = 0;
+        ssa_opt_prepare(&ssa, rank, 8);
+        if (ssa_opt_decompose_randomized(&ssa, rank, 8) != 0 || ssa_opt_reconstruct(&ssa, group, rank, rec) != 0)
+            goto out;
+        for (int i = 0, g = 0; i < N; i++)
+            if (gap[i]) {
+                new_v[g++] = rec[i];
+                x[i] = rec[i];
+            }
+        double sd = 0.0, so = 0.0;
+        for (int g = 0; g < n_gaps; g++) {
+            double d = new_v[g] - old_v[g];
+            sd += d * d;
+            so += old_v[g] * old_v[g];
+        }
+        diff = (so > 1e-12) ? sqrt(sd / so) : sqrt(sd);
+        if (diff < tol) {
+            converged = 1;
+            it++;
+            break;
+        }
+        memcpy(old_v, new_v, sizeof(double) * n_gaps);
+    }
+    if (result) {
+        result->iterations = it;
+        result->final_diff = diff;
+        result->converged = converged;
+        result->n_gaps = n_gaps;
+    }
+    rc = 0;
+out:
+    ssa_opt_free(&ssa);
+    free(gap); free(old_v); free(new_v); free(rec); free(group);
+    return rc;
+}
+
+/* forecast of h values continuing seg[0:len) from its rank-`rank` decomposition
+ * (a fresh context, randomized, oversampling 8); 0 on success */
+static int segment_forecast(const double *seg, int len, int L, int rank, const int *group, int h, double *out)
+{
+    SSA_Opt s;
+    memset(&s, 0, sizeof(s));
+    int rc = ssa_opt_init(&s, seg, len, L);
+    if (rc == 0) {
+        ssa_opt_prepare(&s, rank, 8);
+        rc = ssa_opt_decompose_randomized(&s, rank, 8);
+    }
+    if (rc == 0) rc = ssa_opt_forecast(&s, group, rank, h, out);
+    ssa_opt_free(&s);
+    return rc;
+}
+
+/* MKL/ssa_opt.h:3685-3877.  Each gap in turn (left to right, earlier fills
+ * included in later left segments): a forecast from the data before it when
+ * that is at least L + rank long, a backcast (the forecast of the reversed
+ * tail of the series) when the data after it is, blended with weights falling
+ * linearly away from each side; neither: the mean of the two neighbours.  A
+ * side whose forecast fails (verticality) counts as absent. */
+int ssa_opt_gapfill_simple(ssa_real *x, int N, int L, int rank, SSA_GapFillResult *result)
+{
+    if (!x || N < 4 || L < 2 || L > N - 1 || rank < 1) return -1;
+    unsigned char *gap = (unsigned char *)calloc(N, 1);
+    int *group = (int *)malloc(sizeof(int) * rank);
+    double *fl = (double *)malloc(sizeof(double) * N), *fr = (double *)malloc(sizeof(double) * N);
+    double *rev = (double *)malloc(sizeof(double) * N), *tmp = (double *)malloc(sizeof(double) * N);
+    int rc = -1, n_gaps = 0;
+    if (!gap || !group || !fl || !fr || !rev || !tmp) goto out;
+    for (int i = 0; i < rank; i++) group[i] = i;
+    for (int i = 0; i < N; i++)
+        if (isnan(x[i])) {
+            gap[i] = 1;
+            n_gaps++;
+        }
+    for (int i = 0; i < N;) {
+        if (!gap[i]) {
+            i++;
+            continue;
+        }
+        int a = i;
+        while (i < N && gap[i]) i++;
+        int b = i, len = b - a;
+        int left_len = a, right_len = N - b;
+        int have_l = 0, have_r = 0;
+        if (left_len >= L + rank) have_l = segment_forecast(x, left_len, L, rank, group, len, fl) == 0;
+        if (right_len >= L + rank) {
+            for (int j = 0; j < right_len; j++) rev[j] = x[N - 1 - j];
+            have_r = segment_forecast(rev, right_len, L, rank, group, len, tmp) == 0;
+            if (have_r) for (int j = 0; j < len; j++) fr[j] = tmp[len - 1 - j];
+        }
+        for (int j = 0; j < len; j++) {
+            double v = 0.0, w = 0.0;
+            if (have_l) {
+                double wl = (double)(len - j) / len;
+                v += wl * fl[j];
+                w += wl;
+            }
+            if (have_r) {
+                double wr = (double)(j + 1) / len;
+                v += wr * fr[j];
+                w += wr;
+            }
+            if (w > 0) x[a + j] = v / w;
+            else {
+                double lv = (a > 0) ? x[a - 1] : 0.0;
+                double rv = (b < N) ? x[b] : lv;
+                x[a + j] = (lv + rv) / 2.0;
+            }
+        }
+    }
+    if (result) {
+        result->iterations = 1;
+        result->final_diff = 0.0;
+        result->converged = 1;
+        result->n_gaps = n_gaps;
+    }
+    rc = 0;
+out:
+    free(gap); free(group); free(fl); free(fr); free(rev); free(tmp);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
 /* extensions                                                                */
 /* ------------------------------------------------------------------------ */
 

@@ -35,6 +35,23 @@ class _SSA_LRF(ctypes.Structure):
                 ("valid", ctypes.c_bool)]
 
 
+class _SSA_CadzowResult(ctypes.Structure):
+    # MKL/py/ssa_wrapper.py:108-114
+    _fields_ = [("iterations", ctypes.c_int), ("final_diff", ctypes.c_double), ("converged", ctypes.c_double)]
+
+
+class _SSA_GapFillResult(ctypes.Structure):
+    # MKL/py/ssa_wrapper.py:249-255
+    _fields_ = [("iterations", ctypes.c_int), ("final_diff", ctypes.c_double), ("converged", ctypes.c_int),
+                ("n_gaps", ctypes.c_int)]
+
+
+class _SSA_ParEstimate(ctypes.Structure):
+    # MKL/ssa_opt.h:389-396
+    _fields_ = [("periods", c_double_p), ("frequencies", c_double_p), ("moduli", c_double_p),
+                ("rates", c_double_p), ("n_components", ctypes.c_int)]
+
+
 # (name, restype, argtypes) of every symbol include/ssa_opt.h declares
 _P = ctypes.POINTER(_SSA_Opt)
 _M = ctypes.POINTER(_MSSA_Opt)
@@ -74,6 +91,15 @@ SIGNATURES = [
     ("ssa_opt_vforecast", _I, [_P, c_int_p, _I, _I, c_double_p]),
     ("ssa_opt_vforecast_full", _I, [_P, c_int_p, _I, _I, c_double_p]),
     ("ssa_opt_vforecast_fast", _I, [_P, c_int_p, _I, c_double_p, _I, _I, c_double_p]),
+    ("ssa_opt_cadzow", _I, [c_double_p, _I, _I, _I, _I, _D, c_double_p, ctypes.POINTER(_SSA_CadzowResult)]),
+    ("ssa_opt_cadzow_weighted", _I, [c_double_p, _I, _I, _I, _I, _D, _D, c_double_p,
+                                     ctypes.POINTER(_SSA_CadzowResult)]),
+    ("ssa_opt_cadzow_inplace", _I, [_P, _I, _I, _D, ctypes.POINTER(_SSA_CadzowResult)]),
+    ("ssa_opt_gapfill", _I, [c_double_p, _I, _I, _I, _I, _D, ctypes.POINTER(_SSA_GapFillResult)]),
+    ("ssa_opt_gapfill_simple", _I, [c_double_p, _I, _I, _I, ctypes.POINTER(_SSA_GapFillResult)]),
+    ("ssa_opt_parestimate", _I, [_P, c_int_p, _I, ctypes.POINTER(_SSA_ParEstimate)]),
+    ("ssa_opt_parestimate_free", None, [ctypes.POINTER(_SSA_ParEstimate)]),
+    ("ssa_amd_esprit_poles", _I, [c_double_p, _I, _I, c_double_p, c_double_p]),
     ("mssa_opt_init", _I, [_M, c_double_p, _I, _I, _I]),
     ("mssa_opt_decompose", _I, [_M, _I, _I]),
     ("mssa_opt_reconstruct", _I, [_M, _I, c_int_p, _I, c_double_p]),
@@ -143,6 +169,46 @@ def autotune(precision=1):
     if lib().ssa_amd_autotune(int(precision), _dp(out)) != 0:
         raise RuntimeError("autotune failed")
     return {6 + i: float(v) for i, v in enumerate(out)}
+
+
+def cadzow(x, L, rank, max_iter=20, tol=1e-9, alpha=1.0):
+    """Cadzow iterations (MKL/py/ssa_wrapper.py:415-480): (signal, iterations,
+    final_diff, converged); alpha < 1 blends with the input."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(len(x))
+    r = _SSA_CadzowResult()
+    if alpha == 1.0:
+        it = lib().ssa_opt_cadzow(_dp(x), len(x), L, rank, max_iter, tol, _dp(out), ctypes.byref(r))
+    else:
+        it = lib().ssa_opt_cadzow_weighted(_dp(x), len(x), L, rank, max_iter, tol, alpha, _dp(out), ctypes.byref(r))
+    if it < 0:
+        raise RuntimeError("Cadzow iterations failed")
+    return out, r.iterations, r.final_diff, r.converged > 0.5
+
+
+def esprit_poles(U):
+    """ESPRIT poles of the columns of U (L x k): the host core of
+    ssa_opt_parestimate (no device call)."""
+    U = np.asfortranarray(U, dtype=np.float64)
+    L, k = U.shape
+    wr, wi = np.zeros(k), np.zeros(k)
+    if lib().ssa_amd_esprit_poles(U.ctypes.data_as(c_double_p), L, k, _dp(wr), _dp(wi)) != 0:
+        raise RuntimeError("ESPRIT failed")
+    return wr + 1j * wi
+
+
+def gapfill(x, L, rank, max_iter=50, tol=1e-6, simple=False):
+    """Fill the NaNs of a copy of x (MKL/ssa_opt.h:3498-3877): (filled,
+    iterations, final_diff, converged, n_gaps)."""
+    y = np.array(x, dtype=np.float64)
+    r = _SSA_GapFillResult()
+    if simple:
+        rc = lib().ssa_opt_gapfill_simple(_dp(y), len(y), L, rank, ctypes.byref(r))
+    else:
+        rc = lib().ssa_opt_gapfill(_dp(y), len(y), L, rank, max_iter, tol, ctypes.byref(r))
+    if rc != 0:
+        raise RuntimeError("gap filling failed")
+    return y, r.iterations, r.final_diff, bool(r.converged), r.n_gaps
 
 
 class SSA:
@@ -320,6 +386,36 @@ class SSA:
         if self._l.ssa_opt_vforecast_fast(ctypes.byref(self._ctx), _ip(g), len(group), _dp(base), len(base),
                                           n_forecast, _dp(out)) != 0:
             raise RuntimeError("V-forecast failed")
+        return out
+
+    def cadzow_inplace(self, rank: int, max_iter: int = 20, tol: float = 1e-9):
+        """Cadzow on this context's signal (MKL/ssa_opt.h:3319-3440):
+        (iterations, final_diff, converged); the context keeps the result."""
+        r = _SSA_CadzowResult()
+        it = self._l.ssa_opt_cadzow_inplace(ctypes.byref(self._ctx), rank, max_iter, tol, ctypes.byref(r))
+        if it < 0:
+            raise RuntimeError("Cadzow iterations failed")
+        self.n_components = rank
+        self._decomposed = True
+        return r.iterations, r.final_diff, r.converged > 0.5
+
+    def parestimate(self, group: List[int] = None) -> dict:
+        """ESPRIT periods, frequencies, moduli and rates of the group's
+        eigenvectors (all computed components when group is None;
+        MKL/ssa_opt.h:4304-4558)."""
+        self._check()
+        r = _SSA_ParEstimate()
+        if group is None:
+            rc = self._l.ssa_opt_parestimate(ctypes.byref(self._ctx), None, 0, ctypes.byref(r))
+        else:
+            g = np.ascontiguousarray(group, dtype=np.int32)
+            rc = self._l.ssa_opt_parestimate(ctypes.byref(self._ctx), _ip(g), len(g), ctypes.byref(r))
+        if rc != 0:
+            raise RuntimeError("ESPRIT failed")
+        n = r.n_components
+        out = {f: np.ctypeslib.as_array(getattr(r, f), shape=(n,)).copy()
+               for f in ("periods", "frequencies", "moduli", "rates")}
+        self._l.ssa_opt_parestimate_free(ctypes.byref(r))
         return out
 
     def lrf(self, group: List[int]):

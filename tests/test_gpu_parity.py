@@ -127,6 +127,32 @@ def test_vforecast_against_oracle():
     assert np.abs(dev.vforecast(g, 10) - v[:10]).max() == 0.0
 
 
+def test_parestimate_esprit():
+    """ssa_opt_parestimate (MKL/ssa_opt.h:4304-4558) on a device
+    decomposition: on a rank-4 signal (two sinusoids) the four computed
+    eigenvectors span the signal subspace, so the poles are e^{+-2 pi i / P}."""
+    S = _ssa()
+    t = np.arange(1000)
+    x = np.sin(2 * np.pi * t / 25) + 0.5 * np.sin(2 * np.pi * t / 60 + 1.0)
+    dev = S.SSA(x, 250, q=2)
+    dev.decompose(4, oversampling=8)
+    r = dev.parestimate()
+    assert np.allclose(np.sort(r["periods"]), [25, 25, 60, 60], rtol=1e-8)
+    assert np.allclose(r["moduli"], 1.0, atol=1e-9)
+    assert np.allclose(np.sort(r["frequencies"]), [-1 / 25, -1 / 60, 1 / 60, 1 / 25], atol=1e-10)
+    assert np.all(np.abs(r["rates"]) < 1e-9)
+    # the first pair alone: the two sinusoids are only nearly separable at this
+    # window, so its poles are those of the dense SVD's first two eigenvectors
+    H = hankel(x, 250)
+    U2 = np.linalg.svd(H, full_matrices=False)[0][:, :2]
+    ev = np.linalg.eigvals(np.linalg.lstsq(U2[:-1], U2[1:], rcond=None)[0])
+    r01 = dev.parestimate([0, 1])
+    assert np.allclose(np.sort(r01["periods"]), np.sort(1 / np.abs(np.angle(ev) / (2 * np.pi))), rtol=1e-9)
+    assert abs(r01["periods"][0] - 25) < 0.01
+    with pytest.raises(RuntimeError):
+        dev.parestimate([0, 7])            # index beyond the computed components
+
+
 def test_reconstruct_many_matches_single():
     x, dev, orc = _pair(2000, 500, 12, 10, 2, seed=3)
     groups = [[i] for i in range(12)] + [[1, 2], [0, 3, 5]]

@@ -331,3 +331,77 @@ def test_reference_randomized_qr_defect():
     r = O.RefSSA(x, 250).decompose_randomized(10, 10)
     sv = np.linalg.svd(hankel(x, 250), compute_uv=False)
     assert abs(r.sigma[0] / sv[0] - 1.0) > 1e-3
+
+
+def _sines(N, periods=(17.0, 41.0), amps=(1.0, 0.6), noise=0.0, seed=0):
+    t = np.arange(N)
+    x = sum(a * np.sin(2 * np.pi * t / p + 0.3 * i) for i, (a, p) in enumerate(zip(amps, periods)))
+    if noise:
+        x = x + noise * np.random.default_rng(seed).standard_normal(N)
+    return x
+
+
+def test_cadzow_single_step_is_one_reconstruction():
+    """max_iter = 1 is one rank-r reconstruction with the Philox matrix of a
+    fresh context (seed 42, counter 0), b = min(r + 8, L, K), q = 2."""
+    x = _sines(300, noise=0.3, seed=1)
+    L, r = 60, 4
+    out, it, diff, conv = O.cadzow(x, L, r, max_iter=1, tol=0.0)
+    K = 300 - L + 1
+    ref = O.OracleSSA(x, L).decompose(r, 8, 2, O.philox_gaussian(42, 0, K, r + 8)).reconstruct(list(range(r)))
+    assert it == 1 and not conv
+    assert np.array_equal(out, ref)
+    assert diff == pytest.approx(np.linalg.norm(ref - x) / np.linalg.norm(x), rel=1e-12)
+
+
+def test_cadzow_fixed_point_and_rank_reduction():
+    # a rank-4 signal (two sinusoids) is its own rank-4 Hankel projection
+    x = _sines(400)
+    out, it, diff, conv = O.cadzow(x, 80, 4, max_iter=5, tol=1e-9)
+    assert conv and it <= 2 and np.abs(out - x).max() < 1e-9
+    # on noisy data the iterations approach a rank-4 trajectory matrix
+    y = _sines(400, noise=0.5, seed=2)
+    out, it, diff, conv = O.cadzow(y, 80, 4, max_iter=60, tol=1e-14)
+    s_in = np.linalg.svd(hankel(y, 80), compute_uv=False)
+    s_out = np.linalg.svd(hankel(out, 80), compute_uv=False)
+    assert s_out[4] / s_out[0] < 0.05 * s_in[4] / s_in[0]
+
+
+def test_cadzow_inplace_converges_on_finite_rank():
+    x = _sines(400)
+    s = O.OracleSSA(x, 80)
+    it, diff, conv = s.cadzow_inplace(4, 5, 1e-9)
+    assert conv and it == 2 and diff < 1e-9
+    assert np.abs(s.reconstruct([0, 1, 2, 3]) - x).max() < 1e-9
+
+
+def test_gapfill_recovers_sinusoids():
+    x = _sines(500)
+    y = x.copy()
+    y[200:215] = np.nan
+    y[400] = np.nan
+    filled, it, diff, conv, ng = O.gapfill(y, 80, 4, max_iter=200, tol=1e-12)
+    assert ng == 16 and conv
+    assert np.abs(filled - x).max() < 1e-6
+    # the initial guess is the straight line between the neighbours
+    z = x.copy()
+    z[100:104] = np.nan
+    lin, it1, _, _, _ = O.gapfill(z, 80, 4, max_iter=0, tol=0.0)
+    assert it1 == 0 and np.allclose(lin[100:104], x[99] + (x[104] - x[99]) * np.arange(1, 5) / 5, atol=1e-15)
+    # no gaps: untouched
+    same, it0, _, conv0, ng0 = O.gapfill(x, 80, 4)
+    assert ng0 == 0 and it0 == 0 and conv0 and np.array_equal(same, x)
+
+
+def test_gapfill_simple_exact_on_sinusoids():
+    x = _sines(600)
+    y = x.copy()
+    y[300:320] = np.nan
+    filled, ng = O.gapfill_simple(y, 60, 4)
+    assert ng == 20
+    assert np.abs(filled - x).max() < 1e-8
+    # a gap at the start has no left segment: the backcast alone fills it
+    z = x.copy()
+    z[0:10] = np.nan
+    f0, _ = O.gapfill_simple(z, 60, 4)
+    assert np.abs(f0 - x).max() < 1e-8
