@@ -90,8 +90,10 @@ def roofline_of(batch, plans, Ns, Ls, s):
     phase_tot = {}
     for (ph, lg), v in agg.items():
         phase_tot[ph] = phase_tot.get(ph, 0.0) + sum(v)
-    # dominant kernel instantiation: k_matvec<T, lg, 0> (both directions move
-    # the same bytes; the generated-Omega launch is another instantiation)
+    # dominant kernel: the k_matvec instantiation of the bucket with the largest
+    # share (both directions move the same bytes); the benchmark batch gives
+    # each branch's launches of that bucket thousands of items, so they take
+    # the wide geometry (k_matvec<T, lg, true>)
     mv = {}
     for (ph, lg), v in agg.items():
         if ph in ("matvec_H", "matvec_HT"):
@@ -104,7 +106,7 @@ def roofline_of(batch, plans, Ns, Ls, s):
     bytes_launch = step_bytes / len(v)
     avg_ms = float(np.mean(v))
     achieved = step_bytes / (float(np.sum(v)) * 1e-3) / 1e9
-    kernel = f"k_matvec<{'float' if s == 4 else 'double'}, {lg}>"
+    kernel = f"k_matvec<{'float' if s == 4 else 'double'}, {lg}, true>"
     traffic = measured_traffic(kernel)
     return {
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -197,7 +199,8 @@ def single_configs(reps=3):
     (median of `reps`), plain and mixed, and the singular-value error of both
     forms against a dense SVD of the explicit Hankel matrix at N = 4000,
     L = 1000.  cfg4: MSSA M = 16, N = 10000, L = 2500, k = 64, p = 10:
-    decompose + per-channel reconstruction of components 0..8."""
+    decompose + per-channel reconstruction of components 0..8.  Streaming
+    tick latency (p50, p99) at N = 1000 and N = 20000."""
     out = {}
     x = workloads.single(20000, seed=2)
     groups = [[i] for i in range(50)]
@@ -230,6 +233,23 @@ def single_configs(reps=3):
             if r:
                 t.append(time.perf_counter() - a)
         out[f"cfg4_mssa_{name}_ms"] = round(1e3 * float(np.median(t)), 2)
+    # streaming ticks, the reference's low-latency pattern (MKL/ssa_opt_r2c.h:82-91):
+    # one prepared context; per tick update_signal on the window moved by one
+    # sample, decompose_randomized (k = 50, p = 10, q = 2) and the trend
+    # reconstruction, timed end to end on the host (plain FP64)
+    for N, L in ((1000, 250), (20000, 5000)):
+        xs_long = workloads.single(N + 101, seed=7)
+        s = ssa_amd.SSA(xs_long[:N], L, q=Q_POWER, precision=0, seed=14)
+        s.decompose(50, oversampling=P_OVERSAMPLE)
+        lat = []
+        for tick in range(1, 101):
+            a = time.perf_counter()
+            s.update_signal(xs_long[tick:tick + N])
+            s.decompose(50, oversampling=P_OVERSAMPLE)
+            s.reconstruct([0])
+            lat.append(time.perf_counter() - a)
+        out[f"stream_N{N}_p50_ms"] = round(1e3 * float(np.percentile(lat, 50)), 3)
+        out[f"stream_N{N}_p99_ms"] = round(1e3 * float(np.percentile(lat, 99)), 3)
     return out
 
 

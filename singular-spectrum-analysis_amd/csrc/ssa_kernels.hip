@@ -207,13 +207,13 @@ __global__ __launch_bounds__(256) void k_omega(const ssa_desc *__restrict__ D, c
 // (chunk o = m*AL + a), input chunks run over K; dir 1: the reverse.
 // out[o][i] = sum_p sum_j seg_(o,p)[i + j] in_p[j]  via  T+( X_(o,p) . T+(in_p) ).
 // ---------------------------------------------------------------------------
-template <typename T, int LOGN>
-__global__ __launch_bounds__((FftGeom<T, LOGN, true>::NT), (FftGeom<T, LOGN, true>::OCC)) void k_matvec(const ssa_desc *__restrict__ D,
+template <typename T, int LOGN, bool W>
+__global__ __launch_bounds__((FftGeom<T, LOGN, W>::NT), (FftGeom<T, LOGN, W>::OCC)) void k_matvec(const ssa_desc *__restrict__ D,
                                                                      const ssa_item *__restrict__ items, T *bulk,
                                                                      const C2<T> *__restrict__ tw, int dir,
                                                                      int in_panel, int out_panel, int n_items)
 {
-    using G = FftGeom<T, LOGN, true>;
+    using G = FftGeom<T, LOGN, W>;
     constexpr int E = G::E, NT = G::NT;
     constexpr bool SPLIT = G::SPLIT;
     __shared__ T lre[G::LDS];
@@ -255,7 +255,7 @@ __global__ __launch_bounds__((FftGeom<T, LOGN, true>::NT), (FftGeom<T, LOGN, tru
                         z[e].y = (two && q < len) ? a1[q] : (T)0;
                     }
                 }
-                fft<T, LOGN, 1, SPLIT, true>(z, lre, lim, tw);
+                fft<T, LOGN, 1, SPLIT, W>(z, lre, lim, tw);
             }
             // spectra are indexed [L chunk][K chunk]
             const int lo = dir == 0 ? o : p, ko = dir == 0 ? p : o;
@@ -267,7 +267,7 @@ __global__ __launch_bounds__((FftGeom<T, LOGN, true>::NT), (FftGeom<T, LOGN, tru
                 acc[e].y += xv.x * z[e].y + xv.y * z[e].x;
             }
         }
-        fft<T, LOGN, 1, SPLIT, true>(acc, lre, lim, tw);
+        fft<T, LOGN, 1, SPLIT, W>(acc, lre, lim, tw);
         int row0, len;
         if (dir == 0) {
             const int m = o / d.AL, a = o - m * d.AL;
@@ -1427,19 +1427,30 @@ int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items,
     return (int)hipGetLastError();
 }
 
+enum { MV_WIDE_MIN_ITEMS = 2048 };
+
 int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel, const ssa_desc *D,
                    const ssa_item *items, int n_items, void *bulk, const void *tw, void *stream)
 {
     if (n_items <= 0) return 0;
     if (!lgn_ok(prec, lgn)) return -2;
     hipStream_t st = (hipStream_t)stream;
-#define LAUNCH(T, L)                                                                                          \
-    hipLaunchKernelGGL((k_matvec<T, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L, true>::NT), 0, st, D, items,   \
-                       (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel, n_items)
+    // The wide geometry (fewer LDS exchanges, half the waves) pays when the
+    // launch fills the GPU; a launch of few items is bound by each item's
+    // latency, which the narrow geometry's extra waves hide.
+    const bool wide = n_items >= MV_WIDE_MIN_ITEMS;
+#define LAUNCH(T, L, WD)                                                                                      \
+    hipLaunchKernelGGL((k_matvec<T, L, WD>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L, WD>::NT), 0, st, D,  \
+                       items, (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel, n_items)
 #define CASE(L)                                                                                               \
     case L:                                                                                                   \
-        if (prec == SSA_PREC_F64) LAUNCH(double, L);                                                          \
-        else LAUNCH(float, L);                                                                                \
+        if (prec == SSA_PREC_F64) {                                                                           \
+            if (wide) LAUNCH(double, L, true);                                                                \
+            else LAUNCH(double, L, false);                                                                    \
+        } else {                                                                                              \
+            if (wide) LAUNCH(float, L, true);                                                                 \
+            else LAUNCH(float, L, false);                                                                     \
+        }                                                                                                     \
         break;
     switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
 #undef CASE

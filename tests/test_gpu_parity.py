@@ -153,6 +153,26 @@ def test_parestimate_esprit():
         dev.parestimate([0, 7])            # index beyond the computed components
 
 
+def test_streaming_ticks_match_oracle():
+    """The reference's streaming pattern (MKL/ssa_opt_r2c.h:82-91): one
+    prepared context, update_signal on a window moved one sample per tick,
+    decompose, reconstruct the trend.  Decomposition i draws Philox counter i,
+    so tick t is the oracle's decomposition of that window with counter t."""
+    S = _ssa()
+    N, L, k, p = 1000, 250, 20, 10
+    long = workloads.single(N + 8, seed=9)
+    dev = S.SSA(long[:N], L, q=2, seed=77)
+    K = N - L + 1
+    for t in range(5):
+        w = long[t:t + N]
+        if t:
+            dev.update_signal(w)
+        dev.decompose(k, oversampling=p)
+        orc = O.OracleSSA(w, L).decompose(k, p, 2, O.philox_gaussian(77, t, K, k + p))
+        assert np.max(np.abs(dev.singular_values - orc.sigma) / orc.sigma) < 1e-10, t
+        assert np.abs(dev.reconstruct([0]) - orc.reconstruct([0])).max() / rms(w) < 1e-9, t
+
+
 def test_reconstruct_many_matches_single():
     x, dev, orc = _pair(2000, 500, 12, 10, 2, seed=3)
     groups = [[i] for i in range(12)] + [[1, 2], [0, 3, 5]]
