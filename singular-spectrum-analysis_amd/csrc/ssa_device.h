@@ -104,14 +104,16 @@ int ssa_dev_trmm(int nb, int slot_a, int slot_b, int slot_c, const ssa_desc *D, 
  * Every series of the launch has b <= bmax; instantiated per column-block count SSA_NB(bmax). */
 int ssa_dev_svd(int bmax, int r_slot, int rinv_slot, double early, const ssa_desc *D, const ssa_item *items,
                 int n_items, double *small, void *stream);
-/* reconstruction: items (s, gid, row chunk o, col chunk sk); groups: gdesc[gid] = {start, count},
- * glist = component indices; scale_sigma 1: sum_g sigma_g u_g (*) v_g, 0: u (*) v */
-int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, const ssa_desc *D, const ssa_item *items,
-                  int n_items, const int *gdesc, const int *glist, void *bulk, const double *small,
-                  const void *tw, void *stream);
+/* reconstruction: items (s, gid * S + split, row chunk o, col chunk sk); groups: gdesc[gid] =
+ * {start, count}, glist = component indices; scale_sigma 1: sum_g sigma_g u_g (*) v_g, 0: u (*) v.
+ * S = min(nsplit, staging slots of the series): split j sums its share of the group's components
+ * into its own staged piece; ssa_dev_olap with the same nsplit sums the pieces. */
+int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, int nsplit, const ssa_desc *D,
+                  const ssa_item *items, int n_items, const int *gdesc, const int *glist, void *bulk,
+                  const double *small, const void *tw, void *stream);
 /* overlap-add of the staging pieces: items (s, gid, t tile); weight 0: w_t, 1: sqrt(w_t);
  * channel index taken from gdesc[gid].z.  Output of gid goes to out + gid*N. */
-int ssa_dev_olap(int prec, int weight, const ssa_desc *D, const ssa_item *items, int n_items,
+int ssa_dev_olap(int prec, int weight, int nsplit, const ssa_desc *D, const ssa_item *items, int n_items,
                  const int *gdesc, void *bulk, void *stream);
 /* W-correlation: Gram of the sqrt(w)-weighted component panel (stored at off_out, N x k)
  * then normalisation; W written row-major k x k to small + off_small + SLOT_W */

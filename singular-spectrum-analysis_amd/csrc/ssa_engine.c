@@ -221,6 +221,8 @@ static const ssa_item *it_ptr(const ssa_engine *e, span sp) { return e->ditems +
  * large batch, short ranges for one series with many output chunks (MSSA),
  * where a few long items would leave most CUs idle. */
 enum { MV_MIN_ITEMS = 1024 };
+/* component-loop splits of the batched step's reconstruction items */
+enum { RECON_SPLIT = 2 };
 static int build_matvec(ssa_engine *e, int dir, int lgn, span *sp)
 {
     long long units = 0;   /* (pair, output chunk) units of the launch */
@@ -286,14 +288,18 @@ static int rebuild_lists(ssa_engine *e)
             if (e->hd[s].b > 0 && SSA_NB(e->hd[s].b) == nb && items_push(e, s, 0, 0, 0)) return -1;
         e->ser_nb[nb].n = (int)(e->hitems_n - e->ser_nb[nb].off);
     }
-    /* batched step: group gid = s covering components 0..k-1 */
+    /* batched step: group 0 covering components 0..k-1, its component loop
+     * split RECON_SPLIT ways (as many as the staging slots allow) */
     for (int lg = 0; lg < NLG; lg++) {
         e->rec_step[lg].off = e->hitems_n;
         for (int s = 0; s < e->count; s++) {
             const ssa_desc *d = &e->hd[s];
             if (d->lgn != lg || d->k <= 0) continue;
-            for (int a = 0; a < d->AL; a++)
-                for (int sk = 0; sk < d->SK; sk++) if (items_push(e, s, 0, a, sk)) return -1;
+            const long long slots = d->stage_cap / ((long long)d->AL * d->SK * d->n);
+            const int S = (int)(slots < RECON_SPLIT ? (slots < 1 ? 1 : slots) : RECON_SPLIT);
+            for (int j = 0; j < S; j++)
+                for (int a = 0; a < d->AL; a++)
+                    for (int sk = 0; sk < d->SK; sk++) if (items_push(e, s, j, a, sk)) return -1;
         }
         e->rec_step[lg].n = (int)(e->hitems_n - e->rec_step[lg].off);
     }
@@ -934,9 +940,9 @@ static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weigh
         for (int t = 0; t < ceil_div(d->N, OLAP_TILE); t++) DEVCHK(items_push(e, s, g, t, 0));
     ol.n = (int)(e->hitems_n - ol.off);
     DEVCHK(items_upload(e, e->dyn_base));
-    DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, 0, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
+    DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, 0, 1, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
                          e->dgroups + 4 * ng, e->arena_a, e->small, e->tw, e->st));
-    DEVCHK(ssa_dev_olap(e->prec, weight, e->dd, it_ptr(e, ol), ol.n, e->dgroups, e->arena_a, e->st));
+    DEVCHK(ssa_dev_olap(e->prec, weight, 1, e->dd, it_ptr(e, ol), ol.n, e->dgroups, e->arena_a, e->st));
     return 0;
 }
 
@@ -1140,12 +1146,14 @@ static int launch_step(ssa_engine *e, int q)
     for (int lg = 6; lg < NLG; lg++) {
         if (!e->rec_step[lg].n) continue;
         prof_begin(e, PH_RECON, lg);
-        DEVCHK(ssa_dev_recon(e->prec, lg, 1, 1, e->dd, it_ptr(e, e->rec_step[lg]), e->rec_step[lg].n, e->dgroups,
+        DEVCHK(ssa_dev_recon(e->prec, lg, 1, 1, RECON_SPLIT, e->dd, it_ptr(e, e->rec_step[lg]), e->rec_step[lg].n,
+                             e->dgroups,
                              e->dgroups + 4, e->arena_a, e->small, e->tw, e->st));
         prof_end(e);
     }
     prof_begin(e, PH_OLAP, 0);
-    DEVCHK(ssa_dev_olap(e->prec, 0, e->dd, it_ptr(e, e->olap_step), e->olap_step.n, e->dgroups, e->arena_a, e->st));
+    DEVCHK(ssa_dev_olap(e->prec, 0, RECON_SPLIT, e->dd, it_ptr(e, e->olap_step), e->olap_step.n, e->dgroups,
+                        e->arena_a, e->st));
     prof_end(e);
     if (e->step_h > 0) {
         prof_begin(e, PH_FORECAST, 0);

@@ -1151,6 +1151,14 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     }
 }
 
+// splits of a reconstruction item: at most nsplit, and no more than the staging
+// area has group slots for (stage_cap pieces of AL * SK * n)
+__device__ __forceinline__ int recon_splits(const ssa_desc &d, int nsplit)
+{
+    const long long slots = d.stage_cap / ((long long)d.AL * d.SK * d.n);
+    return (int)max(1LL, min((long long)nsplit, slots));
+}
+
 // ---------------------------------------------------------------------------
 // reconstruction pieces.  Item (s, gid, a, sk): the convolution of the row
 // chunk a of channel m = gdesc[gid].z (U rows) with the column chunk sk
@@ -1166,7 +1174,7 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) vo
                                                               const int4 *__restrict__ gdesc,
                                                               const int *__restrict__ glist,
                                                               const C2<T> *__restrict__ tw, int scale_sigma,
-                                                              int limit_gsize, int n_items)
+                                                              int limit_gsize, int nsplit, int n_items)
 {
     using G = FftGeom<T, LOGN>;
     constexpr int E = G::E, NT = G::NT;
@@ -1177,15 +1185,27 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) vo
     if (item < 0) return;
     const ssa_item it = items[item];
     const ssa_desc d = D[it.s];
-    const int4 gd = gdesc[it.x];
+    // item.x = group * S + split: split j of S sums components [ce j / S,
+    // ce (j+1) / S) of the group's list into its own staged piece (the
+    // overlap-add sums the pieces); S = 1 sums the whole list
+    const int S = recon_splits(d, nsplit);
+    const int4 gd = gdesc[it.x / S];
     const int a = it.y, sk = it.z, m = gd.z;
+    int g0 = 0, g1 = gd.y;
+    if (S > 1) {
+        int ce = min(gd.y, d.k);
+        if (limit_gsize) ce = min(ce, d.gsize);
+        const int j = it.x % S;
+        g0 = ce * j / S;
+        g1 = ce * (j + 1) / S;
+    }
     const int urow0 = m * d.L + a * d.CL, ulen = min(d.CL, d.L - a * d.CL);
     const int vrow0 = sk * d.CK, vlen = min(d.CK, d.K - sk * d.CK);
     const double *sig = small + d.off_small + (long long)SSA_SLOT_MISC * SSA_BMAX * SSA_BMAX + SSA_MISC_SIGMA;
     C2<T> acc[E], z[E];
 #pragma unroll
     for (int e = 0; e < E; e++) acc[e] = C2<T>{0, 0};
-    for (int g = 0; g < gd.y; g++) {
+    for (int g = g0; g < g1; g++) {
         const int idx = glist[gd.x + g];
         if (idx < 0 || idx >= d.k || (limit_gsize && idx >= d.gsize)) continue;
         const T sc = (T)(scale_sigma ? sqrt(fabs(sig[idx])) : 1.0);
@@ -1219,21 +1239,24 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) vo
 // item (s, gid, t tile of 1024); weight 0: 1/c_t, 1: sqrt(1/c_t)
 template <typename T>
 __global__ __launch_bounds__(256) void k_olap(const ssa_desc *D, const ssa_item *items, const int4 *gdesc, T *bulk,
-                                              int weight)
+                                              int weight, int nsplit)
 {
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
-    const int n = d.n;
+    const int n = d.n, S = recon_splits(d, nsplit);
     for (int t = it.y * 1024 + threadIdx.x; t < min(d.N, (it.y + 1) * 1024); t += 256) {
         double s = 0;
-        for (int a = 0; a < d.AL; a++) {
-            const int ulen = min(d.CL, d.L - a * d.CL);
-            for (int sk = 0; sk < d.SK; sk++) {
-                const int off = a * d.CL + sk * d.CK;
-                const int vlen = min(d.CK, d.K - sk * d.CK);
-                const int q = t - off;
-                if (q >= 0 && q < ulen + vlen - 1)
-                    s += (double)bulk[d.off_stage + (((long long)it.x * d.AL + a) * d.SK + sk) * n + q];
+        for (int j = 0; j < S; j++) {
+            const long long piece = (long long)it.x * S + j;
+            for (int a = 0; a < d.AL; a++) {
+                const int ulen = min(d.CL, d.L - a * d.CL);
+                for (int sk = 0; sk < d.SK; sk++) {
+                    const int off = a * d.CL + sk * d.CK;
+                    const int vlen = min(d.CK, d.K - sk * d.CK);
+                    const int q = t - off;
+                    if (q >= 0 && q < ulen + vlen - 1)
+                        s += (double)bulk[d.off_stage + ((piece * d.AL + a) * d.SK + sk) * n + q];
+                }
             }
         }
         const int c = min(min(t + 1, d.L), min(d.K, d.N - t));
@@ -1603,9 +1626,9 @@ int ssa_dev_svd(int bmax, int r_slot, int rinv_slot, double early, const ssa_des
     return (int)hipGetLastError();
 }
 
-int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, const ssa_desc *D, const ssa_item *items,
-                  int n_items, const int *gdesc, const int *glist, void *bulk, const double *small, const void *tw,
-                  void *stream)
+int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, int nsplit, const ssa_desc *D,
+                  const ssa_item *items, int n_items, const int *gdesc, const int *glist, void *bulk,
+                  const double *small, const void *tw, void *stream)
 {
     if (n_items <= 0) return 0;
     if (!lgn_ok(prec, lgn)) return -2;
@@ -1615,27 +1638,27 @@ int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, const ssa
         if (prec == SSA_PREC_F64)                                                                             \
             hipLaunchKernelGGL((k_recon<double, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<double, L>::NT), 0, \
                                st, D, items, (double *)bulk, small, (const int4 *)gdesc, glist,               \
-                               (const C2<double> *)tw, scale_sigma, limit_gsize, n_items);                    \
+                               (const C2<double> *)tw, scale_sigma, limit_gsize, nsplit, n_items);                    \
         else                                                                                                  \
             hipLaunchKernelGGL((k_recon<float, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<float, L>::NT), 0,   \
                                st, D, items, (float *)bulk, small, (const int4 *)gdesc, glist,                \
-                               (const C2<float> *)tw, scale_sigma, limit_gsize, n_items);                     \
+                               (const C2<float> *)tw, scale_sigma, limit_gsize, nsplit, n_items);                    \
         break;
     switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
 #undef CASE
     return (int)hipGetLastError();
 }
 
-int ssa_dev_olap(int prec, int weight, const ssa_desc *D, const ssa_item *items, int n_items, const int *gdesc,
-                 void *bulk, void *stream)
+int ssa_dev_olap(int prec, int weight, int nsplit, const ssa_desc *D, const ssa_item *items, int n_items,
+                 const int *gdesc, void *bulk, void *stream)
 {
     if (n_items <= 0) return 0;
     if (prec == SSA_PREC_F64)
         hipLaunchKernelGGL(k_olap<double>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
-                           (const int4 *)gdesc, (double *)bulk, weight);
+                           (const int4 *)gdesc, (double *)bulk, weight, nsplit);
     else
         hipLaunchKernelGGL(k_olap<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
-                           (const int4 *)gdesc, (float *)bulk, weight);
+                           (const int4 *)gdesc, (float *)bulk, weight, nsplit);
     return (int)hipGetLastError();
 }
 
