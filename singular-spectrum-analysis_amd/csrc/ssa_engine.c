@@ -222,7 +222,7 @@ static const ssa_item *it_ptr(const ssa_engine *e, span sp) { return e->ditems +
  * where a few long items would leave most CUs idle. */
 enum { MV_MIN_ITEMS = 1024 };
 /* component-loop splits of the batched step's reconstruction items */
-enum { RECON_SPLIT = 2 };
+enum { RECON_SPLIT = 3 };
 static int build_matvec(ssa_engine *e, int dir, int lgn, span *sp)
 {
     long long units = 0;   /* (pair, output chunk) units of the launch */
