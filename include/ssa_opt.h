@@ -272,7 +272,12 @@ int ssa_amd_batch_output(ssa_amd_batch *b, int i, ssa_real *out);         /* N_i
 /* every series' output at out + offsets[i] (N_i (+ h) values each) and, when
  * status is not NULL, its device status word; one synchronisation per branch */
 int ssa_amd_batch_outputs(ssa_amd_batch *b, ssa_real *out, const long long *offsets, int *status);
+/* device bytes the batch's arenas use (the sum of ssa_amd_series_bytes over its series) */
 long long ssa_amd_batch_bytes(const ssa_amd_batch *b);
+/* No counterpart in the reference: device bytes one series (N, L, k) takes in a batch of
+ * oversampling p, q power iterations, precision (0 plain, 1 mixed) and horizon h; host
+ * arithmetic only.  For callers that admit work against a device-memory budget (serve.py). */
+long long ssa_amd_series_bytes(int N, int L, int k, int p, int q, int precision, int h);
 /* one eager step with HIP events around every kernel launch: per launch the
  * phase (0 spectra, 1 matvec H, 2 matvec H^T, 3 Gram, 4 Cholesky, 5 apply,
  * 6 small SVD, 7 U/V, 8 reconstruction, 9 overlap-add, 10 forecast, 11 test

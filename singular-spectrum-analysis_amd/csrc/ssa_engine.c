@@ -530,6 +530,32 @@ int ssa_engine_spectra(ssa_engine *e)
 /* panels                                                                    */
 /* ------------------------------------------------------------------------ */
 
+/* Device bytes one series of (N, L, b, k) takes in an engine of precision
+ * prec prepared for max_groups groups and horizon max_h: the signal arena
+ * (x and segment spectra), the panel arena (Y, Z, U, V, staging, outputs) and
+ * the FP64 small arena, with the planner's chunking and the layout of
+ * ssa_engine_create / ssa_engine_prepare (host arithmetic only). */
+long long ssa_engine_series_bytes(int prec, int N, int L, int b, int k, int max_groups, int max_h, int q)
+{
+    if (N < 4 || L < 2 || L > N - 1 || b < 2 || k < 1) return -1;
+    ssa_plan pl;
+    if (ssa_plan_series(prec, N, L, 1, b, k, q, -1, &pl)) return -1;
+    const long long esz = prec == SSA_PREC_F64 ? 8 : 4;
+    const long long K = N - L + 1, ML = L;
+    const int gmax = max_groups > k ? max_groups : k;
+    long long a = align64(N) + align64((long long)pl.AL * pl.SK * 2 * pl.n);
+    long long bulk = align64(ML * b) + align64(K * b) + align64(ML * k) + align64(K * k);
+    bulk += align64((long long)gmax * pl.AL * pl.SK * pl.n);
+    bulk += align64((long long)gmax * N + max_h + 64);
+    int tilesY = ceil_div((int)ML, SSA_GRAM_ROWS), tilesZ = ceil_div((int)K, SSA_GRAM_ROWS);
+    int tiles = tilesY > tilesZ ? tilesY : tilesZ, tout = ceil_div(N, SSA_GRAM_ROWS);
+    if (tout > tiles) tiles = tout;
+    long long part = (long long)tiles * (b * b + b), need = 2LL * L + max_h + 256;
+    if (part < need) part = need;
+    const long long small = align64(part) + align64((long long)SSA_NSLOTS * SSA_BMAX * SSA_BMAX);
+    return esz * (a + bulk) + 8 * small;
+}
+
 int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
 {
     if (max_groups < 1) max_groups = 1;

@@ -25,6 +25,8 @@ typedef struct { int n, lgn, CL, CK, AL, SK; double cost; } ssa_plan;
  * for block width b, k components and q power iterations.  When lgn_force
  * is in [6, 13] only that length is considered. */
 int ssa_plan_series(int prec, int N, int L, int M, int b, int k, int q, int lgn_force, ssa_plan *out);
+/* device bytes of one series in an engine (signal, panel and small arenas) */
+long long ssa_engine_series_bytes(int prec, int N, int L, int b, int k, int max_groups, int max_h, int q);
 
 /* measure the device cost of one transform at every length 2^6..2^13 for
  * precision prec and plan every later engine of that precision with it;

@@ -1376,6 +1376,20 @@ int ssa_amd_batch_profile(ssa_amd_batch *b, int max_rec, int *phase, int *lgn, f
     return n;
 }
 
+/* No counterpart in the reference: device bytes one series (N, L, k) takes
+ * in a batch of oversampling p, q power iterations and horizon h (the layout
+ * the batch's engines use; a batch's ssa_amd_batch_bytes is the sum over its
+ * series).  Host arithmetic only, for admission against a device budget. */
+long long ssa_amd_series_bytes(int N, int L, int k, int p, int q, int precision, int h)
+{
+    if (N < 4 || L < 2 || L > N - 1 || k < 1 || q < 0 || h < 0) return -1;
+    if (p <= 0) p = 8;
+    const int r = imin(L, N - L + 1);
+    const int kp = imin(k + p, r), bm = imax(kp, 2), km = imin(k, kp);
+    if (bm > SSA_BMAX) return -1;
+    return ssa_engine_series_bytes(precision ? SSA_PREC_F32 : SSA_PREC_F64, N, L, bm, km, 1, h, q);
+}
+
 int ssa_amd_batch_plan(const ssa_amd_batch *b, int i, int *out8)
 {
     if (!b || i < 0 || i >= b->count || !out8) return -1;

@@ -7,8 +7,11 @@ iterations) + reconstruct the group + R-forecast h steps (`ssa_opt_forecast`,
 MKL/ssa_opt.h:2843-2867, on the group's reconstruction).
 
 The server gathers the requests that have arrived into a wave (at most
-`max_wave` series), runs the wave as one batched step on the device (one
-`ssa_amd_batch`, mixed or plain form) and hands every request its output.
+`max_wave` series and, with `hbm_budget`, at most hbm_budget / inflight
+device bytes by `ssa_amd.series_bytes`), runs the wave as one batched step on
+the device (one `ssa_amd_batch`, mixed or plain form) and hands every request
+its output.  A request larger than a slot's share runs alone; one larger than
+the whole budget is refused (status -3, no output).
 Up to `inflight` waves are outstanding: while one runs on the GPU the next is
 assembled and uploaded.  The server keeps one batch per in-flight slot and
 re-targets it to each new wave (`ssa_amd_batch_rebind`), so once the slots
@@ -37,6 +40,7 @@ class Request:
     done: float = field(default=-1.0, compare=False)
     out: np.ndarray = field(default=None, compare=False)
     status: int = field(default=0, compare=False)
+    nbytes: int = field(default=0, compare=False)
 
 
 def request_key(rid, seed=6):
@@ -47,20 +51,54 @@ class Server:
     """Waves of requests through `ssa_amd.Batch` (all requests of a wave share
     h, p, q and the precision)."""
 
-    def __init__(self, p=10, q=2, h=50, precision=1, max_wave=512, inflight=2, seed=6):
+    def __init__(self, p=10, q=2, h=50, precision=1, max_wave=512, inflight=2, seed=6, hbm_budget=None):
         self.p, self.q, self.h, self.precision = p, q, h, precision
         self.max_wave, self.inflight, self.seed = max_wave, inflight, seed
+        self.hbm_budget = hbm_budget
         self.waves = []          # (batch, requests) in flight
         self.idle = []           # batches ready for the next wave
         self.wave_sizes = []
+        self.wave_bytes = []     # device bytes of each wave's series
+
+    def footprint(self, r):
+        if not r.nbytes:
+            r.nbytes = ssa_amd.series_bytes(len(r.x), r.L, r.k, self.p, self.q, self.precision, self.h)
+        return r.nbytes
+
+    def next_wave(self, waiting):
+        """The longest prefix of `waiting` (arrival order) that fits a wave:
+        (wave, rest, refused)."""
+        if self.hbm_budget is None:
+            return waiting[:self.max_wave], waiting[self.max_wave:], []
+        share = self.hbm_budget / self.inflight
+        wave, used, refused = [], 0, []
+        i = 0
+        while i < len(waiting) and len(wave) < self.max_wave:
+            r = waiting[i]
+            need = self.footprint(r)
+            if need > self.hbm_budget:
+                refused.append(r)
+            elif not wave or used + need <= share:
+                wave.append(r)
+                used += need
+                if used > share:       # a request above the share runs alone
+                    i += 1
+                    break
+            else:
+                break
+            i += 1
+        return wave, waiting[i:], refused
 
     def warm(self, requests):
-        """Grow every in-flight slot to hold `requests` as one wave (device
-        arenas, pinned staging), outside any timed run."""
+        """Grow every in-flight slot to hold the first wave `requests` would
+        form (device arenas, pinned staging), outside any timed run."""
+        wave, _, _ = self.next_wave(list(requests))
         for _ in range(self.inflight):
-            self._launch(list(requests))
+            self._launch(list(wave))
         while self.waves:
             self._retire(block=True)
+        self.wave_sizes.clear()
+        self.wave_bytes.clear()
 
     def _launch(self, reqs):
         xs, Ls, ks = [r.x for r in reqs], [r.L for r in reqs], [r.k for r in reqs]
@@ -74,6 +112,7 @@ class Server:
         b.run(use_graph=False)
         self.waves.append((b, reqs))
         self.wave_sizes.append(len(reqs))
+        self.wave_bytes.append(sum(self.footprint(r) for r in reqs) if self.hbm_budget is not None else 0)
 
     def _retire(self, block):
         b, reqs = self.waves[0]
@@ -103,9 +142,13 @@ class Server:
             while pending and pending[0].arrival <= now:
                 waiting.append(heapq.heappop(pending))
             if waiting and len(self.waves) < self.inflight:
-                wave, waiting = waiting[:self.max_wave], waiting[self.max_wave:]
-                self._launch(wave)
-                finished.extend(wave)
+                wave, waiting, refused = self.next_wave(waiting)
+                for r in refused:
+                    r.status, r.done = -3, time.perf_counter()
+                finished.extend(refused)
+                if wave:
+                    self._launch(wave)
+                    finished.extend(wave)
                 continue
             if self.waves and self._retire(block=False):
                 continue
@@ -136,6 +179,8 @@ def synthetic_requests(count, rate, n_lo=500, n_hi=50000, k_lo=10, k_hi=50, h=50
 
 
 def latency_stats(reqs):
+    """Latency percentiles over the served requests (refused ones excluded)."""
+    reqs = [r for r in reqs if r.status != -3]
     lat = np.array([r.done - r.arrival for r in reqs])
     return {"p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
             "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),

@@ -123,6 +123,7 @@ SIGNATURES = [
     ("ssa_amd_batch_sigma", _I, [_VP, _I, c_double_p]),
     ("ssa_amd_batch_output", _I, [_VP, _I, c_double_p]),
     ("ssa_amd_batch_bytes", ctypes.c_longlong, [_VP]),
+    ("ssa_amd_series_bytes", ctypes.c_longlong, [_I, _I, _I, _I, _I, _I, _I]),
     ("ssa_amd_batch_free", None, [_VP]),
     ("ssa_amd_batch_profile", _I, [_VP, _I, c_int_p, c_int_p, c_float_p]),
     ("ssa_amd_batch_create_ex", _VP, [_I, c_int_p, c_int_p, c_int_p, c_int_p, _I, _I, _I, _I, _U64]),
@@ -184,6 +185,15 @@ def cadzow(x, L, rank, max_iter=20, tol=1e-9, alpha=1.0):
     if it < 0:
         raise RuntimeError("Cadzow iterations failed")
     return out, r.iterations, r.final_diff, r.converged > 0.5
+
+
+def series_bytes(N, L, k, p=10, q=2, precision=1, h=0):
+    """Device bytes one series takes in a batch (host arithmetic; no device
+    call): what a caller admits against a device-memory budget."""
+    v = lib().ssa_amd_series_bytes(int(N), int(L), int(k), int(p), int(q), int(precision), int(h))
+    if v < 0:
+        raise ValueError("invalid series dimensions")
+    return int(v)
 
 
 def esprit_poles(U):

@@ -65,3 +65,33 @@ def test_rebind_matches_fresh_batch():
         assert np.array_equal(oa[i], of[i]), i
         assert np.array_equal(a.sigma(i), fresh.sigma(i)), i
     assert np.array_equal(sa, sf)
+
+
+def test_series_bytes_match_the_batch():
+    """ssa_amd.series_bytes (host arithmetic) is exactly what a batch's arenas
+    use, series by series."""
+    import ssa_amd as S
+    import workloads
+    xs, Ns, Ls, ks = workloads.ragged_batch(count=12, n_lo=500, n_hi=20000, seed=61, dtype=np.float32)
+    for prec in (0, 1):
+        b = S.Batch(xs, Ls, ks, p=10, q=2, precision=prec, seed=1, h=50)
+        est = sum(S.series_bytes(len(x), int(L), int(k), 10, 2, prec, 50) for x, L, k in zip(xs, Ls, ks))
+        assert b.bytes() == est, prec
+
+
+def test_budgeted_serving_matches_unbudgeted():
+    """Waves cut to a device-memory budget give every request the result it
+    gets in one unbounded wave (bit for bit), and stay within the share."""
+    S = _serve()
+    reqs = S.synthetic_requests(16, rate=np.inf, n_lo=500, n_hi=8000, seed=33)
+    free = {r.rid: r for r in S.Server(max_wave=16, inflight=1).run(reqs)}
+    sv = S.Server(max_wave=16, inflight=2)
+    total = sum(sv.footprint(r) for r in S.synthetic_requests(16, rate=np.inf, n_lo=500, n_hi=8000, seed=33))
+    sv = S.Server(max_wave=16, inflight=2, hbm_budget=total / 2)
+    capped = sv.run(S.synthetic_requests(16, rate=np.inf, n_lo=500, n_hi=8000, seed=33))
+    assert len(sv.wave_sizes) >= 3 and sum(sv.wave_sizes) == 16
+    for wb, ws in zip(sv.wave_bytes, sv.wave_sizes):
+        assert wb <= total / 4 or ws == 1
+    for r in capped:
+        assert r.status == free[r.rid].status == 0
+        assert np.array_equal(r.out, free[r.rid].out), r.rid
