@@ -165,14 +165,18 @@ def cpu_baseline(xs, Ls, ks, budget_s, workers):
             "value_1_core": round(one, 2)}
 
 
+SERVE_BUDGET_GIB = 16
+
+
 def serving(seconds):
     """cfg5: requests with N log-uniform in [500, 50000], L = N // 4,
     k ~ U{10..50}, reconstruct + forecast h = 50 of components
     0..floor(2k/5)-1, mixed form.  Capacity from a saturated backlog; then an
     open-loop Poisson stream at 70 % of it for `seconds`; then the same kind
-    of request served alone."""
+    of request served alone.  Waves are admitted against a device-memory
+    budget of SERVE_BUDGET_GIB (half per in-flight slot)."""
     import serve
-    srv = serve.Server(max_wave=512, inflight=2)
+    srv = serve.Server(max_wave=512, inflight=2, hbm_budget=SERVE_BUDGET_GIB * 2 ** 30)
     # grow the in-flight slots to the largest wave before anything is timed
     srv.warm(sorted(serve.synthetic_requests(4096, rate=np.inf, seed=63), key=lambda r: -len(r.x))[:512])
     done = srv.run(serve.synthetic_requests(2048, rate=np.inf, seed=6))
@@ -188,6 +192,7 @@ def serving(seconds):
     mix, solo = serve.latency_stats(stream), serve.latency_stats(alone)
     return {"workload": "N log-uniform in [500, 50000], L = N // 4, k ~ U{10..50}, group 0..floor(2k/5)-1, "
                         "h = 50, Poisson arrivals, mixed form",
+            "hbm_budget_GiB": SERVE_BUDGET_GIB, "max_wave_GB": round(max(srv.wave_bytes) / 1e9, 2),
             "capacity_req_s": round(capacity, 1), "rate_req_s": round(rate, 1),
             "in_mix": mix, "alone": solo,
             "p50_in_mix_over_alone": round(mix["p50_ms"] / solo["p50_ms"], 3)}

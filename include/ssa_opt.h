@@ -210,6 +210,17 @@ void ssa_opt_parestimate_free(SSA_ParEstimate *result);
  * eigenvectors themselves, and for host-only checks. */
 int ssa_amd_esprit_poles(const ssa_real *U, int L, int k, ssa_real *wr, ssa_real *wi);
 
+/* ---- MKL configuration hooks (MKL/py/ssa_wrapper.c:47-124) ----
+ * Exported for the reference's benches; the device path has no MKL, so they do
+ * nothing (init returns 0, get_threads 1, get_cpu_info reports the online cores). */
+int ssa_mkl_init(int verbose);
+int ssa_mkl_init_14900kf(int verbose);
+int ssa_mkl_init_generic(int verbose);
+void ssa_mkl_set_threads(int n);
+int ssa_mkl_get_threads(void);
+void ssa_mkl_get_cpu_info(int *p_cores, int *e_cores, int *is_hybrid, int *has_avx512);
+void ssa_mkl_disable_affinity(void);
+
 /* ---- MSSA (MKL/ssa_opt.h:3881-4283) ---- */
 int mssa_opt_init(MSSA_Opt *mssa, const ssa_real *X, int M, int N, int L);
 int mssa_opt_decompose(MSSA_Opt *mssa, int k, int oversampling);

@@ -8,12 +8,14 @@
  * bookkeeping the reference also does on the host (getters, statistics,
  * LRF coefficients handed back to the caller).
  */
+#define _DEFAULT_SOURCE   /* sysconf(_SC_NPROCESSORS_ONLN) */
 #include "../../include/ssa_opt.h"
 
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "ssa_engine.h"
 
@@ -922,6 +924,28 @@ out:
     free(gap); free(group); free(fl); free(fr); free(rev); free(tmp);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* MKL configuration hooks (MKL/py/ssa_wrapper.c:47-124)                      */
+/* ------------------------------------------------------------------------ */
+/* The reference's library exports these for its benches, which call them
+ * inside try/except.  The device path has no MKL: there is nothing to
+ * configure, the calls succeed and do nothing, and the CPU query reports the
+ * host's online cores as performance cores. */
+int ssa_mkl_init(int verbose) { (void)verbose; return 0; }
+int ssa_mkl_init_14900kf(int verbose) { (void)verbose; return 0; }
+int ssa_mkl_init_generic(int verbose) { (void)verbose; return 0; }
+void ssa_mkl_set_threads(int n) { (void)n; }
+int ssa_mkl_get_threads(void) { return 1; }
+void ssa_mkl_get_cpu_info(int *p_cores, int *e_cores, int *is_hybrid, int *has_avx512)
+{
+    long n = sysconf(_SC_NPROCESSORS_ONLN);
+    if (p_cores) *p_cores = n > 0 ? (int)n : 1;
+    if (e_cores) *e_cores = 0;
+    if (is_hybrid) *is_hybrid = 0;
+    if (has_avx512) *has_avx512 = 0;
+}
+void ssa_mkl_disable_affinity(void) {}
 
 /* ------------------------------------------------------------------------ */
 /* extensions                                                                */

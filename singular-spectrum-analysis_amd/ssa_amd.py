@@ -124,6 +124,13 @@ SIGNATURES = [
     ("ssa_amd_batch_output", _I, [_VP, _I, c_double_p]),
     ("ssa_amd_batch_bytes", ctypes.c_longlong, [_VP]),
     ("ssa_amd_series_bytes", ctypes.c_longlong, [_I, _I, _I, _I, _I, _I, _I]),
+    ("ssa_mkl_init", _I, [_I]),
+    ("ssa_mkl_init_14900kf", _I, [_I]),
+    ("ssa_mkl_init_generic", _I, [_I]),
+    ("ssa_mkl_set_threads", None, [_I]),
+    ("ssa_mkl_get_threads", _I, []),
+    ("ssa_mkl_get_cpu_info", None, [c_int_p, c_int_p, c_int_p, c_int_p]),
+    ("ssa_mkl_disable_affinity", None, []),
     ("ssa_amd_batch_free", None, [_VP]),
     ("ssa_amd_batch_profile", _I, [_VP, _I, c_int_p, c_int_p, c_float_p]),
     ("ssa_amd_batch_create_ex", _VP, [_I, c_int_p, c_int_p, c_int_p, c_int_p, _I, _I, _I, _I, _U64]),

@@ -56,10 +56,22 @@ def test_exports_every_declared_symbol(lib):
 
 # MKL/py/ssa_wrapper.py:156-168: bound only when present; SSA.vforecast needs them
 WRAPPER_VFORECAST = ["ssa_opt_vforecast", "ssa_opt_vforecast_full", "ssa_opt_vforecast_fast"]
+# MKL/py/ssa_wrapper.py:209-221, :259-266, :236-246 (optional families) and the
+# MKL hooks the reference's benches call (MKL/py/benchmark_vs_rssa.py:33-43)
+WRAPPER_OPTIONAL = ["ssa_opt_cadzow", "ssa_opt_cadzow_weighted", "ssa_opt_cadzow_inplace", "ssa_opt_gapfill",
+                    "ssa_opt_gapfill_simple", "ssa_opt_parestimate", "ssa_opt_parestimate_free", "ssa_mkl_init"]
+
+
+def test_mkl_hooks_are_harmless(lib):
+    assert lib.ssa_mkl_init(1) == 0
+    assert lib.ssa_mkl_get_threads() == 1
+    p = ctypes.c_int(-1)
+    lib.ssa_mkl_get_cpu_info(ctypes.byref(p), None, None, None)
+    assert p.value >= 1
 
 
 def test_wrapper_manifest(lib):
-    for n in WRAPPER_UNCONDITIONAL + WRAPPER_GETTERS + WRAPPER_VFORECAST:
+    for n in WRAPPER_UNCONDITIONAL + WRAPPER_GETTERS + WRAPPER_VFORECAST + WRAPPER_OPTIONAL:
         assert hasattr(lib, n), n
     assert len(WRAPPER_UNCONDITIONAL) == 24
 
