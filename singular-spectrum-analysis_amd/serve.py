@@ -15,9 +15,13 @@ the whole budget is refused (status -3, no output).
 Up to `inflight` waves are outstanding: while one runs on the GPU the next is
 assembled and uploaded.  The server keeps one batch per in-flight slot and
 re-targets it to each new wave (`ssa_amd_batch_rebind`), so once the slots
-have grown to the largest wave no device memory is allocated or freed.  Each request carries its own Philox key (its id), so
-its test matrix, and hence its result, does not depend on the wave it lands
-in.
+have grown to the largest wave no device memory is allocated or freed.  Each
+request carries its own Philox key (its id), so its test matrix, and hence its
+result, does not depend on the wave it lands in.
+
+`Server.run` serves an in-process list of requests; `Server.serve_stream`
+serves request frames of the binary format in `wire.py` read from a stream
+(a socket, a pipe) and answers each with a response frame.
 """
 import heapq
 import time
@@ -160,6 +164,93 @@ class Server:
         for r in finished:
             r.done -= t0
         return finished
+
+
+    def serve_stream(self, rfile, wfile):
+        """Serve request frames read from the binary stream `rfile` until it
+        ends, writing one response frame per request to `wfile` as its wave
+        retires (completion order; the request id pairs them).  Frames are
+        read on a thread of their own, so requests keep arriving while waves
+        run.  Returns (served requests, WireError or None): a frame that cannot
+        be parsed ends the reading, and the requests read before it are still
+        answered."""
+        import queue
+        import threading
+
+        import wire
+        inbox = queue.Queue()
+        state = {"error": None}
+        t0 = time.perf_counter()
+
+        def reader():
+            try:
+                while True:
+                    f = wire.read_request(rfile)
+                    if f is None:
+                        break
+                    rid, x, L, k, g, h = f
+                    why = wire.check_fields(len(x), L, k, g, h, self.p)
+                    if why is None and h != self.h:
+                        why = f"h = {h}, this server forecasts {self.h}"
+                    r = Request(time.perf_counter() - t0, rid, x, L, k, g, h)
+                    if why is not None:
+                        r.status = wire.STATUS_REJECTED
+                    inbox.put(r)
+            except wire.WireError as exc:
+                state["error"] = exc
+            inbox.put(None)
+
+        th = threading.Thread(target=reader, daemon=True)
+        th.start()
+        waiting, finished, reading = [], [], True
+
+        def answer(rs):
+            for r in rs:
+                # a device status word other than 0 (2: verticality >= 1, no forecast) is a failure
+                st = r.status if r.status in (wire.STATUS_OK, wire.STATUS_REFUSED, wire.STATUS_REJECTED) \
+                    else wire.STATUS_FAILED
+                wfile.write(wire.encode_response(r.rid, st, r.out))
+            wfile.flush()
+
+        while reading or waiting or self.waves:
+            try:
+                while True:
+                    r = inbox.get(block=not (waiting or self.waves), timeout=None)
+                    if r is None:
+                        reading = False
+                        break
+                    if r.status == wire.STATUS_REJECTED:
+                        r.done = time.perf_counter() - t0
+                        answer([r])
+                        finished.append(r)
+                    else:
+                        waiting.append(r)
+                    if not reading or inbox.empty():
+                        break
+            except queue.Empty:
+                pass
+            if waiting and len(self.waves) < self.inflight:
+                wave, waiting, refused = self.next_wave(waiting)
+                for r in refused:
+                    r.status, r.done = wire.STATUS_REFUSED, time.perf_counter() - t0
+                answer(refused)
+                finished.extend(refused)
+                if wave:
+                    self._launch(wave)
+                continue
+            if self.waves:
+                reqs = self.waves[0][1]
+                # block on the oldest wave when no slot is free or nothing more can arrive;
+                # otherwise poll, so that a new arrival takes the free slot at once
+                if self._retire(block=len(self.waves) >= self.inflight or not reading):
+                    for r in reqs:
+                        r.done -= t0
+                    answer(reqs)
+                    finished.extend(reqs)
+                    continue
+                time.sleep(2e-4)
+        th.join()
+        return finished, state["error"]
 
 
 def synthetic_requests(count, rate, n_lo=500, n_hi=50000, k_lo=10, k_hi=50, h=50, seed=6, t_start=0.0):

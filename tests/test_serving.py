@@ -95,3 +95,47 @@ def test_budgeted_serving_matches_unbudgeted():
     for r in capped:
         assert r.status == free[r.rid].status == 0
         assert np.array_equal(r.out, free[r.rid].out), r.rid
+
+
+def test_stream_frames_match_in_process_serving():
+    """Requests sent as wire frames over a socket pair are answered with the
+    outputs in-process serving gives the same requests (bit-identical), and a
+    frame with an out-of-range field is answered with status -4."""
+    import socket
+    import threading
+
+    import wire
+    S = _serve()
+    ref = {r.rid: r for r in S.Server(max_wave=12, inflight=1).run(
+        S.synthetic_requests(12, rate=np.inf, n_lo=500, n_hi=6000, seed=33))}
+    a, b = socket.socketpair()
+    got = {}
+
+    def client():
+        with a.makefile("wb") as w:
+            for r in S.synthetic_requests(12, rate=np.inf, n_lo=500, n_hi=6000, seed=33):
+                w.write(wire.encode_request(r.rid, r.x, r.L, r.k, r.g, r.h))
+            w.write(wire.encode_request(99, np.ones(100, np.float32), 100, 5, 1, 50))   # L = N: rejected
+        a.shutdown(socket.SHUT_WR)
+        with a.makefile("rb") as rd:
+            while True:
+                f = wire.read_response(rd)
+                if f is None:
+                    break
+                got[f[0]] = f[1:]
+
+    th = threading.Thread(target=client)
+    th.start()
+    with b.makefile("rb") as rf, b.makefile("wb") as wf:
+        done, err = S.Server(max_wave=4, inflight=2).serve_stream(rf, wf)
+    b.shutdown(socket.SHUT_WR)
+    th.join(timeout=60)
+    a.close()
+    b.close()
+    assert err is None and len(done) == 13
+    assert got[99][0] == wire.STATUS_REJECTED
+    for rid, r in ref.items():
+        st, out = got[rid]
+        assert st == (0 if r.status == 0 else wire.STATUS_FAILED)
+        if st == 0:
+            assert np.array_equal(out, r.out), rid
