@@ -896,6 +896,19 @@ int ssa_engine_decompose(ssa_engine *e, int q, int injected)
     return 0;
 }
 
+double *ssa_engine_host_alloc(size_t n)
+{
+    void *p = NULL;
+    if (hipHostMalloc(&p, n ? n * sizeof(double) : sizeof(double), hipHostMallocDefault) != hipSuccess) return NULL;
+    memset(p, 0, n * sizeof(double));
+    return (double *)p;
+}
+
+void ssa_engine_host_free(double *p)
+{
+    if (p) hipHostFree(p);
+}
+
 int ssa_engine_results(ssa_engine *e, int s, double *sigma, double *lambda, double *totvar, double *U, double *V,
                        double *sweeps)
 {
@@ -903,26 +916,32 @@ int ssa_engine_results(ssa_engine *e, int s, double *sigma, double *lambda, doub
     const ssa_desc *d = &e->hd[s];
     size_t misc_n = 4 * SSA_BMAX + 3;
     size_t u_n = U ? (size_t)d->ML * d->k : 0, v_n = V ? (size_t)d->K * d->k : 0;
-    DEVCHK(pin_reserve(e, misc_n * 8 + (u_n + v_n) * e->esz));
+    const int f64 = e->prec == SSA_PREC_F64;
+    /* U and V are page-locked (ssa_engine_host_alloc): FP64 results are copied into them directly */
+    const int du = f64 && u_n, dv = f64 && v_n;
+    const size_t su = du ? 0 : u_n, sv = dv ? 0 : v_n;   /* elements staged */
+    DEVCHK(pin_reserve(e, misc_n * 8 + (su + sv) * e->esz));
     double *misc = (double *)e->hpin;
     char *pu = (char *)e->hpin + misc_n * 8;
-    char *pv = pu + u_n * e->esz;
+    char *pv = pu + su * e->esz;
     HIPCHK(hipMemcpyAsync(misc, e->small + d->off_small + (long long)SSA_SLOT_MISC * SSA_BMAX * SSA_BMAX,
                           misc_n * 8, hipMemcpyDeviceToHost, e->st));
-    if (u_n) HIPCHK(hipMemcpyAsync(pu, (char *)e->arena_a + e->esz * d->off_U, u_n * e->esz, hipMemcpyDeviceToHost, e->st));
-    if (v_n) HIPCHK(hipMemcpyAsync(pv, (char *)e->arena_a + e->esz * d->off_V, v_n * e->esz, hipMemcpyDeviceToHost, e->st));
+    if (u_n) HIPCHK(hipMemcpyAsync(du ? (void *)U : (void *)pu, (char *)e->arena_a + e->esz * d->off_U, u_n * e->esz,
+                                   hipMemcpyDeviceToHost, e->st));
+    if (v_n) HIPCHK(hipMemcpyAsync(dv ? (void *)V : (void *)pv, (char *)e->arena_a + e->esz * d->off_V, v_n * e->esz,
+                                   hipMemcpyDeviceToHost, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
     if (sigma) memcpy(sigma, misc + SSA_MISC_SIGMA, sizeof(double) * d->k);
     if (lambda) memcpy(lambda, misc + SSA_MISC_LAMBDA, sizeof(double) * d->k);
     if (totvar) *totvar = misc[SSA_MISC_TOTVAR];
     if (sweeps) *sweeps = misc[SSA_MISC_SWEEPS];
-    if (u_n) {
-        if (e->prec == SSA_PREC_F64) memcpy(U, pu, u_n * 8);
-        else for (size_t i = 0; i < u_n; i++) U[i] = ((float *)pu)[i];
+    if (su) {
+        if (f64) memcpy(U, pu, su * 8);
+        else for (size_t i = 0; i < su; i++) U[i] = ((float *)pu)[i];
     }
-    if (v_n) {
-        if (e->prec == SSA_PREC_F64) memcpy(V, pv, v_n * 8);
-        else for (size_t i = 0; i < v_n; i++) V[i] = ((float *)pv)[i];
+    if (sv) {
+        if (f64) memcpy(V, pv, sv * 8);
+        else for (size_t i = 0; i < sv; i++) V[i] = ((float *)pv)[i];
     }
     return d->k;
 }

@@ -11,6 +11,7 @@
 #define SSA_ENGINE_H
 
 #include "ssa_device.h"
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -71,9 +72,14 @@ int ssa_engine_inject_omega(ssa_engine *e, int s, const double *Omega);
 
 /* decomposition of every series (q power iterations) */
 int ssa_engine_decompose(ssa_engine *e, int q, int injected);
-/* results of series s (any pointer may be NULL); returns k */
+/* results of series s (any pointer may be NULL); returns k.  U and V, when
+ * given, must come from ssa_engine_host_alloc: an FP64 engine copies into
+ * them directly, a mixed one through its staging buffer (FP32 to FP64) */
 int ssa_engine_results(ssa_engine *e, int s, double *sigma, double *lambda, double *totvar,
                        double *U, double *V, double *sweeps);
+/* zeroed page-locked host memory for n doubles (NULL on failure), and its release */
+double *ssa_engine_host_alloc(size_t n);
+void ssa_engine_host_free(double *p);
 
 /* reconstruction of n_groups groups of series s (channel ch for MSSA):
  * glist holds the concatenated groups, gcount their sizes; out receives

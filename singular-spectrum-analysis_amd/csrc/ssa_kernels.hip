@@ -187,7 +187,9 @@ __global__ __launch_bounds__(256) void k_omega(const ssa_desc *__restrict__ D, c
     const Panel<T> P = panel_of<T>(bulk, d, SSA_PANEL_Z);
     const int r0 = it.x * SSA_GRAM_ROWS, r1 = min(P.rows, r0 + SSA_GRAM_ROWS);
     const int pairs = (d.b + 1) / 2;
-    for (int j = 0; j < pairs; j++) {
+    // column pairs dealt over gridDim.y workgroups (more than one only when
+    // the launch holds few row tiles)
+    for (int j = blockIdx.y; j < pairs; j += gridDim.y) {
         T *c0 = P.p + (long long)(2 * j) * P.ld;
         T *c1 = c0 + P.ld;
         const bool two = 2 * j + 1 < d.b;
@@ -554,6 +556,45 @@ __device__ __forceinline__ double rdlane(double v, int lane)
 
 __host__ __device__ constexpr int tri_off(int j) { return j * (j + 1) / 2; }
 
+// 1/x and 1/sqrt(x) for normal, finite x (x > 0 for the square root): the
+// hardware estimate refined by two Newton steps, each of which squares the
+// relative error.  They replace IEEE division and square root on the serial
+// chains of the Cholesky and Jacobi steps; the results may differ from the
+// correctly rounded ones in the last bit.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+__device__ __forceinline__ double rsq_nr(double x)
+{
+    const double hx = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * fma(-hx * y, y, 1.5);
+    return y * fma(-hx * y, y, 1.5);
+}
+
+// v summed over each aligned group of 8 lanes, every lane of the group
+// receiving the same bits: DPP quad permutations (lane ^ 1, lane ^ 2) then the
+// half-row mirror (lane i of 8 with lane 7 - i).  All 8 lanes must be active.
+template <int CTRL> __device__ __forceinline__ double dpp64(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double sum8(double v)
+{
+    v += dpp64<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+    v += dpp64<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+    return v + dpp64<0x141>(v);   // row_half_mirror
+}
+
 // Sum the Gram partials of a series in tile order: lane j < b gets column j
 // (slots i < b, zero above), its diagonal dg and its column sum cy; lanes
 // j >= b get column j of the identity (dg = 1).  Reads are clamped into the
@@ -610,10 +651,10 @@ __device__ __forceinline__ void chol_regs(double (&a)[NB * 16], double dg, int b
         if (k >= b) continue;   // one uniform branch per step (it also bounds the scheduling region)
         const double p = rdlane(a[k], k), d0 = rdlane(dg, k);
         const bool dead = !(p > tol * d0) || !(d0 > 0.0);
-        const double piv = dead ? 0.0 : sqrt(p);
+        const double rp = dead ? 0.0 : rsq_nr(p), piv = p * rp;
         ndead += dead;
         if (ln == k) mydead = dead;
-        if (ln >= k) a[k] = dead ? 0.0 : (ln == k ? piv : a[k] / piv);
+        if (ln >= k) a[k] = dead ? 0.0 : (ln == k ? piv : a[k] * rp);
         if (ln >= k && ln < B16) Rl[tri_off(ln) + k] = a[k];   // lanes >= B16 hold no column
         __syncthreads();
 #pragma unroll
@@ -635,14 +676,15 @@ __device__ __forceinline__ void inv_rows(double (&a)[NB * 16], int b, int ln, bo
     for (int j = 0; j < B16; j++) {
         if (j >= b) continue;
         const double rjj = Rl[tri_off(j) + j];
+        const bool deadj = !(rjj > 0.0);
+        const double ir = deadj ? 0.0 : rcp_nr(rjj);
         double s = 0.0;
 #pragma unroll
         for (int l = 0; l < j; l++) {
             s = fma(-a[l], Rl[tri_off(j) + l], s);
             if ((l & 15) == 15) asm volatile("" ::: "memory");
         }
-        const bool deadj = !(rjj > 0.0);
-        if (ln <= j) a[j] = (deadj || mydead) ? 0.0 : (ln == j ? 1.0 / rjj : s / rjj);
+        if (ln <= j) a[j] = (deadj || mydead) ? 0.0 : (ln == j ? ir : s * ir);
         if (ln == j) {
 #pragma unroll
             for (int l = 0; l < j; l++) a[l] = 0.0;
@@ -1064,16 +1106,17 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                     be += y[m] * y[m];
                     ga += x[m] * y[m];
                 }
-                for (int o = 4; o > 0; o >>= 1) {
-                    al += __shfl_xor(al, o);
-                    be += __shfl_xor(be, o);
-                    ga += __shfl_xor(ga, o);
-                }
-                if (ga != 0.0 && fabs(ga) > early * sqrt(al * be)) big_local = 1;
-                if (ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
-                    const double zeta = (be - al) / (2.0 * ga);
-                    const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
+                al = sum8(al);
+                be = sum8(be);
+                ga = sum8(ga);
+                // cosine tests squared: |ga| > e sqrt(al be)  <=>  ga^2 > e^2 al be
+                const double ab = al * be, g2 = ga * ga;
+                if (ga != 0.0 && g2 > early * early * ab) big_local = 1;
+                if (ga != 0.0 && g2 > 1e-30 * ab) {
+                    const double zeta = (be - al) * rcp_nr(2.0 * ga);
+                    const double u = fma(zeta, zeta, 1.0);
+                    const double t = (zeta >= 0 ? 1.0 : -1.0) * rcp_nr(fabs(zeta) + u * rsq_nr(u));
+                    const double c = rsq_nr(fma(t, t, 1.0)), s = c * t;
 #pragma unroll
                     for (int m = 0; m < MB; m++) {
                         const int i = gl + 8 * m;
@@ -1485,8 +1528,13 @@ int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_item
 {
     if (n_items <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    if (prec == SSA_PREC_F64) hipLaunchKernelGGL(k_omega<double>, dim3(n_items), dim3(256), 0, st, D, items, (double *)bulk);
-    else hipLaunchKernelGGL(k_omega<float>, dim3(n_items), dim3(256), 0, st, D, items, (float *)bulk);
+    // about 1024 workgroups: a launch of few row tiles (one short series)
+    // splits each tile's column pairs over up to SSA_BMAX / 2 workgroups
+    int py = 1024 / n_items;
+    py = py < 1 ? 1 : py > SSA_BMAX / 2 ? SSA_BMAX / 2 : py;
+    const dim3 grid(n_items, py);
+    if (prec == SSA_PREC_F64) hipLaunchKernelGGL(k_omega<double>, grid, dim3(256), 0, st, D, items, (double *)bulk);
+    else hipLaunchKernelGGL(k_omega<float>, grid, dim3(256), 0, st, D, items, (float *)bulk);
     return (int)hipGetLastError();
 }
 

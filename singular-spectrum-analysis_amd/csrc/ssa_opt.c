@@ -106,7 +106,7 @@ int ssa_opt_init(SSA_Opt *ssa, const ssa_real *x, int N, int L)
 
 static void free_results(SSA_Opt *ssa)
 {
-    free(ssa->U); free(ssa->V); free(ssa->sigma); free(ssa->eigenvalues);
+    ssa_engine_host_free(ssa->U); ssa_engine_host_free(ssa->V); free(ssa->sigma); free(ssa->eigenvalues);
     ssa->U = ssa->V = ssa->sigma = ssa->eigenvalues = NULL;
 }
 
@@ -143,8 +143,9 @@ static int prepare_caps(SSA_Opt *ssa, int kp, int k)
     if (ssa_engine_set_caps(c->eng, 0, b, imin(k, b))) return -1;
     if (ssa_engine_prepare(c->eng, imin(k, b), SSA_MAX_H)) return -1;
     free_results(ssa);
-    ssa->U = (double *)calloc((size_t)ssa->L * k, sizeof(double));
-    ssa->V = (double *)calloc((size_t)ssa->K * k, sizeof(double));
+    /* page-locked host mirrors: every decomposition copies U and V into them */
+    ssa->U = ssa_engine_host_alloc((size_t)ssa->L * k);
+    ssa->V = ssa_engine_host_alloc((size_t)ssa->K * k);
     ssa->sigma = (double *)calloc(k, sizeof(double));
     ssa->eigenvalues = (double *)calloc(k, sizeof(double));
     if (!ssa->U || !ssa->V || !ssa->sigma || !ssa->eigenvalues) return -1;
@@ -1044,7 +1045,8 @@ void mssa_opt_free(MSSA_Opt *mssa)   /* MKL/ssa_opt.h:3881-3905 */
         free(c->x);
         free(c);
     }
-    free(mssa->U); free(mssa->V); free(mssa->sigma); free(mssa->eigenvalues); free(mssa->inv_diag_count);
+    ssa_engine_host_free(mssa->U); ssa_engine_host_free(mssa->V); free(mssa->sigma); free(mssa->eigenvalues);
+    free(mssa->inv_diag_count);
     memset(mssa, 0, sizeof(MSSA_Opt));
 }
 
@@ -1081,11 +1083,15 @@ int mssa_opt_decompose(MSSA_Opt *mssa, int k, int oversampling)
     int rc = ssa_engine_decompose(c->eng, mssa->q, 0);
     mssa->calls++;
     if (rc) return -1;
-    free(mssa->U); free(mssa->V); free(mssa->sigma); free(mssa->eigenvalues);
-    mssa->U = (double *)malloc(sizeof(double) * (size_t)ML * k);
-    mssa->V = (double *)malloc(sizeof(double) * (size_t)mssa->K * k);
-    mssa->sigma = (double *)malloc(sizeof(double) * k);
-    mssa->eigenvalues = (double *)malloc(sizeof(double) * k);
+    if (!(mssa->U && mssa->V && mssa->sigma && mssa->eigenvalues && mssa->n_components == k)) {
+        /* page-locked mirrors, kept while k is unchanged */
+        ssa_engine_host_free(mssa->U); ssa_engine_host_free(mssa->V); free(mssa->sigma); free(mssa->eigenvalues);
+        mssa->n_components = 0;
+        mssa->U = ssa_engine_host_alloc((size_t)ML * k);
+        mssa->V = ssa_engine_host_alloc((size_t)mssa->K * k);
+        mssa->sigma = (double *)malloc(sizeof(double) * k);
+        mssa->eigenvalues = (double *)malloc(sizeof(double) * k);
+    }
     if (!mssa->U || !mssa->V || !mssa->sigma || !mssa->eigenvalues) return -1;
     double tv = 0;
     if (ssa_engine_results(c->eng, 0, mssa->sigma, mssa->eigenvalues, &tv, mssa->U, mssa->V, NULL) != k) return -1;
