@@ -897,8 +897,11 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply(const ssa_desc *D, c
     double *Ms = sm;
     stage_small<NB>(small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX, b, nc, Ms, 64 * APPLY_WAVES);
     __syncthreads();
-    const int r_begin = it.x * SSA_GRAM_ROWS;
-    const int r_end = min(Pi.rows, r_begin + SSA_GRAM_ROWS);
+    // gridDim.y (a power of two) workgroups share the tile in equal row ranges;
+    // every output row is formed the same way whatever its range
+    const int sub = SSA_GRAM_ROWS / (int)gridDim.y;
+    const int r_begin = it.x * SSA_GRAM_ROWS + (int)blockIdx.y * sub;
+    const int r_end = min(Pi.rows, r_begin + sub);
     T cur[KK][VR], nxt[KK][VR];
     auto load = [&](int r0, T (&a)[KK][VR]) {
 #pragma unroll
@@ -982,8 +985,9 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply_f32(const ssa_desc *
         }
     }
     __syncthreads();
-    const int r_begin = it.x * SSA_GRAM_ROWS;
-    const int r_end = min(Pi.rows, r_begin + SSA_GRAM_ROWS);
+    const int sub = SSA_GRAM_ROWS / (int)gridDim.y;     // as in k_apply
+    const int r_begin = it.x * SSA_GRAM_ROWS + (int)blockIdx.y * sub;
+    const int r_end = min(Pi.rows, r_begin + sub);
     float cur[KK][VR], nxt[KK][VR];
     auto load = [&](int r0, float (&a)[KK][VR]) {
 #pragma unroll
@@ -1047,9 +1051,14 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                                              int rinv_slot, double early)
 {
     extern __shared__ double sm[];
-    const int bq = D[items[blockIdx.x].s].b;
-    const int LDP = bq + 1;                // padded column length
-    double *A = sm;                        // column j at A + j*LDP (rows 0..b-1), bp columns
+    // Column j at A + j*LDP, rows 0..16 NB - 1 (zero at and beyond b), so the
+    // 8 lanes of a group load and store a whole column without masks.  LDP is
+    // an odd multiple of 8 doubles: the 8-double row blocks of columns c and
+    // c + 1, c + 2, c + 3 start 16 banks apart, and the groups of one
+    // half-wave, which hold consecutive pair slots and hence consecutive
+    // column numbers, touch disjoint banks.
+    constexpr int ROWS = 16 * NB, LDP = ROWS + 8;
+    double *A = sm;
     __shared__ int perm[SSA_BMAX + 1];
     __shared__ double S[SSA_BMAX + 1];
     __shared__ int rotated, big;
@@ -1058,9 +1067,9 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     const int b = d.b, bp = (b + 1) & ~1;
     const double *R = small + d.off_small + (long long)r_slot * SSA_BMAX * SSA_BMAX;
     // A = R^T: column j is row j of R (R column-major, ld SSA_BMAX)
-    for (int idx = threadIdx.x; idx < bp * b; idx += 256) {
-        const int i = idx / bp, j = idx - i * bp;
-        A[j * LDP + i] = (j < b) ? R[(long long)i * SSA_BMAX + j] : 0.0;
+    for (int idx = threadIdx.x; idx < bp * ROWS; idx += 256) {
+        const int j = idx / ROWS, i = idx - j * ROWS;
+        A[j * LDP + i] = (j < b && i < b) ? R[(long long)i * SSA_BMAX + j] : 0.0;
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1088,27 +1097,29 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                     q = round - pi;
                     if (q < 0) q += bp - 1;
                 }
-                if (p > q) { int t = p; p = q; q = t; }
-                // rows gl + 8m of both columns (and of W), all loads issued up front
-                constexpr int MB = 2 * NB;     // rows per lane: b <= 16 NB rows over 8 lanes
+                // The rotation is defined on the ordered pair (min, max) of the two
+                // column numbers; columns are loaded in slot order (x = p, y = q) and
+                // a swapped pair exchanges the roles of the norms and the sign of s.
+                const bool sw = p > q;
+                // rows gl + 8m of both columns, all loads issued up front
+                constexpr int MB = 2 * NB;     // rows per lane: 16 NB rows over 8 lanes
                 double x[MB], y[MB];
 #pragma unroll
                 for (int m = 0; m < MB; m++) {
-                    const int i = gl + 8 * m;
-                    const bool in = i < b;
-                    x[m] = in ? A[p * LDP + i] : 0.0;
-                    y[m] = in ? A[q * LDP + i] : 0.0;
+                    x[m] = A[p * LDP + gl + 8 * m];
+                    y[m] = A[q * LDP + gl + 8 * m];
                 }
-                double al = 0, be = 0, ga = 0;
+                double xx = 0, yy = 0, ga = 0;
 #pragma unroll
                 for (int m = 0; m < MB; m++) {
-                    al += x[m] * x[m];
-                    be += y[m] * y[m];
-                    ga += x[m] * y[m];
+                    xx = fma(x[m], x[m], xx);
+                    yy = fma(y[m], y[m], yy);
+                    ga = fma(x[m], y[m], ga);
                 }
-                al = sum8(al);
-                be = sum8(be);
+                xx = sum8(xx);
+                yy = sum8(yy);
                 ga = sum8(ga);
+                const double al = sw ? yy : xx, be = sw ? xx : yy;
                 // cosine tests squared: |ga| > e sqrt(al be)  <=>  ga^2 > e^2 al be
                 const double ab = al * be, g2 = ga * ga;
                 if (ga != 0.0 && g2 > early * early * ab) big_local = 1;
@@ -1116,14 +1127,12 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                     const double zeta = (be - al) * rcp_nr(2.0 * ga);
                     const double u = fma(zeta, zeta, 1.0);
                     const double t = (zeta >= 0 ? 1.0 : -1.0) * rcp_nr(fabs(zeta) + u * rsq_nr(u));
-                    const double c = rsq_nr(fma(t, t, 1.0)), s = c * t;
+                    const double c = rsq_nr(fma(t, t, 1.0)), s = (sw ? -c : c) * t;
+                    // (min, max) <- (c min - s max, s min + c max), written for (p, q)
 #pragma unroll
                     for (int m = 0; m < MB; m++) {
-                        const int i = gl + 8 * m;
-                        if (i < b) {
-                            A[p * LDP + i] = c * x[m] - s * y[m];
-                            A[q * LDP + i] = s * x[m] + c * y[m];
-                        }
+                        A[p * LDP + gl + 8 * m] = c * x[m] - s * y[m];
+                        A[q * LDP + gl + 8 * m] = s * x[m] + c * y[m];
                     }
                     any_local = 1;
                 }
@@ -1611,14 +1620,19 @@ static void launch_apply(int src, int dst, int slot, int ncols_k, const ssa_desc
         const char *v = getenv("SSA_AMD_APPLY_F64");
         return v && atoi(v) != 0;
     }();
+    // a launch of few row tiles (short series) splits each tile over up to 8
+    // workgroups of 256 rows, so that more CUs share it
+    int py = 1;
+    while (py < 8 && n_items * py * 2 <= 1024) py *= 2;
+    const dim3 grid(n_items, py);
     if (sizeof(T) == 4 && !apply_f64) {
         const size_t lds = (size_t)NB * 16 * small_ld<NB>() * sizeof(float);
-        hipLaunchKernelGGL((k_apply_f32<NB>), dim3(n_items), dim3(64 * APPLY_WAVES), lds, st, D, items,
+        hipLaunchKernelGGL((k_apply_f32<NB>), grid, dim3(64 * APPLY_WAVES), lds, st, D, items,
                            (float *)bulk, small, src, dst, slot, ncols_k);
         return;
     }
     const size_t lds = (size_t)NB * 16 * small_ld<NB>() * sizeof(double);
-    hipLaunchKernelGGL((k_apply<T, NB>), dim3(n_items), dim3(64 * APPLY_WAVES), lds, st, D, items, (T *)bulk, small,
+    hipLaunchKernelGGL((k_apply<T, NB>), grid, dim3(64 * APPLY_WAVES), lds, st, D, items, (T *)bulk, small,
                        src, dst, slot, ncols_k);
 }
 }  // extern "C++"
@@ -1661,7 +1675,8 @@ int ssa_dev_svd(int bmax, int r_slot, int rinv_slot, double early, const ssa_des
                 int n_items, double *small, void *stream)
 {
     if (n_items <= 0) return 0;
-    const size_t lds = (size_t)((bmax + 1) & ~1) * (bmax + 1) * sizeof(double);
+    // 16 NB columns of 16 NB + 8 doubles
+    const size_t lds = (size_t)16 * SSA_NB(bmax) * (16 * SSA_NB(bmax) + 8) * sizeof(double);
     hipStream_t st = (hipStream_t)stream;
     switch (SSA_NB(bmax)) {
     case 1: hipLaunchKernelGGL(k_svd<1>, dim3(n_items), dim3(256), lds, st, D, items, small, r_slot, rinv_slot, early); break;
