@@ -394,9 +394,14 @@ template <int NB> constexpr int n_upper_blocks() { return NB * (NB + 1) / 2; }
 // accumulation whatever the panel type; only the upper block triangle is
 // multiplied, the partial is written in full.  USE_M = 0 also emits the column
 // sums of P.  The 4 waves' sums are added in a fixed order (deterministic).
-// NB = column blocks of 16 covering every panel of the launch.
+// NB = column blocks of 16 covering every panel of the launch.  SPLIT: the
+// gridDim.y workgroups of an item share its upper blocks (block m to workgroup
+// m % gridDim.y, which loads only the column blocks those use; the sums of
+// column block I from the owner of block (I, I)) for launches of few items;
+// each block is summed in the same order either way, so the partial is the
+// same.
 // ---------------------------------------------------------------------------
-template <typename T, int NB, int USE_M>
+template <typename T, int NB, int USE_M, bool SPLIT>
 __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items,
                                               const T *__restrict__ bulk, double *small, int which)
 {
@@ -417,13 +422,24 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
     for (int I = 0; I < NB; I++) cs[I] = 0.0;
 
     if constexpr (!USE_M) {
+        // column blocks this workgroup's upper blocks use (all without SPLIT)
+        unsigned need = ~0u;
+        if constexpr (SPLIT) {
+            need = 0;
+            int m = 0;
+#pragma unroll
+            for (int I = 0; I < NB; I++)
+#pragma unroll
+                for (int J = I; J < NB; J++, m++)
+                    if (m % (int)gridDim.y == (int)blockIdx.y) need |= (1u << I) | (1u << J);
+        }
         // 16-row slabs, slab j of the item to wave j % 4
         T cur[NB][4], nxt[NB][4];
         auto load = [&](int r0, T (&x)[NB][4]) {
 #pragma unroll
             for (int I = 0; I < NB; I++) {
                 const int c = I * 16 + cl, r = r0 + 4 * kq;
-                if (c < b && r0 < r_end) load4(P.p + (long long)c * P.ld + r, r_end - r, x[I]);
+                if (((need >> I) & 1u) && c < b && r0 < r_end) load4(P.p + (long long)c * P.ld + r, r_end - r, x[I]);
                 else {
 #pragma unroll
                     for (int v = 0; v < 4; v++) x[I][v] = (T)0;
@@ -443,7 +459,9 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
 #pragma unroll
                 for (int I = 0; I < NB; I++)
 #pragma unroll
-                    for (int J = I; J < NB; J++, m++) acc[m] = mfma64((double)cur[I][v], (double)cur[J][v], acc[m]);
+                    for (int J = I; J < NB; J++, m++)
+                        if (!SPLIT || m % (int)gridDim.y == (int)blockIdx.y)
+                            acc[m] = mfma64((double)cur[I][v], (double)cur[J][v], acc[m]);
             }
 #pragma unroll
             for (int I = 0; I < NB; I++)
@@ -484,7 +502,8 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
 #pragma unroll
                 for (int I = 0; I < NB; I++)
 #pragma unroll
-                    for (int J = I; J < NB; J++, m++) acc[m] = mfma64(t[I][v], t[J][v], acc[m]);
+                    for (int J = I; J < NB; J++, m++)
+                        if (!SPLIT || m % (int)gridDim.y == (int)blockIdx.y) acc[m] = mfma64(t[I][v], t[J][v], acc[m]);
             }
 #pragma unroll
             for (int kk = 0; kk < KK; kk++) cur[kk] = nxt[kk];
@@ -521,6 +540,7 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
     double *out = small + d.off_part + (long long)it.x * (b * b + b);
     for (int idx = threadIdx.x; idx < NBLK * 256; idx += 256) {
         const int m = idx >> 8, v = (idx >> 6) & 3, ll = idx & 63;
+        if (SPLIT && m % (int)gridDim.y != (int)blockIdx.y) continue;
         int I = 0, mm = m;
         while (mm >= NB - I) { mm -= NB - I; I++; }
         const int J = I + mm;
@@ -531,7 +551,10 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
             if (I != J) out[j * b + i] = g;
         }
     }
-    if (!USE_M && threadIdx.x < b) out[b * b + threadIdx.x] = rcs[threadIdx.x];
+    // the sums of column block I come from the workgroup of diagonal block (I, I)
+    const int Ic = threadIdx.x >> 4, mdiag = Ic * NB - Ic * (Ic - 1) / 2;
+    if (!USE_M && threadIdx.x < b && (!SPLIT || mdiag % (int)gridDim.y == (int)blockIdx.y))
+        out[b * b + threadIdx.x] = rcs[threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -1555,11 +1578,22 @@ static void launch_gram(int panel, int use_m, const ssa_desc *D, const ssa_item 
     constexpr int NBLK = n_upper_blocks<NB>(), B16 = NB * 16, MP = small_ld<NB>();
     const size_t red = (size_t)(NBLK * 256 + NB * 16) * sizeof(double);
     const size_t ms = (size_t)B16 * MP * sizeof(double);
-    if (use_m)
-        hipLaunchKernelGGL((k_gram<T, NB, 1>), dim3(n_items), dim3(256), red > ms ? red : ms, st, D, items,
+    // a launch of few row tiles (short series) deals each tile's upper blocks
+    // over up to NBLK workgroups
+    const int py = n_items >= 512 || NBLK == 1 ? 1 : (1024 / n_items < NBLK ? 1024 / n_items : NBLK);
+    const dim3 grid(n_items, py);
+    if (py > 1) {
+        if (use_m)
+            hipLaunchKernelGGL((k_gram<T, NB, 1, true>), grid, dim3(256), red > ms ? red : ms, st, D, items,
+                               (const T *)bulk, small, panel);
+        else
+            hipLaunchKernelGGL((k_gram<T, NB, 0, true>), grid, dim3(256), red, st, D, items, (const T *)bulk,
+                               small, panel);
+    } else if (use_m)
+        hipLaunchKernelGGL((k_gram<T, NB, 1, false>), grid, dim3(256), red > ms ? red : ms, st, D, items,
                            (const T *)bulk, small, panel);
     else
-        hipLaunchKernelGGL((k_gram<T, NB, 0>), dim3(n_items), dim3(256), red, st, D, items, (const T *)bulk, small,
+        hipLaunchKernelGGL((k_gram<T, NB, 0, false>), grid, dim3(256), red, st, D, items, (const T *)bulk, small,
                            panel);
 }
 }  // extern "C++"
