@@ -371,15 +371,29 @@ template <typename T> __device__ __forceinline__ void store_vr(T *p, int n, cons
 // == 16 (mod 32) so the 4 row groups of one MFMA operand read hit disjoint banks
 template <int NB> constexpr int small_ld() { return (NB & 1) ? NB * 16 : NB * 16 + 16; }
 
-// Stage the b x nc column-major small matrix M (ld SSA_BMAX) into Ms[k*MP + c],
-// zero beyond b rows / nc columns, NB*16 rows and columns.
-template <int NB>
-__device__ __forceinline__ void stage_small(const double *M, int b, int nc, double *Ms, int nthreads)
+// Stage the b x nc column-major small matrix M (ld SSA_BMAX) into Ms[k*MP + c]
+// (as type S), zero beyond b rows / nc columns, NB*16 rows and columns, with
+// NTHREADS threads.  Every load is issued before the first store (addresses
+// clamped into the slot, values masked afterwards), so staging costs one
+// memory latency instead of one per pass.
+template <int NB, int NTHREADS, typename S = double>
+__device__ __forceinline__ void stage_small(const double *M, int b, int nc, S *Ms)
 {
-    constexpr int B16 = NB * 16, MP = small_ld<NB>();
-    for (int idx = threadIdx.x; idx < B16 * B16; idx += nthreads) {
+    constexpr int B16 = NB * 16, MP = small_ld<NB>(), PER = (B16 * B16 + NTHREADS - 1) / NTHREADS;
+    double v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int idx = (int)threadIdx.x + q * NTHREADS;
         const int c = idx / B16, k = idx - c * B16;
-        Ms[k * MP + c] = (k < b && c < nc) ? M[(long long)c * SSA_BMAX + k] : 0.0;
+        const bool in = idx < B16 * B16 && k < b && c < nc;
+        const double x = M[in ? (long long)c * SSA_BMAX + k : 0];
+        v[q] = in ? x : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int idx = (int)threadIdx.x + q * NTHREADS;
+        const int c = idx / B16, k = idx - c * B16;
+        if (idx < B16 * B16) Ms[k * MP + c] = (S)v[q];
     }
 }
 
@@ -472,7 +486,7 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
         // one row per lane (16-row slabs): P M for a slab stays in registers
         constexpr int KK = NB * 4;
         double *Ms = sm;
-        stage_small<NB>(slot(small, d, SSA_SLOT_R1INV), b, b, Ms, 256);
+        stage_small<NB, 256>(slot(small, d, SSA_SLOT_R1INV), b, b, Ms);
         __syncthreads();
         T cur[KK], nxt[KK];
         auto load = [&](int r0, T (&a)[KK]) {
@@ -917,9 +931,6 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply(const ssa_desc *D, c
     const Panel<T> Po = panel_of<T>(bulk, d, dst);
     const int b = d.b, nc = ncols_k ? d.k : d.b;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, cl = l & 15, kq = l >> 4;
-    double *Ms = sm;
-    stage_small<NB>(small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX, b, nc, Ms, 64 * APPLY_WAVES);
-    __syncthreads();
     // gridDim.y (a power of two) workgroups share the tile in equal row ranges;
     // every output row is formed the same way whatever its range
     const int sub = SSA_GRAM_ROWS / (int)gridDim.y;
@@ -938,7 +949,10 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply(const ssa_desc *D, c
         }
     };
     int r0 = r_begin + SL * w;
-    load(r0, cur);
+    load(r0, cur);       // the first rows are in flight while M is staged
+    double *Ms = sm;
+    stage_small<NB, 64 * APPLY_WAVES>(small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX, b, nc, Ms);
+    __syncthreads();
     for (; r0 < r_end; r0 += APPLY_WAVES * SL) {
         load(r0 + APPLY_WAVES * SL, nxt);
         const int r = r0 + VR * cl;
@@ -999,15 +1013,6 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply_f32(const ssa_desc *
     const Panel<float> Po = panel_of<float>(bulk, d, dst);
     const int b = d.b, nc = ncols_k ? d.k : d.b;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, cl = l & 15, kq = l >> 4;
-    float *Ms = smf;
-    {
-        const double *M = small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX;
-        for (int idx = threadIdx.x; idx < B16 * B16; idx += 64 * APPLY_WAVES) {
-            const int c = idx / B16, k = idx - c * B16;
-            Ms[k * MP + c] = (k < b && c < nc) ? (float)M[(long long)c * SSA_BMAX + k] : 0.0f;
-        }
-    }
-    __syncthreads();
     const int sub = SSA_GRAM_ROWS / (int)gridDim.y;     // as in k_apply
     const int r_begin = it.x * SSA_GRAM_ROWS + (int)blockIdx.y * sub;
     const int r_end = min(Pi.rows, r_begin + sub);
@@ -1024,7 +1029,10 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply_f32(const ssa_desc *
         }
     };
     int r0 = r_begin + SL * w;
-    load(r0, cur);
+    load(r0, cur);       // the first rows are in flight while M is staged
+    float *Ms = smf;
+    stage_small<NB, 64 * APPLY_WAVES, float>(small + d.off_small + (long long)mslot * SSA_BMAX * SSA_BMAX, b, nc, Ms);
+    __syncthreads();
     for (; r0 < r_end; r0 += APPLY_WAVES * SL) {
         load(r0 + APPLY_WAVES * SL, nxt);
         const int r = r0 + VR * cl;
@@ -1089,10 +1097,24 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     const ssa_desc d = D[it.s];
     const int b = d.b, bp = (b + 1) & ~1;
     const double *R = small + d.off_small + (long long)r_slot * SSA_BMAX * SSA_BMAX;
-    // A = R^T: column j is row j of R (R column-major, ld SSA_BMAX)
-    for (int idx = threadIdx.x; idx < bp * ROWS; idx += 256) {
-        const int j = idx / ROWS, i = idx - j * ROWS;
-        A[j * LDP + i] = (j < b && i < b) ? R[(long long)i * SSA_BMAX + j] : 0.0;
+    // A = R^T: column j is row j of R (R column-major, ld SSA_BMAX); all loads
+    // are issued before the first LDS store (consecutive threads read
+    // consecutive j of one row i of R)
+    {
+        constexpr int PER = ROWS * ROWS / 256;
+        double v[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int idx = (int)threadIdx.x + 256 * q, i = idx / ROWS, j = idx - i * ROWS;
+            const bool in = i < b && j < b;
+            const double x = R[in ? (long long)i * SSA_BMAX + j : 0];
+            v[q] = in ? x : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int idx = (int)threadIdx.x + 256 * q, i = idx / ROWS, j = idx - i * ROWS;
+            A[j * LDP + i] = v[q];
+        }
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
