@@ -113,24 +113,27 @@ void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint3
 }
 
 /* out[c*rows + r] ~ N(0,1), column-major rows x cols.  Row r of the column
- * pair (2j, 2j+1) comes from one Philox call with counter (r, j, call_lo,
- * call_hi) under key (seed_lo, seed_hi): two 53-bit uniforms u1, u2 give the
- * Box-Muller pair rad*cos(2 pi u2) -> column 2j, rad*sin(2 pi u2) -> 2j+1,
- * rad = sqrt(-2 ln(1 - u1)).  The device path draws the same values. */
+ * quad (4j .. 4j+3) comes from one Philox call with counter (r, j, call_lo,
+ * call_hi) under key (seed_lo, seed_hi); its four 32-bit outputs o give two
+ * Box-Muller pairs: u1 = o[2h] 2^-32, u2 = o[2h+1] 2^-32,
+ * rad = sqrt(-2 ln(1 - u1)), rad*cos(2 pi u2) -> column 4j+2h,
+ * rad*sin(2 pi u2) -> column 4j+2h+1.  The device path draws the same values. */
 void orc_philox_gaussian(uint64_t seed, uint64_t call, int rows, int cols, double *out)
 {
     uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
-    for (int j = 0; 2 * j < cols; j++) {
+    for (int j = 0; 4 * j < cols; j++) {
         for (int r = 0; r < rows; r++) {
             uint32_t ctr[4] = { (uint32_t)r, (uint32_t)j, (uint32_t)call, (uint32_t)(call >> 32) };
             uint32_t o[4];
             orc_philox4x32_10(ctr, key, o);
-            double u1 = (double)(((uint64_t)(o[0] >> 5) << 26) | (o[1] >> 6)) * 0x1p-53;
-            double u2 = (double)(((uint64_t)(o[2] >> 5) << 26) | (o[3] >> 6)) * 0x1p-53;
-            double rad = sqrt(-2.0 * log(1.0 - u1));
-            double ang = 6.283185307179586476925286766559 * u2;
-            out[(size_t)(2 * j) * rows + r] = rad * cos(ang);
-            if (2 * j + 1 < cols) out[(size_t)(2 * j + 1) * rows + r] = rad * sin(ang);
+            for (int h = 0; h < 2; h++) {
+                double u1 = (double)o[2 * h] * 0x1p-32, u2 = (double)o[2 * h + 1] * 0x1p-32;
+                double rad = sqrt(-2.0 * log(1.0 - u1));
+                double ang = 6.283185307179586476925286766559 * u2;
+                int c = 4 * j + 2 * h;
+                if (c < cols) out[(size_t)c * rows + r] = rad * cos(ang);
+                if (c + 1 < cols) out[(size_t)(c + 1) * rows + r] = rad * sin(ang);
+            }
         }
     }
 }

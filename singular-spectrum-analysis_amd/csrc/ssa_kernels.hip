@@ -59,7 +59,11 @@ template <typename T> __device__ __forceinline__ Panel<T> panel_of(T *bulk, cons
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 + Box-Muller: identical definition to the oracle's
-// orc_philox_gaussian (row r, column pair j -> counter (r, j, call_lo, call_hi))
+// orc_philox_gaussian.  Row r of column quad j (columns 4j .. 4j+3) comes from
+// counter (r, j, call_lo, call_hi); the call's four 32-bit outputs o are the
+// uniforms of two Box-Muller pairs: u1 = o[2h] 2^-32, u2 = o[2h+1] 2^-32,
+// rad = sqrt(-2 ln(1 - u1)), rad cos(2 pi u2) -> column 4j+2h and
+// rad sin(2 pi u2) -> column 4j+2h+1.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
@@ -78,32 +82,36 @@ __device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1)
     }
 }
 
-__device__ __forceinline__ void gauss_pair(uint64_t seed, uint64_t call, int row, int pair, double &g0, double &g1)
+__device__ __forceinline__ void gauss_quad(uint64_t seed, uint64_t call, int row, int quad, double (&g)[4])
 {
-    uint32_t c[4] = {(uint32_t)row, (uint32_t)pair, (uint32_t)call, (uint32_t)(call >> 32)};
+    uint32_t c[4] = {(uint32_t)row, (uint32_t)quad, (uint32_t)call, (uint32_t)(call >> 32)};
     philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    double u1 = (double)(((uint64_t)(c[0] >> 5) << 26) | (c[1] >> 6)) * 0x1p-53;
-    double u2 = (double)(((uint64_t)(c[2] >> 5) << 26) | (c[3] >> 6)) * 0x1p-53;
-    double rad = sqrt(-2.0 * log(1.0 - u1));
-    double ang = 6.283185307179586476925286766559 * u2;
-    g0 = rad * cos(ang);
-    g1 = rad * sin(ang);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const double u1 = (double)c[2 * h] * 0x1p-32, u2 = (double)c[2 * h + 1] * 0x1p-32;
+        const double rad = sqrt(-2.0 * log(1.0 - u1));
+        const double ang = 6.283185307179586476925286766559 * u2;
+        g[2 * h] = rad * cos(ang);
+        g[2 * h + 1] = rad * sin(ang);
+    }
 }
 
 // The same draw with the transcendental steps in FP32 (the FP32 panels of
 // the mixed form): the uniforms and 1 - u1 are formed exactly as above in
 // FP64, then rounded, so the tail of the radius is kept.
-__device__ __forceinline__ void gauss_pair_f(uint64_t seed, uint64_t call, int row, int pair, float &g0, float &g1)
+__device__ __forceinline__ void gauss_quad_f(uint64_t seed, uint64_t call, int row, int quad, float (&g)[4])
 {
-    uint32_t c[4] = {(uint32_t)row, (uint32_t)pair, (uint32_t)call, (uint32_t)(call >> 32)};
+    uint32_t c[4] = {(uint32_t)row, (uint32_t)quad, (uint32_t)call, (uint32_t)(call >> 32)};
     philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
-    const double u1 = (double)(((uint64_t)(c[0] >> 5) << 26) | (c[1] >> 6)) * 0x1p-53;
-    const double u2 = (double)(((uint64_t)(c[2] >> 5) << 26) | (c[3] >> 6)) * 0x1p-53;
-    const float rad = sqrtf(-2.0f * logf((float)(1.0 - u1)));
-    float sn, cs;
-    sincosf(6.2831853071795865f * (float)u2, &sn, &cs);
-    g0 = rad * cs;
-    g1 = rad * sn;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const double u1 = (double)c[2 * h] * 0x1p-32, u2 = (double)c[2 * h + 1] * 0x1p-32;
+        const float rad = sqrtf(-2.0f * logf((float)(1.0 - u1)));
+        float sn, cs;
+        sincosf(6.2831853071795865f * (float)u2, &sn, &cs);
+        g[2 * h] = rad * cs;
+        g[2 * h + 1] = rad * sn;
+    }
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin; launches over 8*ceil(n/8)
@@ -173,8 +181,8 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_spectra(const ssa
 }
 
 // ---------------------------------------------------------------------------
-// Gaussian test matrix into panel Z (K x b): column pair j of row r is the
-// Philox draw (row r, pair j) of the series' key and call counter; FP32
+// Gaussian test matrix into panel Z (K x b): columns 4j .. 4j+3 of row r are
+// the Philox draw (row r, quad j) of the series' key and call counter; FP32
 // panels use the FP32 Box-Muller of the same uniforms.  Item (s, tile of
 // SSA_GRAM_ROWS rows).
 // ---------------------------------------------------------------------------
@@ -186,19 +194,19 @@ __global__ __launch_bounds__(256) void k_omega(const ssa_desc *__restrict__ D, c
     const ssa_desc d = D[it.s];
     const Panel<T> P = panel_of<T>(bulk, d, SSA_PANEL_Z);
     const int r0 = it.x * SSA_GRAM_ROWS, r1 = min(P.rows, r0 + SSA_GRAM_ROWS);
-    const int pairs = (d.b + 1) / 2;
-    // column pairs dealt over gridDim.y workgroups (more than one only when
+    const int quads = (d.b + 3) / 4;
+    // column quads dealt over gridDim.y workgroups (more than one only when
     // the launch holds few row tiles)
-    for (int j = blockIdx.y; j < pairs; j += gridDim.y) {
-        T *c0 = P.p + (long long)(2 * j) * P.ld;
-        T *c1 = c0 + P.ld;
-        const bool two = 2 * j + 1 < d.b;
+    for (int j = blockIdx.y; j < quads; j += gridDim.y) {
+        T *c0 = P.p + (long long)(4 * j) * P.ld;
+        const int nc = min(4, d.b - 4 * j);
         for (int r = r0 + (int)threadIdx.x; r < r1; r += 256) {
-            T g0, g1;
-            if constexpr (sizeof(T) == 4) gauss_pair_f(d.seed, d.call, r, j, g0, g1);
-            else gauss_pair(d.seed, d.call, r, j, g0, g1);
-            c0[r] = g0;
-            if (two) c1[r] = g1;
+            T g[4];
+            if constexpr (sizeof(T) == 4) gauss_quad_f(d.seed, d.call, r, j, g);
+            else gauss_quad(d.seed, d.call, r, j, g);
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                if (t < nc) c0[(long long)t * P.ld + r] = g[t];
         }
     }
 }
@@ -1583,9 +1591,9 @@ int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_item
     if (n_items <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     // about 1024 workgroups: a launch of few row tiles (one short series)
-    // splits each tile's column pairs over up to SSA_BMAX / 2 workgroups
+    // splits each tile's column quads over up to SSA_BMAX / 4 workgroups
     int py = 1024 / n_items;
-    py = py < 1 ? 1 : py > SSA_BMAX / 2 ? SSA_BMAX / 2 : py;
+    py = py < 1 ? 1 : py > SSA_BMAX / 4 ? SSA_BMAX / 4 : py;
     const dim3 grid(n_items, py);
     if (prec == SSA_PREC_F64) hipLaunchKernelGGL(k_omega<double>, grid, dim3(256), 0, st, D, items, (double *)bulk);
     else hipLaunchKernelGGL(k_omega<float>, grid, dim3(256), 0, st, D, items, (float *)bulk);
