@@ -245,6 +245,11 @@ int ssa_amd_reconstruct_many(const SSA_Opt *ssa, const int *glist, const int *gc
                              ssa_real *output);
 /* the device correlation plan of the context: out[0..5] = n, CL, CK, AL, SK, Jacobi sweeps */
 int ssa_amd_plan_info(const SSA_Opt *ssa, int *out6);
+/* streaming tick: slide the window by m samples (1 <= m <= N), x <- x[m:N]
+ * followed by x_new[0:m], and refresh the device spectra; the same state as
+ * ssa_opt_update_signal (MKL/ssa_opt.h:1545-1558) with the shifted window,
+ * without the caller holding the window.  Clears decomposed. */
+int ssa_amd_push_samples(SSA_Opt *ssa, const ssa_real *x_new, int m);
 
 /* autotuner: time the block matvec at every FFT length 2^6..2^13 on this
  * device for the precision (0 plain FP64, 1 mixed) and plan every context

@@ -180,6 +180,21 @@ int ssa_opt_update_signal(SSA_Opt *ssa, const ssa_real *new_x)
     return 0;
 }
 
+/* the context holds the window (c->x); a tick moves it by m samples */
+int ssa_amd_push_samples(SSA_Opt *ssa, const ssa_real *x_new, int m)
+{
+    if (!ssa || !ssa->initialized || !x_new || m < 1 || m > ssa->N) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    const int N = ssa->N;
+    memmove(c->x, c->x + m, sizeof(double) * (size_t)(N - m));
+    memcpy(c->x + (N - m), x_new, sizeof(double) * (size_t)m);
+    ssa->decomposed = false;
+    ssa->fft_cached = false;
+    if (ssa_engine_upload_signal(c->eng, 0, c->x) || ssa_engine_spectra(c->eng)) return -1;
+    if (ssa_engine_read_spectrum(c->eng, 0, ssa->fft_x, ssa->r2c_len)) return -1;
+    return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* decomposition                                                             */
 /* ------------------------------------------------------------------------ */

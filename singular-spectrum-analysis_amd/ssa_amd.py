@@ -112,6 +112,7 @@ SIGNATURES = [
     ("ssa_amd_inject_omega", _I, [_P, c_double_p, _I, _I]),
     ("ssa_amd_reconstruct_many", _I, [_P, c_int_p, c_int_p, _I, c_double_p]),
     ("ssa_amd_plan_info", _I, [_P, c_int_p]),
+    ("ssa_amd_push_samples", _I, [_P, c_double_p, _I]),
     ("ssa_amd_batch_create", _VP, [_I, c_int_p, c_int_p, c_int_p, _I, _I, _I, _U64]),
     ("ssa_amd_batch_set_signals_f32", _I, [_VP, c_float_p, c_ll_p]),
     ("ssa_amd_batch_set_key", _I, [_VP, _I, _U64]),
@@ -308,6 +309,15 @@ class SSA:
         if self._l.ssa_opt_update_signal(ctypes.byref(self._ctx), _dp(new_x)) != 0:
             raise RuntimeError("update_signal() failed")
         self._x = new_x
+        return self
+
+    def push(self, samples) -> "SSA":
+        """Slide the window by len(samples) new samples (ssa_amd_push_samples)."""
+        a = np.ascontiguousarray(samples, dtype=np.float64)
+        if self._l.ssa_amd_push_samples(ctypes.byref(self._ctx), _dp(a), len(a)) != 0:
+            raise RuntimeError("push_samples failed")
+        self._x = np.concatenate([self._x[len(a):], a]) if len(a) < self.N else a[-self.N:].copy()
+        self._decomposed = False
         return self
 
     def decompose(self, k: int, method: str = "randomized", **kwargs) -> "SSA":

@@ -173,6 +173,42 @@ def test_streaming_ticks_match_oracle():
         assert np.abs(dev.reconstruct([0]) - orc.reconstruct([0])).max() / rms(w) < 1e-9, t
 
 
+def test_push_samples_slides_the_window():
+    """ssa_amd_push_samples(m) leaves the context in the state
+    update_signal(window moved by m) gives: with the same Philox key and call
+    counter, the decompositions and reconstructions are bit-identical, and
+    they match the oracle on the moved window."""
+    S = _ssa()
+    N, L, k, p = 1500, 375, 16, 10
+    long = workloads.single(N + 60, seed=9)
+    a = S.SSA(long[:N], L, q=2, seed=5)
+    b = S.SSA(long[:N], L, q=2, seed=5)
+    K = N - L + 1
+    off = 0
+    for t, m in enumerate((1, 7, 32)):
+        off += m
+        a.push(long[off + N - m:off + N])
+        b.update_signal(long[off:off + N])
+        a.decompose(k, oversampling=p)
+        b.decompose(k, oversampling=p)
+        assert np.array_equal(a.singular_values, b.singular_values), m
+        assert np.array_equal(a.reconstruct([0, 1]), b.reconstruct([0, 1])), m
+        w = long[off:off + N]
+        orc = O.OracleSSA(w, L).decompose(k, p, 2, O.philox_gaussian(5, t, K, k + p))
+        assert np.max(np.abs(a.singular_values - orc.sigma) / orc.sigma) < 1e-10, m
+    # a push of the whole length replaces the window; 0 and N + 1 are refused
+    a.push(long[:N])
+    b.update_signal(long[:N])
+    a.decompose(k, oversampling=p)
+    b.decompose(k, oversampling=p)
+    assert np.array_equal(a.singular_values, b.singular_values)
+    import ctypes
+    one = np.zeros(N + 1)
+    ptr = one.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert a._l.ssa_amd_push_samples(ctypes.byref(a._ctx), ptr, 0) == -1
+    assert a._l.ssa_amd_push_samples(ctypes.byref(a._ctx), ptr, N + 1) == -1
+
+
 def test_reconstruct_many_matches_single():
     x, dev, orc = _pair(2000, 500, 12, 10, 2, seed=3)
     groups = [[i] for i in range(12)] + [[1, 2], [0, 3, 5]]
