@@ -133,11 +133,24 @@ def _oracle_ssa(args):
     return len(x)
 
 
-def _oracle_worker(w, workers, xs, Ls, ks, budget_s):
+def _ref_ssa(args):
+    """One SSA of series i by the reference's own dependency-free build
+    (MKL/ssa_opt_ref.h compiled from source): its decompose_randomized (q = 0,
+    the reference has no power iterations; p = 10, its xorshift test matrix)
+    and reconstruction of all k components."""
+    x, L, k = args
+    import oracle as O
+    s = O.RefSSA(np.asarray(x, dtype=np.float64), L).decompose_randomized(k, P_OVERSAMPLE)
+    s.reconstruct(list(range(s.k)))
+    return len(x)
+
+
+def _oracle_worker(w, workers, xs, Ls, ks, budget_s, fn=None):
+    fn = fn or _oracle_ssa
     done = nsum = 0
     t0 = time.perf_counter()
     for i in range(w, len(xs), workers):
-        nsum += _oracle_ssa((xs[i], int(Ls[i]), int(ks[i])))
+        nsum += fn((xs[i], int(Ls[i]), int(ks[i])))
         done += 1
         if time.perf_counter() - t0 > budget_s:
             break
@@ -163,6 +176,27 @@ def cpu_baseline(xs, Ls, ks, budget_s, workers):
                       f"(series i to process i mod {workers}), decompose q=2 p=10 + reconstruct all k, FP64, "
                       f"{el:.1f} s; one thread: {one:.2f} SSA/s over the first {d1} series ({n1} samples)",
             "value_1_core": round(one, 2)}
+
+
+def cpu_reference(xs, Ls, ks, budget_s, workers):
+    """The reference's own CPU code (MKL/ssa_opt_ref.h, built from its source
+    into oracle/_ref) timed the same way: q = 0 decompose + reconstruct all k,
+    FP64.  None when the build is absent."""
+    import multiprocessing as mp
+    import oracle as O
+    if not O.ref_available():
+        return None
+    d1, n1, t1 = _oracle_worker(0, 1, xs, Ls, ks, budget_s, _ref_ssa)
+    ctx = mp.get_context("fork")
+    with ctx.Pool(workers) as pool:
+        res = pool.starmap(_oracle_worker, [(w, workers, xs, Ls, ks, budget_s, _ref_ssa) for w in range(workers)])
+    done_all = sum(r[0] for r in res)
+    el = max(r[2] for r in res)
+    return {"value": round(done_all / el, 2), "unit": "SSA/s", "cores": workers, "kind": "reference",
+            "sample": f"{done_all} series of the same batch over {workers} processes, the reference's "
+                      f"ssa_opt_ref_decompose_randomized (q=0, p=10) + reconstruct all k, FP64, {el:.1f} s; "
+                      f"one thread: {d1 / t1:.2f} SSA/s over the first {d1} series ({n1} samples)",
+            "value_1_core": round(d1 / t1, 2)}
 
 
 SERVE_BUDGET_GIB = 16
@@ -284,8 +318,12 @@ def main():
     # CPU baseline first: its worker processes are forked before any GPU call
     workers = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
     cpu = None if args.no_cpu else cpu_baseline(xs, Ls, ks, args.cpu_seconds, workers)
+    cpu_ref = None if args.no_cpu else cpu_reference(xs, Ls, ks, args.cpu_seconds, workers)
     if cpu:
         print(f"[bench] cpu baseline: {cpu['value']} SSA/s on {workers} cores, {cpu['value_1_core']} on one", flush=True)
+    if cpu_ref:
+        print(f"[bench] reference cpu (ssa_opt_ref.h, q=0): {cpu_ref['value']} SSA/s on {workers} cores, "
+              f"{cpu_ref['value_1_core']} on one", flush=True)
 
     global ssa_amd
     import ssa_amd as _ssa_amd
@@ -357,6 +395,7 @@ def main():
     out["serving"] = serving_out
     out["single_series"] = configs_out
     out["cpu_baseline"] = cpu
+    out["cpu_reference"] = cpu_ref
     print(json.dumps(out), flush=True)
 
 
