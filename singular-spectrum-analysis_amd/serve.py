@@ -236,7 +236,15 @@ class Server:
                 answer(refused)
                 finished.extend(refused)
                 if wave:
-                    self._launch(wave)
+                    try:
+                        self._launch(wave)
+                    except RuntimeError:
+                        # the device could not take the wave (allocation or set-up failed):
+                        # its requests are answered as failed and serving goes on
+                        for r in wave:
+                            r.status, r.done = wire.STATUS_FAILED, time.perf_counter() - t0
+                        answer(wave)
+                        finished.extend(wave)
                 continue
             if self.waves:
                 reqs = self.waves[0][1]

@@ -173,6 +173,30 @@ def test_serve_stream_answers_every_frame():
     assert max(srv.wave_sizes) <= 4
 
 
+def test_serve_stream_answers_a_wave_the_device_cannot_take():
+    """A wave whose launch fails is answered with status -1, and the
+    requests after it are still served."""
+    srv = StandIn.make(h=5, max_wave=1, inflight=1)
+    launch = srv._launch
+
+    def flaky(reqs):
+        if reqs[0].rid == 1:
+            raise RuntimeError("batch creation failed")
+        launch(reqs)
+
+    srv._launch = flaky
+    frames = b"".join(wire.encode_request(i, np.ones(40, np.float32) * (i + 1), 10, 3, 1, 5) for i in range(3))
+    out = io.BytesIO()
+    done, err = srv.serve_stream(io.BytesIO(frames), out)
+    assert err is None and len(done) == 3
+    out.seek(0)
+    got = {}
+    for _ in range(3):
+        rid, st, o = wire.read_response(out)
+        got[rid] = (st, o.size)
+    assert got == {0: (0, 45), 1: (wire.STATUS_FAILED, 0), 2: (0, 45)}
+
+
 def test_serve_stream_stops_at_a_broken_frame():
     good = b"".join(wire.encode_request(i, np.ones(50, np.float32) * (i + 1), 10, 3, 1, 5) for i in range(3))
     srv = StandIn.make(h=5, max_wave=8, inflight=1)
