@@ -276,19 +276,25 @@ def single_configs(reps=3):
     # one prepared context; per tick update_signal on the window moved by one
     # sample, decompose_randomized (k = 50, p = 10, q = 2) and the trend
     # reconstruction, timed end to end on the host (plain FP64)
-    for N, L in ((1000, 250), (20000, 5000)):
-        xs_long = workloads.single(N + 101, seed=7)
-        s = ssa_amd.SSA(xs_long[:N], L, q=Q_POWER, precision=0, seed=14)
-        s.decompose(50, oversampling=P_OVERSAMPLE)
-        lat = []
-        for tick in range(1, 101):
-            a = time.perf_counter()
-            s.update_signal(xs_long[tick:tick + N])
+    # "lean": the same ticks with the host U/V mirrors off (ssa_amd_set_mirror),
+    # so a decomposition does not copy U and V to the host
+    for lean in (False, True):
+        for N, L in ((1000, 250), (20000, 5000)):
+            xs_long = workloads.single(N + 101, seed=7)
+            s = ssa_amd.SSA(xs_long[:N], L, q=Q_POWER, precision=0, seed=14)
+            if lean:
+                s.set_mirror(False)
             s.decompose(50, oversampling=P_OVERSAMPLE)
-            s.reconstruct([0])
-            lat.append(time.perf_counter() - a)
-        out[f"stream_N{N}_p50_ms"] = round(1e3 * float(np.percentile(lat, 50)), 3)
-        out[f"stream_N{N}_p99_ms"] = round(1e3 * float(np.percentile(lat, 99)), 3)
+            lat = []
+            for tick in range(1, 101):
+                a = time.perf_counter()
+                s.update_signal(xs_long[tick:tick + N])
+                s.decompose(50, oversampling=P_OVERSAMPLE)
+                s.reconstruct([0])
+                lat.append(time.perf_counter() - a)
+            tag = "stream_lean" if lean else "stream"
+            out[f"{tag}_N{N}_p50_ms"] = round(1e3 * float(np.percentile(lat, 50)), 3)
+            out[f"{tag}_N{N}_p99_ms"] = round(1e3 * float(np.percentile(lat, 99)), 3)
     return out
 
 

@@ -250,6 +250,13 @@ int ssa_amd_plan_info(const SSA_Opt *ssa, int *out6);
  * ssa_opt_update_signal (MKL/ssa_opt.h:1545-1558) with the shifted window,
  * without the caller holding the window.  Clears decomposed. */
 int ssa_amd_push_samples(SSA_Opt *ssa, const ssa_real *x_new, int m);
+/* host mirrors of U and V: on (the default, as the reference's fields) every
+ * decomposition copies them to the host; off, a decomposition leaves them
+ * stale and they are copied when host code needs them (compute_lrf,
+ * parestimate), by ssa_amd_sync_mirrors, or when mirroring is turned on
+ * again.  sigma, eigenvalues and total_variance are always current. */
+int ssa_amd_set_mirror(SSA_Opt *ssa, int on);
+int ssa_amd_sync_mirrors(const SSA_Opt *ssa);
 
 /* autotuner: time the block matvec at every FFT length 2^6..2^13 on this
  * device for the precision (0 plain FP64, 1 mixed) and plan every context

@@ -279,6 +279,7 @@ void ssa_opt_parestimate_free(SSA_ParEstimate *result)
 int ssa_opt_parestimate(const SSA_Opt *ssa, const int *group, int n_group, SSA_ParEstimate *result)
 {
     if (!ssa || !ssa->decomposed || !result || !ssa->U) return -1;
+    if (ssa_amd_sync_mirrors(ssa)) return -1;
     const int k = (n_group > 0 && group) ? n_group : ssa->n_components;
     const int L = ssa->L;
     if (k < 1 || k > ssa->n_components || L < 3) return -1;

@@ -27,6 +27,8 @@ typedef struct {
     double *hostblk;           /* host views: fft_x and ws_* */
     int injected;              /* width b of an Omega injected for the next decomposition (0: none) */
     int pending_q;             /* q override of the next decomposition (-1: ctx q) */
+    int no_mirror;             /* 1: decompositions leave the host U/V mirrors stale */
+    int uv_stale;              /* the host U/V mirrors lag the device's */
 } ssa_ctx;
 
 static int imin(int a, int b) { return a < b ? a : b; }
@@ -149,6 +151,7 @@ static int prepare_caps(SSA_Opt *ssa, int kp, int k)
     ssa->sigma = (double *)calloc(k, sizeof(double));
     ssa->eigenvalues = (double *)calloc(k, sizeof(double));
     if (!ssa->U || !ssa->V || !ssa->sigma || !ssa->eigenvalues) return -1;
+    c->uv_stale = 0;
     ssa->n_components = k;
     ssa->prepared_kp = kp;
     ssa->prepared = true;
@@ -210,7 +213,11 @@ static int run_decomposition(SSA_Opt *ssa, int kp, int k, int q)
     ssa->calls++;
     if (rc) return -1;
     double tv = 0;
-    if (ssa_engine_results(c->eng, 0, ssa->sigma, ssa->eigenvalues, &tv, ssa->U, ssa->V, NULL) != k) return -1;
+    const int mirror = !c->no_mirror;
+    if (ssa_engine_results(c->eng, 0, ssa->sigma, ssa->eigenvalues, &tv, mirror ? ssa->U : NULL,
+                           mirror ? ssa->V : NULL, NULL) != k)
+        return -1;
+    c->uv_stale = !mirror;
     ssa->total_variance = tv;
     ssa->n_components = k;
     ssa->decomposed = true;
@@ -504,6 +511,7 @@ ssa_real ssa_opt_get_total_variance(const SSA_Opt *ssa)                         
 int ssa_opt_compute_lrf(const SSA_Opt *ssa, const int *group, int n_group, SSA_LRF *lrf)
 {
     if (!ssa || !ssa->decomposed || !group || !lrf || n_group < 1) return -1;
+    if (ssa_amd_sync_mirrors(ssa)) return -1;
     int L = ssa->L;
     memset(lrf, 0, sizeof(SSA_LRF));
     lrf->L = L;
@@ -966,6 +974,28 @@ void ssa_mkl_disable_affinity(void) {}
 /* ------------------------------------------------------------------------ */
 /* extensions                                                                */
 /* ------------------------------------------------------------------------ */
+
+/* The host U/V mirrors are refreshed by every decomposition unless
+ * mirroring is off; then they are copied when host code needs them (LRF,
+ * ESPRIT) or the caller asks. */
+int ssa_amd_set_mirror(SSA_Opt *ssa, int on)
+{
+    if (!ssa || !ssa->initialized) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    c->no_mirror = !on;
+    return on ? ssa_amd_sync_mirrors(ssa) : 0;
+}
+
+int ssa_amd_sync_mirrors(const SSA_Opt *ssa)
+{
+    if (!ssa || !ssa->initialized) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    if (!c->uv_stale) return 0;
+    if (!ssa->decomposed) return -1;
+    if (ssa_engine_results(c->eng, 0, NULL, NULL, NULL, ssa->U, ssa->V, NULL) != ssa->n_components) return -1;
+    c->uv_stale = 0;
+    return 0;
+}
 
 int ssa_amd_set_options(SSA_Opt *ssa, int q, int precision, unsigned long long seed)
 {

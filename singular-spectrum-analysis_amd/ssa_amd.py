@@ -113,6 +113,8 @@ SIGNATURES = [
     ("ssa_amd_reconstruct_many", _I, [_P, c_int_p, c_int_p, _I, c_double_p]),
     ("ssa_amd_plan_info", _I, [_P, c_int_p]),
     ("ssa_amd_push_samples", _I, [_P, c_double_p, _I]),
+    ("ssa_amd_set_mirror", _I, [_P, _I]),
+    ("ssa_amd_sync_mirrors", _I, [_P]),
     ("ssa_amd_batch_create", _VP, [_I, c_int_p, c_int_p, c_int_p, _I, _I, _I, _U64]),
     ("ssa_amd_batch_set_signals_f32", _I, [_VP, c_float_p, c_ll_p]),
     ("ssa_amd_batch_set_key", _I, [_VP, _I, _U64]),
@@ -309,6 +311,12 @@ class SSA:
         if self._l.ssa_opt_update_signal(ctypes.byref(self._ctx), _dp(new_x)) != 0:
             raise RuntimeError("update_signal() failed")
         self._x = new_x
+        return self
+
+    def set_mirror(self, on: bool) -> "SSA":
+        """Host U/V mirrors refreshed by every decomposition (on) or on demand (off)."""
+        if self._l.ssa_amd_set_mirror(ctypes.byref(self._ctx), int(bool(on))) != 0:
+            raise RuntimeError("set_mirror failed")
         return self
 
     def push(self, samples) -> "SSA":

@@ -209,6 +209,62 @@ def test_push_samples_slides_the_window():
     assert a._l.ssa_amd_push_samples(ctypes.byref(a._ctx), ptr, N + 1) == -1
 
 
+def test_lazy_mirrors():
+    """With the host U/V mirrors off, decompositions give the same singular
+    values, the host LRF and ESPRIT (which read U) refresh the mirror first,
+    and an explicit refresh gives the U and V a mirrored context holds."""
+    import ctypes
+    S = _ssa()
+    x = workloads.single(1500, seed=12)
+    L, k, p = 375, 12, 10
+    on = S.SSA(x, L, q=2, seed=8)
+    off = S.SSA(x, L, q=2, seed=8).set_mirror(False)
+    on.decompose(k, oversampling=p)
+    off.decompose(k, oversampling=p)
+    assert np.array_equal(on.singular_values, off.singular_values)
+    g = [0, 1, 2]
+    r1, R1, n1 = on.lrf(g)
+    r2, R2, n2 = off.lrf(g)                     # ssa_opt_compute_lrf refreshes U first
+    assert r1 == r2 == 0 and np.array_equal(R1, R2) and n1 == n2
+    assert np.array_equal(on.forecast(g, 20), off.forecast(g, 20))
+    # a second decomposition leaves the mirror stale again; parestimate refreshes it
+    on.decompose(k, oversampling=p)
+    off.decompose(k, oversampling=p)
+    pa, pb = on.parestimate(g), off.parestimate(g)
+    assert np.array_equal(np.asarray(pa["periods"]), np.asarray(pb["periods"]))
+    off.decompose(k, oversampling=p)
+    on.decompose(k, oversampling=p)
+    assert off._l.ssa_amd_sync_mirrors(ctypes.byref(off._ctx)) == 0
+    oU, oV = _field_offsets()
+
+    def field(s, off_, n):
+        addr = ctypes.c_void_p.from_buffer(s._ctx, off_).value
+        return np.ctypeslib.as_array((ctypes.c_double * n).from_address(addr)).copy()
+
+    assert np.array_equal(field(on, oU, L * k), field(off, oU, L * k))
+    off.set_mirror(True)
+    off.decompose(k, oversampling=p)
+    on.decompose(k, oversampling=p)
+    m = (len(x) - L + 1) * k
+    assert np.array_equal(field(on, oV, m), field(off, oV, m))
+
+
+def _field_offsets():
+    """offsetof(SSA_Opt, U) and offsetof(SSA_Opt, V), from the public header."""
+    import os
+    import subprocess
+    import tempfile
+    hdr = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "include")
+    prog = '#include <stdio.h>\n#include <stddef.h>\n#include "ssa_opt.h"\n' \
+           'int main(void) { printf("%zu %zu", offsetof(SSA_Opt, U), offsetof(SSA_Opt, V)); return 0; }\n'
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "t.c"), "w").write(prog)
+        subprocess.run(["gcc", "-std=c11", "-I", hdr, os.path.join(d, "t.c"), "-o", os.path.join(d, "t")],
+                       check=True)
+        out = subprocess.run([os.path.join(d, "t")], capture_output=True, text=True, check=True).stdout.split()
+    return int(out[0]), int(out[1])
+
+
 def test_reconstruct_many_matches_single():
     x, dev, orc = _pair(2000, 500, 12, 10, 2, seed=3)
     groups = [[i] for i in range(12)] + [[1, 2], [0, 3, 5]]
