@@ -443,18 +443,18 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
 #pragma unroll
     for (int I = 0; I < NB; I++) cs[I] = 0.0;
 
+    // column blocks this workgroup's upper blocks use (all without SPLIT)
+    unsigned need = ~0u;
+    if constexpr (SPLIT) {
+        need = 0;
+        int m = 0;
+#pragma unroll
+        for (int I = 0; I < NB; I++)
+#pragma unroll
+            for (int J = I; J < NB; J++, m++)
+                if (m % (int)gridDim.y == (int)blockIdx.y) need |= (1u << I) | (1u << J);
+    }
     if constexpr (!USE_M) {
-        // column blocks this workgroup's upper blocks use (all without SPLIT)
-        unsigned need = ~0u;
-        if constexpr (SPLIT) {
-            need = 0;
-            int m = 0;
-#pragma unroll
-            for (int I = 0; I < NB; I++)
-#pragma unroll
-                for (int J = I; J < NB; J++, m++)
-                    if (m % (int)gridDim.y == (int)blockIdx.y) need |= (1u << I) | (1u << J);
-        }
         // 16-row slabs, slab j of the item to wave j % 4
         T cur[NB][4], nxt[NB][4];
         auto load = [&](int r0, T (&x)[NB][4]) {
@@ -513,6 +513,7 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
 #pragma unroll
             for (int cb = 0; cb < NB; cb++) {
                 t[cb] = d4v{0, 0, 0, 0};
+                if (!((need >> cb) & 1u)) continue;   // column block used by none of this workgroup's blocks
                 // M upper triangular: rows k < 16 (cb + 1) only
 #pragma unroll
                 for (int kk = 0; kk < 4 * (cb + 1); kk++)
