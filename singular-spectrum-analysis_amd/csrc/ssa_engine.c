@@ -61,6 +61,13 @@ struct ssa_engine {
     /* groups (int4 descriptors followed by component lists) */
     int *dgroups;
     long long dgroups_cap;
+    /* host copies of what the dynamic item region and the group buffer last
+     * received: a call that would upload the same content skips the copy and
+     * its synchronisation (repeated reconstructions of one group) */
+    ssa_item *dyn_last;
+    long long dyn_last_n, dyn_last_cap, dyn_last_from;
+    int *grp_last;
+    long long grp_last_n, grp_last_cap;
     int step_groups_ready;
     /* pinned host staging */
     void *hpin;
@@ -203,11 +210,28 @@ static int items_upload(ssa_engine *e, long long from)
         e->ditems_cap = cap;
         from = 0;
     }
+    if (from == 0) e->dyn_last_n = -1;   /* the whole list moves: forget the dynamic region */
     if (n > from) {
-        DEVCHK(pin_reserve(e, sizeof(ssa_item) * (n - from)));
-        memcpy(e->hpin, e->hitems + from, sizeof(ssa_item) * (n - from));
-        HIPCHK(hipMemcpyAsync(e->ditems + from, e->hpin, sizeof(ssa_item) * (n - from), hipMemcpyHostToDevice, e->st));
+        const long long m = n - from;
+        if (from > 0 && from == e->dyn_last_from && m == e->dyn_last_n &&
+            !memcmp(e->dyn_last, e->hitems + from, sizeof(ssa_item) * m))
+            return 0;
+        DEVCHK(pin_reserve(e, sizeof(ssa_item) * m));
+        memcpy(e->hpin, e->hitems + from, sizeof(ssa_item) * m);
+        HIPCHK(hipMemcpyAsync(e->ditems + from, e->hpin, sizeof(ssa_item) * m, hipMemcpyHostToDevice, e->st));
         HIPCHK(hipStreamSynchronize(e->st));
+        if (from > 0) {
+            e->dyn_last_n = -1;
+            if (m > e->dyn_last_cap) {
+                ssa_item *p = (ssa_item *)realloc(e->dyn_last, sizeof(ssa_item) * m);
+                if (!p) return 0;       /* the cache is an optimisation only */
+                e->dyn_last = p;
+                e->dyn_last_cap = m;
+            }
+            memcpy(e->dyn_last, e->hitems + from, sizeof(ssa_item) * m);
+            e->dyn_last_n = m;
+            e->dyn_last_from = from;
+        }
     }
     return 0;
 }
@@ -404,6 +428,7 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
     e->prec = prec;
     e->esz = prec == SSA_PREC_F64 ? 8 : 4;
     e->q_hint = q_hint;
+    e->dyn_last_n = e->grp_last_n = -1;
     if (hipStreamCreate(&e->st) != hipSuccess) goto fail;
     if (!g_tw[prec]) {
         if (ssa_dev_twiddles(prec, &g_tw[prec], e->st) != 0) goto fail;
@@ -447,6 +472,8 @@ void ssa_engine_destroy(ssa_engine *e)
     if (e->dd) hipFree(e->dd);
     if (e->ditems) hipFree(e->ditems);
     if (e->dgroups) hipFree(e->dgroups);
+    free(e->dyn_last);
+    free(e->grp_last);
     if (e->hpin) hipHostFree(e->hpin);
     for (int i = 0; i < 2 * e->prof_cap; i++) hipEventDestroy(e->prof_ev[i]);
     free(e->prof_ev); free(e->prof_phase); free(e->prof_lgn);
@@ -959,13 +986,27 @@ static int groups_upload(ssa_engine *e, const int *gdesc4, int ng, const int *gl
         long long cap = need + 1024;
         HIPCHK(hipMalloc((void **)&e->dgroups, sizeof(int) * cap));
         e->dgroups_cap = cap;
+        e->grp_last_n = -1;
     }
+    const long long m = 4LL * ng + nl;
+    if (m == e->grp_last_n && !memcmp(e->grp_last, gdesc4, sizeof(int) * 4 * ng) &&
+        !memcmp(e->grp_last + 4 * ng, glist, sizeof(int) * nl))
+        return 0;
     DEVCHK(pin_reserve(e, sizeof(int) * need));
     int *h = (int *)e->hpin;
     memcpy(h, gdesc4, sizeof(int) * 4 * ng);
     memcpy(h + 4 * ng, glist, sizeof(int) * nl);
-    HIPCHK(hipMemcpyAsync(e->dgroups, h, sizeof(int) * (4 * ng + nl), hipMemcpyHostToDevice, e->st));
+    HIPCHK(hipMemcpyAsync(e->dgroups, h, sizeof(int) * m, hipMemcpyHostToDevice, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
+    e->grp_last_n = -1;
+    if (m > e->grp_last_cap) {
+        int *p = (int *)realloc(e->grp_last, sizeof(int) * m);
+        if (!p) return 0;               /* the cache is an optimisation only */
+        e->grp_last = p;
+        e->grp_last_cap = m;
+    }
+    memcpy(e->grp_last, h, sizeof(int) * m);
+    e->grp_last_n = m;
     return 0;
 }
 
