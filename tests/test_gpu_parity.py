@@ -329,6 +329,32 @@ def test_mixed_batch_against_oracle():
     assert worst_r < 1e-5
 
 
+@pytest.mark.parametrize("k,p", [(70, 10), (60, 10)])
+def test_wide_blocks_plain_parity(k, p):
+    """Block widths above 64 (b = 80 = the widest, and b = 70) take the
+    five-column-block kernels (Gram, apply, Jacobi) and the LDS Cholesky."""
+    x, dev, orc = _pair(2000, 500, k, p, 2, seed=k)
+    assert np.max(np.abs(dev.singular_values - orc.sigma) / orc.sigma) < 1e-10
+    r = rms(x)
+    for g in ([0], [1, 2], list(range(k))):
+        assert np.abs(dev.reconstruct(g) - orc.reconstruct(g)).max() / r < 1e-9, g
+
+
+def test_mssa_parity_cfg4_width():
+    """The MSSA benchmark's width: k = 64, p = 10 (b = 74) on four channels."""
+    S = _ssa()
+    X = workloads.mssa_channels(M=4, N=1200, seed=5)
+    L, k, p, q = 300, 64, 10, 2
+    m = S.MSSA(X, L, q=q)
+    m.decompose(k, oversampling=p)
+    K = 1200 - L + 1
+    o = O.OracleMSSA(X, L).decompose(k, p, q, O.philox_gaussian(42, 0, K, k + p))
+    for ch in range(4):
+        assert np.abs(m.reconstruct(ch, [0, 1, 2]) - o.reconstruct(ch, [0, 1, 2])).max() / rms(X[ch]) < 1e-9
+        assert np.abs(m.reconstruct(ch, list(range(k))) - o.reconstruct(ch, list(range(k)))).max() / rms(X[ch]) < 1e-9
+    assert np.allclose(m.series_contributions(), o.contributions(), atol=1e-10)
+
+
 def test_mssa_parity():
     S = _ssa()
     X = workloads.mssa_channels(M=4, N=1200, seed=5)
