@@ -340,6 +340,21 @@ def test_wide_blocks_plain_parity(k, p):
         assert np.abs(dev.reconstruct(g) - orc.reconstruct(g)).max() / r < 1e-9, g
 
 
+def test_wide_blocks_mixed_error():
+    """The mixed form at b = 80 (FP32 panels, FP32-MFMA apply, FP64 small
+    solves) with the tolerances of the mixed form at the benchmark widths:
+    singular values within 1e-5 relative, reconstructions within 1e-5
+    relative (2-norm) and pointwise within 1e-3 of RMS(x) of the FP64
+    oracle."""
+    x, dev, orc = _pair(2000, 500, 70, 10, 2, seed=71, precision=1)
+    assert np.max(np.abs(dev.singular_values - orc.sigma) / orc.sigma) < 1e-5
+    r = rms(x)
+    for g in ([0], [1, 2], list(range(70))):
+        a, b = dev.reconstruct(g), orc.reconstruct(g)
+        assert rel2(a, b) < 1e-5, g
+        assert np.abs(a - b).max() / r < 1e-3, g
+
+
 def test_mssa_parity_cfg4_width():
     """The MSSA benchmark's width: k = 64, p = 10 (b = 74) on four channels."""
     S = _ssa()
