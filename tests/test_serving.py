@@ -25,6 +25,20 @@ def test_alone_vs_in_the_mix_bit_identical():
         assert m.status == alone.status
 
 
+def test_alone_vs_a_wave_of_700_bit_identical():
+    """A wave of 700 short series launches its Gram, apply and test-matrix
+    kernels one workgroup per row tile; a request served alone deals each
+    tile over several workgroups.  Every output row and every Gram block is
+    formed in the same order either way, so the results are bit-identical."""
+    S = _serve()
+    reqs = S.synthetic_requests(700, rate=np.inf, n_lo=500, n_hi=1500, seed=35)
+    big = {r.rid: r for r in S.Server(max_wave=700, inflight=1).run(reqs)}
+    for r in S.synthetic_requests(700, rate=np.inf, n_lo=500, n_hi=1500, seed=35)[::97]:
+        alone = S.Server(max_wave=1, inflight=1).run([r])[0]
+        assert np.array_equal(big[r.rid].out, alone.out), r.rid
+        assert big[r.rid].status == alone.status
+
+
 def test_open_loop_stream_completes_with_forecasts():
     S = _serve()
     reqs = S.synthetic_requests(40, rate=400.0, n_lo=500, n_hi=8000, seed=32)
