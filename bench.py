@@ -181,7 +181,9 @@ def cpu_baseline(xs, Ls, ks, budget_s, workers):
 def cpu_reference(xs, Ls, ks, budget_s, workers):
     """The reference's own CPU code (MKL/ssa_opt_ref.h, built from its source
     into oracle/_ref) timed the same way: q = 0 decompose + reconstruct all k,
-    FP64.  None when the build is absent."""
+    FP64.  Its randomized SVD returns a wrong basis (a sign error in its
+    Householder QR, MKL/ssa_opt_ref.h:441-519); it is timed for the work it
+    does.  None when the build is absent."""
     import multiprocessing as mp
     import oracle as O
     if not O.ref_available():
