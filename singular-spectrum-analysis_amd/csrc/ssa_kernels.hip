@@ -90,9 +90,10 @@ __device__ __forceinline__ void gauss_quad(uint64_t seed, uint64_t call, int row
     for (int h = 0; h < 2; h++) {
         const double u1 = (double)c[2 * h] * 0x1p-32, u2 = (double)c[2 * h + 1] * 0x1p-32;
         const double rad = sqrt(-2.0 * log(1.0 - u1));
-        const double ang = 6.283185307179586476925286766559 * u2;
-        g[2 * h] = rad * cos(ang);
-        g[2 * h + 1] = rad * sin(ang);
+        double sn, cs;
+        sincos(6.283185307179586476925286766559 * u2, &sn, &cs);
+        g[2 * h] = rad * cs;
+        g[2 * h + 1] = rad * sn;
     }
 }
 
