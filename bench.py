@@ -235,7 +235,9 @@ def serving(seconds):
 
 
 def single_configs(reps=3):
-    """cfg2: one series N = 20000, L = 5000, k = 50, p = 10, q = 2:
+    """cfg1: one series N = 1000, L = 250, k = 50: decompose + the
+    reconstructions [0] and [1, 2] + W-correlation, plain and mixed.
+    cfg2: one series N = 20000, L = 5000, k = 50, p = 10, q = 2:
     decompose + the 50 single-component reconstructions + W-correlation
     (median of `reps`), plain and mixed, and the singular-value error of both
     forms against a dense SVD of the explicit Hankel matrix at N = 4000,
@@ -243,6 +245,21 @@ def single_configs(reps=3):
     decompose + per-channel reconstruction of components 0..8.  Streaming
     tick latency (p50, p99) at N = 1000 and N = 20000."""
     out = {}
+    # cfg1: N = 1000, L = 250, k = 50: decompose + reconstructions [0] and
+    # [1, 2] + the 50 x 50 W-correlation
+    x1 = workloads.single(1000, seed=1)
+    for prec, name in ((0, "plain"), (1, "mixed")):
+        s = ssa_amd.SSA(x1, 250, q=Q_POWER, precision=prec, seed=11)
+        t = []
+        for r in range(reps + 3):
+            a = time.perf_counter()
+            s.decompose(50, oversampling=P_OVERSAMPLE)
+            s.reconstruct([0])
+            s.reconstruct([1, 2])
+            s.wcorr_matrix()
+            if r >= 2:
+                t.append(time.perf_counter() - a)
+        out[f"cfg1_{name}_ms"] = round(1e3 * float(np.median(t)), 3)
     x = workloads.single(20000, seed=2)
     groups = [[i] for i in range(50)]
     for prec, name in ((0, "plain"), (1, "mixed")):
