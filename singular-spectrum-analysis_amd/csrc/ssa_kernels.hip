@@ -810,13 +810,24 @@ __global__ __launch_bounds__(256) void k_trmm(const ssa_desc *D, const ssa_item 
         const int I = m / NB, J = m - I * NB;
         d4v acc = {0, 0, 0, 0};
         if (I <= J) {
-            for (int k0 = 16 * I; k0 < 16 * J + 16; k0 += 4) {
+            // every k step's operands loaded before the first product (steps
+            // beyond 16 J + 16 carry zeros and add exact zeros)
+            constexpr int KS = 4 * NB;
+            double av[KS], bv[KS];
+#pragma unroll
+            for (int t = 0; t < KS; t++) {
+                const int k0 = 16 * I + 4 * t;
+                const bool in = k0 < 16 * J + 16;
                 const int ra = 16 * I + cl, ka = k0 + kq;     // A[ra][ka]
                 const int kb = k0 + kq, cb = 16 * J + cl;     // B[kb][cb]
-                const double av = (ra < b && ka < b) ? A[(long long)ka * SSA_BMAX + ra] : 0.0;
-                const double bv = (kb < b && cb < b) ? B[(long long)cb * SSA_BMAX + kb] : 0.0;
-                acc = mfma64(av, bv, acc);
+                const bool ia = in && ra < b && ka < b, ib = in && kb < b && cb < b;
+                const double x = A[ia ? (long long)ka * SSA_BMAX + ra : 0];
+                const double y = B[ib ? (long long)cb * SSA_BMAX + kb : 0];
+                av[t] = ia ? x : 0.0;
+                bv[t] = ib ? y : 0.0;
             }
+#pragma unroll
+            for (int t = 0; t < KS; t++) acc = mfma64(av[t], bv[t], acc);
         }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
