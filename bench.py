@@ -79,6 +79,18 @@ def measured_traffic(kernel):
         float(entry["write_kib"]) * 1024 * float(entry["write_scale"])
 
 
+def measured_valu_issue(kernel):
+    """Fraction of the VALU issue rate `kernel` reaches on this workload, from
+    the rocprofv3 SQ_INSTS_VALU / GRBM_GUI_ACTIVE pass recorded in
+    profiles/traffic.json, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            entry = json.load(f).get(kernel) or {}
+    except (OSError, ValueError):
+        return None
+    return entry.get("valu_issue")
+
+
 def roofline_of(batch, plans, Ns, Ls, s):
     """Live HIP-event timing of every launch of one eager step; the dominant
     kernel's algorithmic bytes per launch over its average duration."""
@@ -114,7 +126,9 @@ def roofline_of(batch, plans, Ns, Ls, s):
                      "traffic": None if traffic is None else round(traffic),
                      "kernel": kernel,
                      "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_ms, 4),
-                     "series_in_bucket": len(sel), "launches_timed": len(v)},
+                     "series_in_bucket": len(sel), "launches_timed": len(v),
+                     # the kernel is bound by its FFT arithmetic: the VALU issue rate it reaches
+                     "valu_issue": measured_valu_issue(kernel)},
         "phase_ms": {k: round(v, 3) for k, v in sorted(phase_tot.items(), key=lambda kv: -kv[1])},
         "profiled_step_ms": round(total, 3),
     }
