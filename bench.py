@@ -63,6 +63,19 @@ def timed_steps(batch, steps, warmup):
     return total, np.array(lat)
 
 
+def step_check(batch):
+    """After the timed steps: every series' device status word and whether
+    its output is finite (the parity of these outputs with the oracle is
+    tests/test_gpu_wide.py's)."""
+    outs, status = batch.outputs()
+    bad = int(np.count_nonzero(status))
+    nonfinite = int(sum(1 for o in outs if not np.all(np.isfinite(o))))
+    if bad or nonfinite:
+        print(f"[bench] WARNING: {bad} series with a failure status, {nonfinite} with non-finite output",
+              file=sys.stderr, flush=True)
+    return {"series": len(outs), "status_nonzero": bad, "nonfinite": nonfinite}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` on this workload, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE passes recorded in profiles/traffic.json (each
@@ -305,6 +318,26 @@ def single_configs(reps=3):
             if r:
                 t.append(time.perf_counter() - a)
         out[f"cfg4_mssa_{name}_ms"] = round(1e3 * float(np.median(t)), 2)
+    # the reference's one published latency point (MKL/ssa_opt_r2c.h:9-12: N = 500,
+    # k = 15, randomized decompose ~210 us, decompose + reconstruct ~250 us on an
+    # i9-14900KF; its harness MKL/test/ssa_block_test.c:426-436 uses L = 375, p = 10):
+    # the same calls on one prepared plain FP64 context, p50 over 200 calls, at
+    # q = 0 (the reference's algorithm) and q = 2 (this library's default)
+    x5 = workloads.single(500, seed=15)
+    for q in (0, 2):
+        s = ssa_amd.SSA(x5, 375, q=q, precision=0, seed=15)
+        s.decompose(15, oversampling=P_OVERSAMPLE)
+        dec, pipe = [], []
+        for r in range(200):
+            a = time.perf_counter()
+            s.decompose(15, oversampling=P_OVERSAMPLE)
+            b_ = time.perf_counter()
+            s.reconstruct(list(range(15)))
+            c = time.perf_counter()
+            dec.append(b_ - a)
+            pipe.append(c - a)
+        out[f"ref_shape_N500_k15_q{q}_decompose_p50_ms"] = round(1e3 * float(np.percentile(dec, 50)), 4)
+        out[f"ref_shape_N500_k15_q{q}_pipeline_p50_ms"] = round(1e3 * float(np.percentile(pipe, 50)), 4)
     # streaming ticks, the reference's low-latency pattern (MKL/ssa_opt_r2c.h:82-91):
     # one prepared context; per tick update_signal on the window moved by one
     # sample, decompose_randomized (k = 50, p = 10, q = 2) and the trend
@@ -379,15 +412,17 @@ def main():
     total, lat = timed_steps(mixed, args.steps, args.warmup)
     ssa_s = len(xs) * args.steps / total
     print(f"[bench] mixed: {ssa_s:.0f} SSA/s, {1e3 * total / args.steps:.2f} ms/step", flush=True)
+    checked = step_check(mixed)
     roof = roofline_of(mixed, plans, Ns, Ls, 4)
     alg = sum(algorithmic_bytes(int(Ns[i]), int(Ls[i]), plans[i]["k"], plans[i]["b"], 4) for i in range(len(xs)))
     del mixed
 
-    plain = None
+    plain = checked_plain = None
     if not args.no_plain:
         pb = ssa_amd.Batch(xs, Ls, ks, p=P_OVERSAMPLE, q=Q_POWER, precision=0, seed=4)
         ptot, plat = timed_steps(pb, args.plain_steps, 1)
         plain = len(xs) * args.plain_steps / ptot
+        checked_plain = step_check(pb)
         print(f"[bench] plain FP64: {plain:.0f} SSA/s, {1e3 * ptot / args.plain_steps:.2f} ms/step", flush=True)
         del pb
 
@@ -430,6 +465,7 @@ def main():
         "step_achieved_GBs": round(alg / (ms_step * 1e-3) / 1e9, 1),
     }
     out.update(roof)
+    out["output_check"] = {"mixed": checked, "plain": checked_plain}
     out["autotune"] = tuned
     out["serving"] = serving_out
     out["single_series"] = configs_out

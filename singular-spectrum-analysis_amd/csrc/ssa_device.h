@@ -20,6 +20,7 @@ extern "C" {
 
 #define SSA_BMAX 80           /* max block width k + p */
 #define SSA_GRAM_ROWS 2048    /* panel rows per Gram / apply work item */
+#define SSA_EDGE 64           /* samples at each end of a reconstruction summed directly (power of two) */
 
 enum { SSA_PREC_F64 = 0, SSA_PREC_F32 = 1 };
 
@@ -84,8 +85,12 @@ enum { SSA_PANEL_Y = 0, SSA_PANEL_Z = 1, SSA_PANEL_U = 2, SSA_PANEL_V = 3, SSA_P
 int ssa_dev_twiddles(int prec, void **table_out, void *stream);
 int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items, int n_items,
                     void *bulk, const void *tw, void *stream);
-/* dir 0: out = H * in (in: K-dim panel, out: L-dim panel); dir 1: out = H^T * in. */
-int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel,
+/* dir 0: out = H * in (in: K-dim panel, out: L-dim panel); dir 1: out = H^T * in.
+ * wide 1: the wide transform geometry (k_matvec<T, lgn, true>: E = 32 values per
+ * thread in FP32 at n = 8192, E = 16 in FP64 at 1024 <= n <= 4096, half the
+ * waves); it pays on launches that fill the device.  Other lengths compile to
+ * the same code either way. */
+int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel, int wide,
                    const ssa_desc *D, const ssa_item *items, int n_items,
                    void *bulk, const void *tw, void *stream);
 /* Philox Gaussian test matrix into panel Z; items (s, row tile) of the K-dim panel */
@@ -115,6 +120,11 @@ int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, int nspli
  * channel index taken from gdesc[gid].z.  Output of gid goes to out + gid*N. */
 int ssa_dev_olap(int prec, int weight, int nsplit, const ssa_desc *D, const ssa_item *items, int n_items,
                  const int *gdesc, void *bulk, void *stream);
+/* the first and last SSA_EDGE samples of every group's reconstruction by direct
+ * anti-diagonal sums (FP64), overwriting the overlap-add's values there; items
+ * (s, gid); scale_sigma, weight, limit_gsize and groups as in ssa_dev_recon / ssa_dev_olap */
+int ssa_dev_edge(int prec, int scale_sigma, int weight, int limit_gsize, const ssa_desc *D, const ssa_item *items,
+                 int n_items, const int *gdesc, const int *glist, void *bulk, const double *small, void *stream);
 /* W-correlation: Gram of the sqrt(w)-weighted component panel (stored at off_out, N x k)
  * then normalisation; W written row-major k x k to small + off_small + SLOT_W */
 int ssa_dev_wcorr(int prec, int nb, const ssa_desc *D, const ssa_item *tile_items, int n_tiles,

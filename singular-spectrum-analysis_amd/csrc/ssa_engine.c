@@ -81,6 +81,7 @@ struct ssa_engine {
     int step_h;                /* forecast horizon of the batched step (0: none) */
     int cholqr2_all;           /* CholeskyQR2 for every orthonormalisation (default: last one only) */
     int final2;                /* CholeskyQR2 for the last orthonormalisation of Y and for B */
+    int mv_wide_min;           /* block-matvec launches of at least this many items take the wide geometry */
     hipEvent_t *prof_ev;
     int *prof_phase, *prof_lgn;
 };
@@ -245,6 +246,8 @@ static const ssa_item *it_ptr(const ssa_engine *e, span sp) { return e->ditems +
  * large batch, short ranges for one series with many output chunks (MSSA),
  * where a few long items would leave most CUs idle. */
 enum { MV_MIN_ITEMS = 1024 };
+/* default item count from which a block-matvec launch takes the wide geometry */
+enum { MV_WIDE_MIN_ITEMS = 2048 };
 /* component-loop splits of the batched step's reconstruction items */
 enum { RECON_SPLIT = 3 };
 static int build_matvec(ssa_engine *e, int dir, int lgn, span *sp)
@@ -445,6 +448,13 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
          * CholeskyQR2 there. */
         const char *f = getenv("SSA_AMD_FINAL_CHOLQR2");
         e->final2 = (prec == SSA_PREC_F64) || (f && *f && *f != '0');
+        /* The wide matvec geometry (fewer LDS exchanges, half the waves) pays
+         * when a launch fills the device; a launch of few items is bound by
+         * each item's latency, which the narrow geometry's extra waves hide.
+         * SSA_AMD_MV_WIDE_MIN moves the threshold (1: always wide). */
+        const char *w = getenv("SSA_AMD_MV_WIDE_MIN");
+        e->mv_wide_min = (w && *w) ? atoi(w) : MV_WIDE_MIN_ITEMS;
+        if (e->mv_wide_min < 1) e->mv_wide_min = 1;
     }
     *err = 0;
     return e;
@@ -772,7 +782,8 @@ static int matvec_all(ssa_engine *e, int dir, int in_panel, int out_panel)
         span sp = dir == 0 ? e->mvf[lg] : e->mva[lg];
         if (!sp.n) continue;
         prof_begin(e, dir == 0 ? PH_MV_FWD : PH_MV_ADJ, lg);
-        DEVCHK(ssa_dev_matvec(e->prec, lg, dir, in_panel, out_panel, e->dd, it_ptr(e, sp), sp.n, e->arena_a,
+        DEVCHK(ssa_dev_matvec(e->prec, lg, dir, in_panel, out_panel, sp.n >= e->mv_wide_min, e->dd,
+                              it_ptr(e, sp), sp.n, e->arena_a,
                               e->tw, e->st));
         prof_end(e);
     }
@@ -1015,7 +1026,7 @@ static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weigh
 {
     const ssa_desc *d = &e->hd[s];
     e->hitems_n = e->dyn_base;
-    span rec, ol;
+    span rec, ol, edge;
     rec.off = e->hitems_n;
     for (int g = 0; g < ng; g++)
         for (int a = 0; a < d->AL; a++)
@@ -1025,10 +1036,15 @@ static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weigh
     for (int g = 0; g < ng; g++)
         for (int t = 0; t < ceil_div(d->N, OLAP_TILE); t++) DEVCHK(items_push(e, s, g, t, 0));
     ol.n = (int)(e->hitems_n - ol.off);
+    edge.off = e->hitems_n;
+    for (int g = 0; g < ng; g++) DEVCHK(items_push(e, s, g, 0, 0));
+    edge.n = ng;
     DEVCHK(items_upload(e, e->dyn_base));
     DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, 0, 1, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
                          e->dgroups + 4 * ng, e->arena_a, e->small, e->tw, e->st));
     DEVCHK(ssa_dev_olap(e->prec, weight, 1, e->dd, it_ptr(e, ol), ol.n, e->dgroups, e->arena_a, e->st));
+    DEVCHK(ssa_dev_edge(e->prec, scale_sigma, weight, 0, e->dd, it_ptr(e, edge), edge.n, e->dgroups,
+                        e->dgroups + 4 * ng, e->arena_a, e->small, e->st));
     return 0;
 }
 
@@ -1191,8 +1207,8 @@ int ssa_engine_matvec(ssa_engine *e, int s, int dir, const double *in, double *o
         rc = items_upload(e, e->dyn_base);
         if (rc == 0)
             rc = ssa_dev_matvec(e->prec, d->lgn, dir, dir == 0 ? SSA_PANEL_Z : SSA_PANEL_Y,
-                                dir == 0 ? SSA_PANEL_Y : SSA_PANEL_Z, e->dd, it_ptr(e, sp), sp.n, e->arena_a, e->tw,
-                                e->st);
+                                dir == 0 ? SSA_PANEL_Y : SSA_PANEL_Z, sp.n >= e->mv_wide_min, e->dd, it_ptr(e, sp),
+                                sp.n, e->arena_a, e->tw, e->st);
     }
     if (rc == 0 && hipMemcpyAsync(e->hpin, (char *)e->arena_a + e->esz * oout, (size_t)lout * e->esz,
                                   hipMemcpyDeviceToHost, e->st) != hipSuccess) rc = -1;
@@ -1240,6 +1256,8 @@ static int launch_step(ssa_engine *e, int q)
     prof_begin(e, PH_OLAP, 0);
     DEVCHK(ssa_dev_olap(e->prec, 0, RECON_SPLIT, e->dd, it_ptr(e, e->olap_step), e->olap_step.n, e->dgroups,
                         e->arena_a, e->st));
+    DEVCHK(ssa_dev_edge(e->prec, 1, 0, 1, e->dd, it_ptr(e, e->ser), e->ser.n, e->dgroups, e->dgroups + 4,
+                        e->arena_a, e->small, e->st));
     prof_end(e);
     if (e->step_h > 0) {
         prof_begin(e, PH_FORECAST, 0);
@@ -1312,7 +1330,8 @@ int ssa_engine_autotune(int prec, double *ns_out)
         float best = -1.0f;
         for (int rep = 0; rc == 0 && rep < 4; rep++) {
             if (hipEventRecord(e0, e->st) != hipSuccess) { rc = -1; break; }
-            rc = ssa_dev_matvec(e->prec, lg, 0, SSA_PANEL_Z, SSA_PANEL_Y, e->dd, it_ptr(e, e->mvf[lg]), e->mvf[lg].n,
+            rc = ssa_dev_matvec(e->prec, lg, 0, SSA_PANEL_Z, SSA_PANEL_Y, e->mvf[lg].n >= e->mv_wide_min, e->dd,
+                                it_ptr(e, e->mvf[lg]), e->mvf[lg].n,
                                 e->arena_a, e->tw, e->st);
             if (rc || hipEventRecord(e1, e->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) { rc = -1; break; }
             float ms = 0;

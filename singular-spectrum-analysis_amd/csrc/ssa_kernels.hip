@@ -1383,6 +1383,72 @@ __global__ __launch_bounds__(256) void k_olap(const ssa_desc *D, const ssa_item 
     }
 }
 
+// ---------------------------------------------------------------------------
+// The first and last SSA_EDGE samples of a group's reconstruction by direct
+// anti-diagonal sums, overwriting what k_olap wrote there:
+//   out[t] = w_t * sum_g s_g sum_{i + j = t} u_g[i] v_g[j],  i < L, j < K.
+// A transform-based convolution leaves an absolute error of about eps * s_g
+// |u_g| |v_g| at every output sample; diagonal averaging divides it by the
+// count min(t + 1, L, K, N - t), which is small only near the two ends, so the
+// ends carry almost all of the error of an FP32 reconstruction.  There the sums
+// have at most SSA_EDGE terms each.  Item (s, gid); products and sums in FP64.
+// scale_sigma, weight, limit_gsize and the group layout as in k_recon/k_olap.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(2 * SSA_EDGE) void k_edge(const ssa_desc *__restrict__ D,
+                                                       const ssa_item *__restrict__ items,
+                                                       const int4 *__restrict__ gdesc,
+                                                       const int *__restrict__ glist, T *bulk,
+                                                       const double *__restrict__ small, int scale_sigma,
+                                                       int weight, int limit_gsize)
+{
+    constexpr int E = SSA_EDGE;
+    __shared__ double uh[E], vh[E], ut[E], vt[E];
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int4 gd = gdesc[it.x];
+    const int L = d.L, K = d.K, N = d.N, m = gd.z;
+    // thread j < E: head sample t = j; thread E + j: tail sample t = N - E + j
+    const int j = threadIdx.x & (E - 1);
+    const bool tail = threadIdx.x >= E;
+    const int t = tail ? N - E + j : j;
+    // a sample both ranges contain (N < 2E) is written by the head thread
+    const bool mine = t >= 0 && t < N && (!tail || t >= E);
+    const int lo = max(0, t - K + 1), hi = min(t, L - 1);
+    const double *sig = small + d.off_small + (long long)SSA_SLOT_MISC * SSA_BMAX * SSA_BMAX + SSA_MISC_SIGMA;
+    double acc = 0.0;
+    for (int g = 0; g < gd.y; g++) {
+        const int idx = glist[gd.x + g];
+        if (idx < 0 || idx >= d.k || (limit_gsize && idx >= d.gsize)) continue;   // uniform over the block
+        const T *u = bulk + d.off_U + (long long)idx * d.ML + (long long)m * L;
+        const T *v = bulk + d.off_V + (long long)idx * K;
+        __syncthreads();   // the previous component's values are no longer read
+        // head rows [0, E) and tail rows [L - E, L) of u, [0, E) and [K - E, K) of v
+        if (!tail) {
+            uh[j] = j < L ? (double)u[j] : 0.0;
+            vh[j] = j < K ? (double)v[j] : 0.0;
+        } else {
+            ut[j] = L - E + j >= 0 ? (double)u[L - E + j] : 0.0;
+            vt[j] = K - E + j >= 0 ? (double)v[K - E + j] : 0.0;
+        }
+        __syncthreads();
+        if (!mine) continue;
+        double s = 0.0;
+        if (!tail) {
+            // t < E: both indices below E
+            for (int i = lo; i <= hi; i++) s = fma(uh[i], vh[t - i], s);
+        } else {
+            // t >= N - E: i >= L - E and t - i >= K - E
+            for (int i = lo; i <= hi; i++) s = fma(ut[i - (L - E)], vt[t - i - (K - E)], s);
+        }
+        acc = fma(scale_sigma ? sig[idx] : 1.0, s, acc);
+    }
+    if (!mine) return;
+    const int c = min(min(t + 1, L), min(K, N - t));
+    const double w = 1.0 / (double)c;
+    bulk[d.off_out + (long long)it.x * N + t] = (T)(acc * (weight ? sqrt(w) : w));
+}
+
 // W-correlation normalisation: W_ij = G_ij / sqrt(G_ii G_jj), rows whose
 // sigma * ||h||_w <= 1e-12 are zero (MKL/ssa_opt.h:2466-2471); row-major k x k.
 __global__ __launch_bounds__(256) void k_wnorm(const ssa_desc *D, const ssa_item *items, double *small)
@@ -1568,18 +1634,12 @@ int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items,
     return (int)hipGetLastError();
 }
 
-enum { MV_WIDE_MIN_ITEMS = 2048 };
-
-int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel, const ssa_desc *D,
+int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel, int wide, const ssa_desc *D,
                    const ssa_item *items, int n_items, void *bulk, const void *tw, void *stream)
 {
     if (n_items <= 0) return 0;
     if (!lgn_ok(prec, lgn)) return -2;
     hipStream_t st = (hipStream_t)stream;
-    // The wide geometry (fewer LDS exchanges, half the waves) pays when the
-    // launch fills the GPU; a launch of few items is bound by each item's
-    // latency, which the narrow geometry's extra waves hide.
-    const bool wide = n_items >= MV_WIDE_MIN_ITEMS;
 #define LAUNCH(T, L, WD)                                                                                      \
     hipLaunchKernelGGL((k_matvec<T, L, WD>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L, WD>::NT), 0, st, D,  \
                        items, (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel, n_items)
@@ -1799,6 +1859,19 @@ int ssa_dev_olap(int prec, int weight, int nsplit, const ssa_desc *D, const ssa_
     else
         hipLaunchKernelGGL(k_olap<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
                            (const int4 *)gdesc, (float *)bulk, weight, nsplit);
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_edge(int prec, int scale_sigma, int weight, int limit_gsize, const ssa_desc *D, const ssa_item *items,
+                 int n_items, const int *gdesc, const int *glist, void *bulk, const double *small, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (prec == SSA_PREC_F64)
+        hipLaunchKernelGGL(k_edge<double>, dim3(n_items), dim3(2 * SSA_EDGE), 0, (hipStream_t)stream, D, items,
+                           (const int4 *)gdesc, glist, (double *)bulk, small, scale_sigma, weight, limit_gsize);
+    else
+        hipLaunchKernelGGL(k_edge<float>, dim3(n_items), dim3(2 * SSA_EDGE), 0, (hipStream_t)stream, D, items,
+                           (const int4 *)gdesc, glist, (float *)bulk, small, scale_sigma, weight, limit_gsize);
     return (int)hipGetLastError();
 }
 

@@ -289,10 +289,10 @@ def rel2(a, b):
 
 def test_mixed_precision_error():
     """Mixed form against the FP64 oracle: reconstruction error <= 1e-5
-    relative (2-norm) for every group, singular values 1e-5 relative.  The
-    pointwise error peaks at the two ends of the series, where diagonal
-    averaging divides the FP32 transform error (~eps32 of the convolution
-    peak) by a count of 1; it is held at 1e-3 of RMS(x)."""
+    relative (2-norm) for every group, singular values 1e-5 relative,
+    pointwise within 1e-4 of RMS(x).  The two ends of the series, where
+    diagonal averaging would divide the FP32 transform error by a count of 1,
+    are summed directly (k_edge)."""
     x, dev, orc = _pair(4000, 1000, 30, 10, 2, seed=5, precision=1)
     so = orc.sigma
     assert np.max(np.abs(dev.singular_values[:10] - so[:10]) / so[:10]) < 1e-5
@@ -300,7 +300,7 @@ def test_mixed_precision_error():
     for g in ([0], [1, 2], [3, 4], [0, 1, 2, 3, 4], list(range(30))):
         a, b = dev.reconstruct(g), orc.reconstruct(g)
         assert rel2(a, b) < 1e-5, g
-        assert np.abs(a - b).max() / r < 1e-3, g
+        assert np.abs(a - b).max() / r < 1e-4, g
 
 
 def test_mixed_batch_against_oracle():
@@ -344,7 +344,7 @@ def test_wide_blocks_mixed_error():
     """The mixed form at b = 80 (FP32 panels, FP32-MFMA apply, FP64 small
     solves) with the tolerances of the mixed form at the benchmark widths:
     singular values within 1e-5 relative, reconstructions within 1e-5
-    relative (2-norm) and pointwise within 1e-3 of RMS(x) of the FP64
+    relative (2-norm) and pointwise within 1e-4 of RMS(x) of the FP64
     oracle."""
     x, dev, orc = _pair(2000, 500, 70, 10, 2, seed=71, precision=1)
     assert np.max(np.abs(dev.singular_values - orc.sigma) / orc.sigma) < 1e-5
@@ -352,7 +352,7 @@ def test_wide_blocks_mixed_error():
     for g in ([0], [1, 2], list(range(70))):
         a, b = dev.reconstruct(g), orc.reconstruct(g)
         assert rel2(a, b) < 1e-5, g
-        assert np.abs(a - b).max() / r < 1e-3, g
+        assert np.abs(a - b).max() / r < 1e-4, g
 
 
 def test_mssa_parity_cfg4_width():

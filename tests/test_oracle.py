@@ -291,6 +291,32 @@ def test_mssa_matvec_and_single_channel():
     assert np.allclose(c.sum(axis=0), 1.0, atol=1e-12)
 
 
+def test_mssa_reconstruct_per_channel_explicit():
+    """orc_mssa_reconstruct for M > 1 (MKL/ssa_opt.h:4212-4242): channel m of
+    a group is the anti-diagonal average of sum_g sigma_g u_g^(m) v_g^T, where
+    u_g^(m) = rows [m L, (m+1) L) of the stacked eigenvector; built here from
+    the oracle's own factors with numpy, so a wrong channel offset or a
+    transposed factor fails.  With every component of the stacked matrix
+    (b = min(M L, K), an exact range) the channels sum back to the input."""
+    M, N, L = 3, 60, 12
+    X = workloads.mssa_channels(M=M, N=N, seed=7)
+    K = N - L + 1
+    full = min(M * L, K)
+    m = O.OracleMSSA(X, L).decompose(full, 8, 1, O.philox_gaussian(8, 0, K, full))
+    U, V, s = m.factors()
+    Hs = np.vstack([hankel(X[c], L) for c in range(M)])
+    assert np.allclose(s, np.linalg.svd(Hs, compute_uv=False)[:full], rtol=1e-10)
+    for g in ([0], [1, 2], [0, 3, 5], list(range(full))):
+        for ch in range(M):
+            Xm = (U[ch * L:(ch + 1) * L, g] * s[g]) @ V[:, g].T
+            assert np.abs(m.reconstruct(ch, g) - diag_average(Xm)).max() < 1e-11 * np.abs(X).max(), (g, ch)
+    for ch in range(M):
+        assert np.abs(m.reconstruct(ch, list(range(full))) - X[ch]).max() < 1e-10 * np.abs(X).max()
+    # channels differ: the explicit average of another channel's rows does not match
+    Xo = (U[L:2 * L, [0]] * s[[0]]) @ V[:, [0]].T
+    assert np.abs(m.reconstruct(0, [0]) - diag_average(Xo)).max() > 1e-3 * np.abs(X).max()
+
+
 # ---------------------------------------------------------------------------
 # the reference's own dependency-free build (compiled from MKL/ssa_opt_ref.h)
 # ---------------------------------------------------------------------------
