@@ -82,6 +82,8 @@ enum { SSA_PANEL_Y = 0, SSA_PANEL_Z = 1, SSA_PANEL_U = 2, SSA_PANEL_V = 3, SSA_P
 #define SSA_NB(c) (((c) + 15) / 16)
 
 /* ---- launchers (return 0 or a hipError_t value) ---- */
+/* twiddle tables of the precision on the current device (static device arrays,
+ * filled on the first call per device and precision; no device allocation) */
 int ssa_dev_twiddles(int prec, void **table_out, void *stream);
 int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items, int n_items,
                     void *bulk, const void *tw, void *stream);

@@ -173,7 +173,6 @@ int ssa_plan_series(int prec, int N, int L, int M, int b, int k, int q, int lgn_
 
 static long long align64(long long v) { return (v + 63) & ~63LL; }
 
-static void *g_tw[2];          /* twiddle tables per precision (process lifetime, device 0) */
 
 static int pin_reserve(ssa_engine *e, size_t bytes)
 {
@@ -433,10 +432,7 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
     e->q_hint = q_hint;
     e->dyn_last_n = e->grp_last_n = -1;
     if (hipStreamCreate(&e->st) != hipSuccess) goto fail;
-    if (!g_tw[prec]) {
-        if (ssa_dev_twiddles(prec, &g_tw[prec], e->st) != 0) goto fail;
-    }
-    e->tw = g_tw[prec];
+    if (ssa_dev_twiddles(prec, &e->tw, e->st) != 0) goto fail;
     if (engine_layout(e, count, N, L, M, bmax, kmax)) goto fail;
     {
         const char *v = getenv("SSA_AMD_CHOLQR2_ALL");

@@ -27,6 +27,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "ssa_device.h"
 #include "ssa_fft.hpp"
 
@@ -1567,45 +1569,64 @@ __global__ __launch_bounds__(256) void k_forecast(ssa_desc *D, const ssa_item *i
 // ===========================================================================
 extern "C" {
 
-static void put_twiddle(int prec, void *host, long long m, long double a)
-{
-    const long double c = cosl(a), s = sinl(a);
-    if (prec == SSA_PREC_F64) {
-        ((double *)host)[2 * m] = (double)c;
-        ((double *)host)[2 * m + 1] = (double)s;
-    } else {
-        ((float *)host)[2 * m] = (float)c;
-        ((float *)host)[2 * m + 1] = (float)s;
-    }
-}
+// Twiddle tables live in static device arrays (one copy per device, filled
+// once from a host table built in long double), so no entry point allocates
+// device memory for them.  FP64: the per-pass tables (tw_offset); FP32: the
+// base table exp(+2 pi i m / 2^TW_BASE_LOG).
+__device__ __align__(256) C2<double> tw_table64[TW_ENTRIES];
+__device__ __align__(256) C2<float> tw_table32[1 << TW_BASE_LOG];
 
-int ssa_dev_twiddles(int prec, void **table_out, void *stream)
+constexpr int TW_MAX_DEVICES = 64;
+static C2<double> h_tw64[TW_ENTRIES];
+static C2<float> h_tw32[1 << TW_BASE_LOG];
+static void *tw_dev[2][TW_MAX_DEVICES];
+static std::mutex tw_lock;
+
+static void build_host_twiddles()
 {
     const long double two_pi = 6.283185307179586476925286766559005768L;
-    const long long n = prec == SSA_PREC_F64 ? TW_ENTRIES : (1LL << TW_BASE_LOG);
-    const size_t esz = prec == SSA_PREC_F64 ? 16 : 8;
-    void *host = malloc(n * esz);
-    if (!host) return -1;
-    if (prec != SSA_PREC_F64)   // base table exp(+2 pi i m / 2^16)
-        for (long long m = 0; m < n; m++) put_twiddle(prec, host, m, two_pi * (long double)m / (long double)n);
-    for (int rlog = 1; prec == SSA_PREC_F64 && rlog <= 4; rlog++)
+    const long long n32 = 1LL << TW_BASE_LOG;
+    for (long long m = 0; m < n32; m++) {
+        const long double a = two_pi * (long double)m / (long double)n32;
+        h_tw32[m] = C2<float>{(float)cosl(a), (float)sinl(a)};
+    }
+    for (int rlog = 1; rlog <= 4; rlog++)
         for (int nslog = 0; nslog <= TW_NSLOG_MAX; nslog++) {
             const long long base = tw_offset(nslog, rlog);
             const int NS = 1 << nslog, R = 1 << rlog;
             for (int kk = 0; kk < NS; kk++)
                 for (int r = 1; r < R; r++) {
                     // exp(+2 pi i r kk / (NS R))
-                    put_twiddle(prec, host, base + (long long)(r - 1) * NS + kk,
-                                two_pi * (long double)(r * kk) / (long double)(NS * R));
+                    const long double a = two_pi * (long double)(r * kk) / (long double)(NS * R);
+                    h_tw64[base + (long long)(r - 1) * NS + kk] = C2<double>{(double)cosl(a), (double)sinl(a)};
                 }
         }
-    void *dev = nullptr;
-    hipError_t e = hipMalloc(&dev, n * esz);
-    if (e == hipSuccess) e = hipMemcpyAsync(dev, host, n * esz, hipMemcpyHostToDevice, (hipStream_t)stream);
-    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
-    free(host);
+}
+
+int ssa_dev_twiddles(int prec, void **table_out, void *stream)
+{
+    (void)stream;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return (int)e;
-    *table_out = dev;
+    if (dev < 0 || dev >= TW_MAX_DEVICES) return -1;
+    const int pi = prec == SSA_PREC_F64 ? 0 : 1;
+    std::lock_guard<std::mutex> guard(tw_lock);
+    if (!tw_dev[pi][dev]) {
+        static bool host_built = false;
+        if (!host_built) {
+            build_host_twiddles();
+            host_built = true;
+        }
+        void *p = nullptr;
+        e = pi == 0 ? hipGetSymbolAddress(&p, HIP_SYMBOL(tw_table64)) : hipGetSymbolAddress(&p, HIP_SYMBOL(tw_table32));
+        if (e != hipSuccess) return (int)e;
+        e = pi == 0 ? hipMemcpy(p, h_tw64, sizeof(h_tw64), hipMemcpyHostToDevice)
+                    : hipMemcpy(p, h_tw32, sizeof(h_tw32), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return (int)e;
+        tw_dev[pi][dev] = p;
+    }
+    *table_out = tw_dev[pi][dev];
     return 0;
 }
 
