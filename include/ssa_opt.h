@@ -307,6 +307,21 @@ long long ssa_amd_series_bytes(int N, int L, int k, int p, int q, int precision,
  * matrix), the FFT length log2 (or panel id / column blocks) and the
  * milliseconds; branch after branch; returns the number of launches */
 int ssa_amd_batch_profile(ssa_amd_batch *b, int max_rec, int *phase, int *lgn, float *ms);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the reconstruction of ssa_opt_reconstruct (FP64) from
+ * factors the caller holds on the device -- U (L x k, column-major), V
+ * (K x k, column-major), sigma (k) -- into the device array output (N
+ * values).  group is a host array (indices outside 0..k-1 are skipped, as in
+ * ssa_opt_reconstruct).  Work space comes from a static 32 MiB device array
+ * per GPU; nothing is allocated.  The work is queued on stream (NULL: the null
+ * stream) and the call returns without waiting.  Calls that overlap on one GPU
+ * share that array and race.  Returns 0; -1 on invalid arguments (including
+ * n_group < 1 and pointers not 8-byte aligned); -2 when the series needs more
+ * work space than the array holds (N above about 210000 at L = N/4, 180000
+ * at L = N/2) or n_group exceeds 256; a HIP error code otherwise. */
+int ssa_amd_reconstruct_noalloc(const ssa_real *U, const ssa_real *V, const ssa_real *sigma, int N, int L, int k,
+                                const int *group, int n_group, ssa_real *output, void *stream);
 /* plan of series i: n, CL, CK, AL, SK, b, k, log2 n */
 int ssa_amd_batch_plan(const ssa_amd_batch *b, int i, int *out8);
 void ssa_amd_batch_free(ssa_amd_batch *b);

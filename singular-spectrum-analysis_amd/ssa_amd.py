@@ -140,6 +140,7 @@ SIGNATURES = [
     ("ssa_amd_batch_query", _I, [_VP]),
     ("ssa_amd_batch_status", _I, [_VP, _I]),
     ("ssa_amd_batch_plan", _I, [_VP, _I, c_int_p]),
+    ("ssa_amd_reconstruct_noalloc", _I, [_VP, _VP, _VP, _I, _I, _I, c_int_p, _I, _VP, _VP]),
 ]
 
 
