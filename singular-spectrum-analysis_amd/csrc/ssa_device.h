@@ -51,6 +51,11 @@ typedef struct ssa_desc {
     unsigned long long call;  /* decomposition counter (Philox counter word) */
     long long stage_cap;      /* staging capacity in T elements */
     long long out_cap;        /* output capacity in T elements */
+    /* component spectra cache (element offset into the bulk arena): for
+     * component g and chunk c (c < M*AL: row chunk c of U, else column chunk
+     * c - M*AL of V), the n/2 + 1 complex values of T-(chunk) at
+     * off_cache + 2 ((g (M AL + SK) + c) (n/2 + 1)) */
+    long long off_cache;
 } ssa_desc;
 
 /* FP64 small-matrix slots, each SSA_BMAX*SSA_BMAX doubles, relative to off_small.
@@ -122,6 +127,18 @@ int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, int nspli
  * channel index taken from gdesc[gid].z.  Output of gid goes to out + gid*N. */
 int ssa_dev_olap(int prec, int weight, int nsplit, const ssa_desc *D, const ssa_item *items, int n_items,
                  const int *gdesc, void *bulk, void *stream);
+/* component spectra cache: items (s, component pair p, chunk c); components 2p and
+ * 2p + 1 (< k) of U's row chunk c (c < M AL) or V's column chunk c - M AL, transformed
+ * together and separated; the n/2 + 1 values of each land at off_cache (ssa_desc) */
+int ssa_dev_cache_build(int prec, int lgn, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk,
+                        const void *tw, void *stream);
+/* reconstruction pieces from the cache: pieces (gid, a, sk) numbered (gid AL + a) SK + sk
+ * for n_pieces = groups AL SK; item (s, first piece) takes that piece and the next
+ * (two real pieces share one inverse transform); staged as ssa_dev_recon stages
+ * them with one split, for ssa_dev_olap with nsplit 1 */
+int ssa_dev_recon_cached(int prec, int lgn, int scale_sigma, const ssa_desc *D, const ssa_item *items, int n_items,
+                         int n_pieces, const int *gdesc, const int *glist, void *bulk, const double *small,
+                         const void *tw, void *stream);
 /* the first and last SSA_EDGE samples of every group's reconstruction by direct
  * anti-diagonal sums (FP64), overwriting the overlap-add's values there; items
  * (s, gid); scale_sigma, weight, limit_gsize and groups as in ssa_dev_recon / ssa_dev_olap */

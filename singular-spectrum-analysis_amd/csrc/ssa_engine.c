@@ -82,6 +82,11 @@ struct ssa_engine {
     int cholqr2_all;           /* CholeskyQR2 for every orthonormalisation (default: last one only) */
     int final2;                /* CholeskyQR2 for the last orthonormalisation of Y and for B */
     int mv_wide_min;           /* block-matvec launches of at least this many items take the wide geometry */
+    /* component spectra cache (ssa_engine_cache_build): its own allocation,
+     * addressed from arena_a like the panel arena */
+    void *cache;
+    long long cache_cap;       /* allocated T elements */
+    int cache_valid;           /* built for the current decomposition of every series */
     hipEvent_t *prof_ev;
     int *prof_phase, *prof_lgn;
 };
@@ -408,6 +413,7 @@ static int engine_layout(ssa_engine *e, int count, const int *N, const int *L, c
         e->dd_cap = count;
     }
     e->prepared = 0;
+    e->cache_valid = 0;
     e->max_groups = 0;
     e->max_h = 0;
     e->step_h = 0;
@@ -475,6 +481,7 @@ void ssa_engine_destroy(ssa_engine *e)
     if (e->arena_a) hipFree(e->arena_a);
     if (e->arena_b) hipFree(e->arena_b);
     if (e->small) hipFree(e->small);
+    if (e->cache) hipFree(e->cache);
     if (e->dd) hipFree(e->dd);
     if (e->ditems) hipFree(e->ditems);
     if (e->dgroups) hipFree(e->dgroups);
@@ -646,6 +653,7 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
         if (d->gsize <= 0 || d->gsize > d->k) d->gsize = d->k;
     }
     e->prepared = 1;
+    e->cache_valid = 0;   /* arenas may have moved */
     e->max_groups = max_groups;
     e->max_h = max_h;
     e->lists_dirty = 1;
@@ -735,7 +743,9 @@ int ssa_engine_status(ssa_engine *e, int s)
 {
     if (s < 0 || s >= e->count) return -1;
     int st = 0;
-    HIPCHK(hipMemcpy(&st, (char *)(e->dd + s) + offsetof(ssa_desc, status), sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(&st, (char *)(e->dd + s) + offsetof(ssa_desc, status), sizeof(int), hipMemcpyDeviceToHost,
+                          e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
     return st;
 }
 
@@ -925,6 +935,7 @@ static int launch_decompose(ssa_engine *e, int q, int injected)
 int ssa_engine_decompose(ssa_engine *e, int q, int injected)
 {
     if (!e->prepared || q < 0) return -1;
+    e->cache_valid = 0;   /* the cached spectra belong to the previous factors */
     DEVCHK(sync_desc(e));
     DEVCHK(launch_decompose(e, q, injected));
     return 0;
@@ -1018,15 +1029,22 @@ static int groups_upload(ssa_engine *e, const int *gdesc4, int ng, const int *gl
 }
 
 /* launch the reconstruction of ng groups of series s into its output slots */
-static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weight)
+static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weight, int use_cache)
 {
     const ssa_desc *d = &e->hd[s];
+    const int cached = use_cache && e->cache_valid;
+    const int n_pieces = ng * d->AL * d->SK;
     e->hitems_n = e->dyn_base;
     span rec, ol, edge;
     rec.off = e->hitems_n;
-    for (int g = 0; g < ng; g++)
-        for (int a = 0; a < d->AL; a++)
-            for (int sk = 0; sk < d->SK; sk++) DEVCHK(items_push(e, s, g, a, sk));
+    if (cached) {
+        /* two pieces per item (one inverse transform) */
+        for (int p = 0; p < n_pieces; p += 2) DEVCHK(items_push(e, s, p, 0, 0));
+    } else {
+        for (int g = 0; g < ng; g++)
+            for (int a = 0; a < d->AL; a++)
+                for (int sk = 0; sk < d->SK; sk++) DEVCHK(items_push(e, s, g, a, sk));
+    }
     rec.n = (int)(e->hitems_n - rec.off);
     ol.off = e->hitems_n;
     for (int g = 0; g < ng; g++)
@@ -1036,8 +1054,12 @@ static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weigh
     for (int g = 0; g < ng; g++) DEVCHK(items_push(e, s, g, 0, 0));
     edge.n = ng;
     DEVCHK(items_upload(e, e->dyn_base));
-    DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, 0, 1, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
-                         e->dgroups + 4 * ng, e->arena_a, e->small, e->tw, e->st));
+    if (cached)
+        DEVCHK(ssa_dev_recon_cached(e->prec, d->lgn, scale_sigma, e->dd, it_ptr(e, rec), rec.n, n_pieces, e->dgroups,
+                                    e->dgroups + 4 * ng, e->arena_a, e->small, e->tw, e->st));
+    else
+        DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, 0, 1, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
+                             e->dgroups + 4 * ng, e->arena_a, e->small, e->tw, e->st));
     DEVCHK(ssa_dev_olap(e->prec, weight, 1, e->dd, it_ptr(e, ol), ol.n, e->dgroups, e->arena_a, e->st));
     DEVCHK(ssa_dev_edge(e->prec, scale_sigma, weight, 0, e->dd, it_ptr(e, edge), edge.n, e->dgroups,
                         e->dgroups + 4 * ng, e->arena_a, e->small, e->st));
@@ -1059,6 +1081,12 @@ static int fetch_out(ssa_engine *e, int s, long long count, double *out)
 int ssa_engine_reconstruct(ssa_engine *e, int s, int ch, const int *glist, const int *gcount, int n_groups,
                            int scale_sigma, int weight, double *out)
 {
+    return ssa_engine_reconstruct_ex(e, s, ch, glist, gcount, n_groups, scale_sigma, weight, 1, out);
+}
+
+int ssa_engine_reconstruct_ex(ssa_engine *e, int s, int ch, const int *glist, const int *gcount, int n_groups,
+                              int scale_sigma, int weight, int use_cache, double *out)
+{
     if (!e->prepared || s < 0 || s >= e->count || n_groups < 1) return -1;
     const ssa_desc *d = &e->hd[s];
     if (ch < 0 || ch >= d->M) return -1;
@@ -1078,7 +1106,7 @@ int ssa_engine_reconstruct(ssa_engine *e, int s, int ch, const int *glist, const
             start += gcount[g0 + g];
         }
         rc = groups_upload(e, gd, ng, glist + first, start);
-        if (rc == 0) rc = launch_recon(e, s, ng, scale_sigma, weight);
+        if (rc == 0) rc = launch_recon(e, s, ng, scale_sigma, weight, use_cache);
         if (rc == 0 && out) rc = fetch_out(e, s, (long long)ng * d->N, out + (size_t)g0 * d->N);
         first += start;
     }
@@ -1086,7 +1114,7 @@ int ssa_engine_reconstruct(ssa_engine *e, int s, int ch, const int *glist, const
     return rc;
 }
 
-int ssa_engine_wcorr(ssa_engine *e, int s, double *W)
+int ssa_engine_wcorr(ssa_engine *e, int s, int use_cache, double *W)
 {
     if (!e->prepared || s < 0 || s >= e->count || !W) return -1;
     const ssa_desc *d = &e->hd[s];
@@ -1094,7 +1122,7 @@ int ssa_engine_wcorr(ssa_engine *e, int s, double *W)
     int *gl = (int *)malloc(sizeof(int) * k), *gc = (int *)malloc(sizeof(int) * k);
     if (!gl || !gc) { free(gl); free(gc); return -1; }
     for (int i = 0; i < k; i++) { gl[i] = i; gc[i] = 1; }
-    int rc = ssa_engine_reconstruct(e, s, 0, gl, gc, k, 0, 1, NULL);
+    int rc = ssa_engine_reconstruct_ex(e, s, 0, gl, gc, k, 0, 1, use_cache, NULL);
     free(gl); free(gc);
     DEVCHK(rc);
     /* Gram tiles over the N x k weighted component panel */
@@ -1161,7 +1189,9 @@ int ssa_engine_forecast(ssa_engine *e, int s, const int *group, int n_group, int
     HIPCHK(hipStreamSynchronize(e->st));
     if (status) {
         int zero = 0;
-        HIPCHK(hipMemcpy((char *)(e->dd + s) + offsetof(ssa_desc, status), &zero, sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync((char *)(e->dd + s) + offsetof(ssa_desc, status), &zero, sizeof(int),
+                              hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipStreamSynchronize(e->st));
         return -1;
     }
     double *tmp = (double *)malloc(sizeof(double) * ((size_t)d->N + h));
@@ -1218,6 +1248,72 @@ int ssa_engine_matvec(ssa_engine *e, int s, int dir, const double *in, double *o
     e->lists_dirty = 1;
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* component spectra cache                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* T elements of series s's cache: k components x (M AL + SK) chunks x (n/2 + 1) complex */
+static long long cache_elems(const ssa_desc *d)
+{
+    return 2LL * d->k * (d->M * d->AL + d->SK) * (d->n / 2 + 1);
+}
+
+int ssa_engine_cache_build(ssa_engine *e)
+{
+    if (!e->prepared) return -1;
+    long long total = 0;
+    for (int s = 0; s < e->count; s++) total += align64(cache_elems(&e->hd[s]));
+    if (total > e->cache_cap || !e->cache) {
+        if (e->cache) { hipFree(e->cache); e->cache = NULL; e->cache_cap = 0; }
+        HIPCHK(hipMalloc(&e->cache, e->esz * (size_t)(total > 64 ? total : 64)));
+        e->cache_cap = total > 64 ? total : 64;
+    }
+    long long delta = (char *)e->cache - (char *)e->arena_a;
+    if (delta % (long long)e->esz) return -1;
+    delta /= (long long)e->esz;
+    long long off = 0;
+    for (int s = 0; s < e->count; s++) {
+        e->hd[s].off_cache = delta + off;
+        off += align64(cache_elems(&e->hd[s]));
+    }
+    e->desc_dirty = 1;
+    DEVCHK(sync_desc(e));
+    /* items (s, component pair, chunk), one launch per transform length */
+    e->hitems_n = e->dyn_base;
+    span per[NLG];
+    for (int lg = 0; lg < NLG; lg++) {
+        per[lg].off = e->hitems_n;
+        for (int s = 0; s < e->count; s++) {
+            const ssa_desc *d = &e->hd[s];
+            if (d->lgn != lg) continue;
+            for (int p = 0; p < (d->k + 1) / 2; p++)
+                for (int c = 0; c < d->M * d->AL + d->SK; c++) DEVCHK(items_push(e, s, p, c, 0));
+        }
+        per[lg].n = (int)(e->hitems_n - per[lg].off);
+    }
+    DEVCHK(items_upload(e, e->dyn_base));
+    for (int lg = 6; lg < NLG; lg++)
+        if (per[lg].n)
+            DEVCHK(ssa_dev_cache_build(e->prec, lg, e->dd, it_ptr(e, per[lg]), per[lg].n, e->arena_a, e->tw, e->st));
+    /* later reconstructions are queued behind the build on the same stream */
+    e->cache_valid = 1;
+    return 0;
+}
+
+void ssa_engine_cache_free(ssa_engine *e)
+{
+    if (!e) return;
+    e->cache_valid = 0;
+    if (e->cache) {
+        hipStreamSynchronize(e->st);
+        hipFree(e->cache);
+        e->cache = NULL;
+        e->cache_cap = 0;
+    }
+}
+
+int ssa_engine_cache_valid(const ssa_engine *e) { return e && e->cache_valid; }
 
 /* ------------------------------------------------------------------------ */
 /* batched step (spectra + decomposition + reconstruction of all k comps)    */

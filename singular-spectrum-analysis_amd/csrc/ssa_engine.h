@@ -86,8 +86,20 @@ void ssa_engine_host_free(double *p);
  * n_groups * N values (weight 0: diagonal averaging).  scale_sigma 1: sigma-weighted */
 int ssa_engine_reconstruct(ssa_engine *e, int s, int ch, const int *glist, const int *gcount, int n_groups,
                            int scale_sigma, int weight, double *out);
-/* W-correlation matrix (k x k row-major) of series s */
-int ssa_engine_wcorr(ssa_engine *e, int s, double *W);
+/* as ssa_engine_reconstruct; use_cache 0 forms every component's transforms on the fly
+ * even when the component spectra cache is valid (ssa_engine_reconstruct uses it then) */
+int ssa_engine_reconstruct_ex(ssa_engine *e, int s, int ch, const int *glist, const int *gcount, int n_groups,
+                              int scale_sigma, int weight, int use_cache, double *out);
+/* W-correlation matrix (k x k row-major) of series s, from the component spectra
+ * cache when use_cache is set and the cache is valid */
+int ssa_engine_wcorr(ssa_engine *e, int s, int use_cache, double *W);
+/* component spectra cache: the transforms of every chunk of every component's U
+ * and V columns of every series, kept in HBM until the next decomposition (or
+ * prepare, rebind, ssa_engine_cache_free); reconstructions read them instead of
+ * transforming the factors again */
+int ssa_engine_cache_build(ssa_engine *e);
+void ssa_engine_cache_free(ssa_engine *e);
+int ssa_engine_cache_valid(const ssa_engine *e);
 /* reconstruction of group + LRF forecast of h steps.  out: N + h values
  * (full) or the h forecasts; returns -1 when the verticality is >= 1 */
 /* R-forecast (vmode 0) or V-forecast (vmode 1) of group from the group's

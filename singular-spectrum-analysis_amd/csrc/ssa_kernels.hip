@@ -1355,6 +1355,159 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) vo
     }
 }
 
+// ---------------------------------------------------------------------------
+// component spectra cache (MKL/ssa_opt.h:2110-2323 caches FFT(sigma u) and FFT(v)
+// per component; here per chunk, unscaled, the sigma applied when summing).
+// Item (s, pair p, chunk c): x = chunk c of component 2p, y = the same chunk of
+// component 2p + 1 (zero beyond k); Z = T-(x + i y), and since x and y are real,
+// X[j] = (Z[j] + conj Z[n-j]) / 2 and Y[j] = (Z[j] - conj Z[n-j]) / (2i).  The
+// mirror values come through LDS one plane at a time.  Only the n/2 + 1 values
+// j <= n/2 are stored (the rest are conjugates).
+// ---------------------------------------------------------------------------
+template <typename T, int LOGN>
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) void k_cache_build(
+    const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items, T *bulk, const C2<T> *__restrict__ tw)
+{
+    using G = FftGeom<T, LOGN>;
+    constexpr int E = G::E, NT = G::NT, N = G::N, H = N / 2 + 1;
+    constexpr bool SPLIT = G::SPLIT;
+    __shared__ T lre[G::LDS];
+    __shared__ T lim[SPLIT ? 1 : G::LDS];
+    const ssa_item it = items[blockIdx.x];
+    const ssa_desc d = D[it.s];
+    const int g0 = 2 * it.x, g1 = g0 + 1, c = it.y, nLc = d.M * d.AL;
+    const bool two = g1 < d.k;
+    const T *src;
+    long long ld;
+    int len;
+    if (c < nLc) {
+        const int m = c / d.AL, a = c - m * d.AL;
+        src = bulk + d.off_U + (long long)m * d.L + (long long)a * d.CL;
+        ld = d.ML;
+        len = min(d.CL, d.L - a * d.CL);
+    } else {
+        const int sk = c - nLc;
+        src = bulk + d.off_V + (long long)sk * d.CK;
+        ld = d.K;
+        len = min(d.CK, d.K - sk * d.CK);
+    }
+    const T *x0 = src + (long long)g0 * ld, *x1 = src + (long long)(two ? g1 : g0) * ld;
+    C2<T> z[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int q = threadIdx.x + e * NT;
+        z[e].x = q < len ? x0[q] : (T)0;
+        z[e].y = (two && q < len) ? x1[q] : (T)0;
+    }
+    fft<T, LOGN, -1, SPLIT>(z, lre, lim, tw);
+    // mirror values Z[n - j] through LDS, real plane then imaginary plane
+    T mr[E], mi[E];
+#pragma unroll
+    for (int ph = 0; ph < 2; ph++) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; e++) lre[threadIdx.x + e * NT] = ph ? z[e].y : z[e].x;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int j = threadIdx.x + e * NT;
+            const T v = lre[(N - j) & (N - 1)];
+            if (ph) mi[e] = v;
+            else mr[e] = v;
+        }
+    }
+    const int nC = nLc + d.SK;
+    C2<T> *out0 = reinterpret_cast<C2<T> *>(bulk + d.off_cache) + ((long long)g0 * nC + c) * H;
+    C2<T> *out1 = reinterpret_cast<C2<T> *>(bulk + d.off_cache) + ((long long)g1 * nC + c) * H;
+    const T h = (T)0.5;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int j = threadIdx.x + e * NT;
+        if (j < H) {
+            out0[j] = C2<T>{h * (z[e].x + mr[e]), h * (z[e].y - mi[e])};
+            if (two) out1[j] = C2<T>{h * (z[e].y + mi[e]), h * (mr[e] - z[e].x)};
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction pieces from the cache.  Piece P = (gid AL + a) SK + sk of the
+// launch: sum over the group's components of s_g U_g,a[j] V_g,sk[j] (the
+// product of two real-signal spectra is Hermitian, so j > n/2 reads n - j and
+// conjugates).  Item (s, P0): pieces P0 and P0 + 1 share one inverse
+// transform, T+(A + i B) = a + i b for real a and b; each is staged where
+// k_recon stages a piece with one split.
+// ---------------------------------------------------------------------------
+template <typename T, int LOGN>
+__global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) void k_recon_cached(
+    const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items, T *bulk, const double *__restrict__ small,
+    const int4 *__restrict__ gdesc, const int *__restrict__ glist, const C2<T> *__restrict__ tw, int scale_sigma,
+    int n_pieces, int n_items)
+{
+    using G = FftGeom<T, LOGN>;
+    constexpr int E = G::E, NT = G::NT, N = G::N, H = N / 2 + 1;
+    constexpr bool SPLIT = G::SPLIT;
+    __shared__ T lre[G::LDS];
+    __shared__ T lim[SPLIT ? 1 : G::LDS];
+    const int item = xcd_item(n_items);
+    if (item < 0) return;
+    const ssa_item it = items[item];
+    const ssa_desc d = D[it.s];
+    const int nC = d.M * d.AL + d.SK, per = d.AL * d.SK;
+    const double *sig = small + d.off_small + (long long)SSA_SLOT_MISC * SSA_BMAX * SSA_BMAX + SSA_MISC_SIGMA;
+    const C2<T> *cache = reinterpret_cast<const C2<T> *>(bulk + d.off_cache);
+    C2<T> acc[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) acc[e] = C2<T>{0, 0};
+    int plen[2] = {0, 0};
+    for (int h = 0; h < 2; h++) {
+        const int P = it.x + h;
+        if (P >= n_pieces) break;
+        const int gid = P / per, rem = P - gid * per, a = rem / d.SK, sk = rem - a * d.SK;
+        const int4 gd = gdesc[gid];
+        const int cu = gd.z * d.AL + a, cv = d.M * d.AL + sk;
+        plen[h] = min(d.CL, d.L - a * d.CL) + min(d.CK, d.K - sk * d.CK) - 1;
+        C2<T> p[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) p[e] = C2<T>{0, 0};
+        for (int g = 0; g < gd.y; g++) {
+            const int idx = glist[gd.x + g];
+            if (idx < 0 || idx >= d.k) continue;
+            const T sc = (T)(scale_sigma ? sig[idx] : 1.0);
+            const C2<T> *cu_p = cache + ((long long)idx * nC + cu) * H;
+            const C2<T> *cv_p = cache + ((long long)idx * nC + cv) * H;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int j = threadIdx.x + e * NT;
+                const int jj = j <= N / 2 ? j : N - j;
+                const C2<T> uu = cu_p[jj], vv = cv_p[jj];
+                p[e].x += sc * (uu.x * vv.x - uu.y * vv.y);
+                p[e].y += sc * (uu.x * vv.y + uu.y * vv.x);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int j = threadIdx.x + e * NT;
+            const T py = j <= N / 2 ? p[e].y : -p[e].y;   // conjugate beyond n/2
+            // piece 0 into the real part, piece 1 (times i) into the imaginary part
+            if (h == 0) acc[e] = C2<T>{p[e].x, py};
+            else acc[e] = C2<T>{acc[e].x - py, acc[e].y + p[e].x};
+        }
+    }
+    fft<T, LOGN, 1, SPLIT>(acc, lre, lim, tw);
+    const T sc = (T)1 / (T)N;
+    for (int h = 0; h < 2; h++) {
+        const int P = it.x + h;
+        if (P >= n_pieces) break;
+        T *st = bulk + d.off_stage + (long long)P * N;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int q = threadIdx.x + e * NT;
+            if (q < plen[h]) st[q] = (h ? acc[e].y : acc[e].x) * sc;
+        }
+    }
+}
+
 // overlap-add of the pieces of group gid and diagonal-averaging weights.
 // item (s, gid, t tile of 1024); weight 0: 1/c_t, 1: sqrt(1/c_t)
 template <typename T>
@@ -1605,7 +1758,6 @@ static void build_host_twiddles()
 
 int ssa_dev_twiddles(int prec, void **table_out, void *stream)
 {
-    (void)stream;
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return (int)e;
@@ -1621,8 +1773,12 @@ int ssa_dev_twiddles(int prec, void **table_out, void *stream)
         void *p = nullptr;
         e = pi == 0 ? hipGetSymbolAddress(&p, HIP_SYMBOL(tw_table64)) : hipGetSymbolAddress(&p, HIP_SYMBOL(tw_table32));
         if (e != hipSuccess) return (int)e;
-        e = pi == 0 ? hipMemcpy(p, h_tw64, sizeof(h_tw64), hipMemcpyHostToDevice)
-                    : hipMemcpy(p, h_tw32, sizeof(h_tw32), hipMemcpyHostToDevice);
+        // on the caller's stream: work on the null stream would occupy a
+        // hardware queue of its own, which the engines' streams then share
+        hipStream_t st = (hipStream_t)stream;
+        e = pi == 0 ? hipMemcpyAsync(p, h_tw64, sizeof(h_tw64), hipMemcpyHostToDevice, st)
+                    : hipMemcpyAsync(p, h_tw32, sizeof(h_tw32), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return (int)e;
         tw_dev[pi][dev] = p;
     }
@@ -1880,6 +2036,49 @@ int ssa_dev_olap(int prec, int weight, int nsplit, const ssa_desc *D, const ssa_
     else
         hipLaunchKernelGGL(k_olap<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
                            (const int4 *)gdesc, (float *)bulk, weight, nsplit);
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_cache_build(int prec, int lgn, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk,
+                        const void *tw, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (!lgn_ok(prec, lgn)) return -2;
+    hipStream_t st = (hipStream_t)stream;
+#define CASE(L)                                                                                               \
+    case L:                                                                                                   \
+        if (prec == SSA_PREC_F64)                                                                             \
+            hipLaunchKernelGGL((k_cache_build<double, L>), dim3(n_items), dim3(FftGeom<double, L>::NT), 0, st, \
+                               D, items, (double *)bulk, (const C2<double> *)tw);                             \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_cache_build<float, L>), dim3(n_items), dim3(FftGeom<float, L>::NT), 0, st,   \
+                               D, items, (float *)bulk, (const C2<float> *)tw);                               \
+        break;
+    switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
+#undef CASE
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_recon_cached(int prec, int lgn, int scale_sigma, const ssa_desc *D, const ssa_item *items, int n_items,
+                         int n_pieces, const int *gdesc, const int *glist, void *bulk, const double *small,
+                         const void *tw, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (!lgn_ok(prec, lgn)) return -2;
+    hipStream_t st = (hipStream_t)stream;
+#define CASE(L)                                                                                               \
+    case L:                                                                                                   \
+        if (prec == SSA_PREC_F64)                                                                             \
+            hipLaunchKernelGGL((k_recon_cached<double, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<double, L>::NT), \
+                               0, st, D, items, (double *)bulk, small, (const int4 *)gdesc, glist,            \
+                               (const C2<double> *)tw, scale_sigma, n_pieces, n_items);                       \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_recon_cached<float, L>), dim3(xcd_grid(n_items)), dim3(FftGeom<float, L>::NT), \
+                               0, st, D, items, (float *)bulk, small, (const int4 *)gdesc, glist,             \
+                               (const C2<float> *)tw, scale_sigma, n_pieces, n_items);                        \
+        break;
+    switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
+#undef CASE
     return (int)hipGetLastError();
 }
 

@@ -72,8 +72,9 @@ DeviceState g_state[SSA_NOALLOC_MAX_DEVICES];
 std::mutex g_lock;
 
 // once per device, with the device current: the scratch address (never the
-// host-side name of the array) and the twiddle tables
-int device_state(DeviceState *out)
+// host-side name of the array) and the twiddle tables (filled on the caller's
+// stream the first time)
+int device_state(DeviceState *out, void *stream)
 {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
@@ -90,7 +91,7 @@ int device_state(DeviceState *out)
             return (int)e;
         }
         void *tw = nullptr;
-        const int rc = ssa_dev_twiddles(SSA_PREC_F64, &tw, nullptr);
+        const int rc = ssa_dev_twiddles(SSA_PREC_F64, &tw, stream);
         if (rc != 0) return rc;
         s.tw = tw;
         s.base = (double *)p;
@@ -134,7 +135,7 @@ extern "C" int ssa_amd_reconstruct_noalloc(const double *U, const double *V, con
         return -2;
     }
     DeviceState st;
-    int rc = device_state(&st);
+    int rc = device_state(&st, stream);
     if (rc != 0) return rc;
     double *base = st.base;
     // element offsets from the scratch base: the caller's buffers are addressed

@@ -176,8 +176,8 @@ int ssa_opt_update_signal(SSA_Opt *ssa, const ssa_real *new_x)
     if (!ssa || !ssa->initialized || !new_x) return -1;
     ssa_ctx *c = (ssa_ctx *)ssa->dev;
     memcpy(c->x, new_x, sizeof(double) * ssa->N);
+    ssa_opt_free_cached_ffts(ssa);
     ssa->decomposed = false;
-    ssa->fft_cached = false;
     if (ssa_engine_upload_signal(c->eng, 0, c->x) || ssa_engine_spectra(c->eng)) return -1;
     if (ssa_engine_read_spectrum(c->eng, 0, ssa->fft_x, ssa->r2c_len)) return -1;
     return 0;
@@ -191,8 +191,8 @@ int ssa_amd_push_samples(SSA_Opt *ssa, const ssa_real *x_new, int m)
     const int N = ssa->N;
     memmove(c->x, c->x + m, sizeof(double) * (size_t)(N - m));
     memcpy(c->x + (N - m), x_new, sizeof(double) * (size_t)m);
+    ssa_opt_free_cached_ffts(ssa);
     ssa->decomposed = false;
-    ssa->fft_cached = false;
     if (ssa_engine_upload_signal(c->eng, 0, c->x) || ssa_engine_spectra(c->eng)) return -1;
     if (ssa_engine_read_spectrum(c->eng, 0, ssa->fft_x, ssa->r2c_len)) return -1;
     return 0;
@@ -327,31 +327,47 @@ int ssa_amd_reconstruct_many(const SSA_Opt *ssa, const int *glist, const int *gc
     return ssa_engine_reconstruct(c->eng, 0, 0, glist, gcount, n_groups, 1, 0, output);
 }
 
-/* MKL/ssa_opt.h:2110-2323.  The device reconstruction reads U and V from
- * HBM and forms the component spectra on the fly, so there is no separate
- * cache to build; the call keeps the reference's contract (requires a
- * decomposition, sets the flag that update_signal / decompose clear). */
+/* MKL/ssa_opt.h:2110-2323: the transforms of every component's U and V
+ * chunks, kept in HBM (ssa_engine_cache_build) until the next decomposition,
+ * update_signal or free_cached_ffts.  While it is valid, reconstructions and
+ * wcorr_matrix_fast read the cached spectra instead of transforming U and V
+ * again.  A second call rebuilds it (MKL/ssa_opt.h:2115-2116). */
 int ssa_opt_cache_ffts(SSA_Opt *ssa)
 {
     if (!ssa || !ssa->decomposed) return -1;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    ssa_opt_free_cached_ffts(ssa);
+    if (ssa_engine_cache_build(c->eng)) {
+        ssa_engine_cache_free(c->eng);
+        return -1;
+    }
     ssa->fft_cached = true;
     return 0;
 }
 
+/* MKL/ssa_opt.h:2058-2078; safe without a cache */
 void ssa_opt_free_cached_ffts(SSA_Opt *ssa)
 {
-    if (ssa) ssa->fft_cached = false;
+    if (!ssa) return;
+    ssa_ctx *c = (ssa_ctx *)ssa->dev;
+    if (c && c->eng && ssa->fft_cached) ssa_engine_cache_free(c->eng);
+    ssa->fft_cached = false;
 }
 
-/* MKL/ssa_opt.h:2407-2477 */
+/* MKL/ssa_opt.h:2407-2477: per-component transforms formed on the fly */
 int ssa_opt_wcorr_matrix(const SSA_Opt *ssa, ssa_real *W)
 {
     if (!ssa || !ssa->decomposed || !W) return -1;
-    return ssa_engine_wcorr(((ssa_ctx *)ssa->dev)->eng, 0, W);
+    return ssa_engine_wcorr(((ssa_ctx *)ssa->dev)->eng, 0, 0, W);
 }
 
-/* MKL/ssa_opt.h:2492-2548 (same result as the slow path) */
-int ssa_opt_wcorr_matrix_fast(const SSA_Opt *ssa, ssa_real *W) { return ssa_opt_wcorr_matrix(ssa, W); }
+/* MKL/ssa_opt.h:2492-2548: from the cached component spectra; without a
+ * cache, the slow path (as the reference falls back, :2497-2498) */
+int ssa_opt_wcorr_matrix_fast(const SSA_Opt *ssa, ssa_real *W)
+{
+    if (!ssa || !ssa->decomposed || !W) return -1;
+    return ssa_engine_wcorr(((ssa_ctx *)ssa->dev)->eng, 0, ssa->fft_cached ? 1 : 0, W);
+}
 
 /* MKL/ssa_opt.h:2551-2625 */
 ssa_real ssa_opt_wcorr_pair(const SSA_Opt *ssa, int i, int j)
@@ -366,53 +382,57 @@ ssa_real ssa_opt_wcorr_pair(const SSA_Opt *ssa, int i, int j)
     return r;
 }
 
-/* MKL/ssa_opt.h:2627-2676 */
+/* log(v) with the reference's guard against log(0) (MKL/ssa_opt.h:2647) */
+static double guarded_log(double v) { return log(v + 1e-300); }
+
+/* index of the first largest of n values (n >= 1) */
+static int first_max(const double *v, int n)
+{
+    int at = 0;
+    for (int i = 1; i < n; i++) at = v[i] > v[at] ? i : at;
+    return at;
+}
+
+/* MKL/ssa_opt.h:2627-2676.  The five arrays share one allocation owned by
+ * singular_values (released by ssa_opt_component_stats_free):
+ *   log_sv[i] = log(sigma_i), gaps[i] = sigma_i / sigma_{i+1} (i < n-1),
+ *   cumulative_var[i] = sum_{j<=i} lambda_j / total_variance,
+ *   second_diff[i] = log_sv[i-1] - 2 log_sv[i] + log_sv[i+1] inside, 0 at both ends;
+ * the suggested signal rank is one past the first largest gap. */
 int ssa_opt_component_stats(const SSA_Opt *ssa, SSA_ComponentStats *stats)
 {
     if (!ssa || !ssa->decomposed || !stats) return -1;
-    int n = ssa->n_components;
+    const int n = ssa->n_components;
     if (n < 2) return -1;
     memset(stats, 0, sizeof(SSA_ComponentStats));
+    double *blk = (double *)calloc(5 * (size_t)n, sizeof(double));
+    if (!blk) return -1;
     stats->n = n;
-    stats->singular_values = (double *)malloc(sizeof(double) * n);
-    stats->log_sv = (double *)malloc(sizeof(double) * n);
-    stats->gaps = (double *)malloc(sizeof(double) * (n - 1));
-    stats->cumulative_var = (double *)malloc(sizeof(double) * n);
-    stats->second_diff = (double *)malloc(sizeof(double) * n);
-    if (!stats->singular_values || !stats->log_sv || !stats->gaps || !stats->cumulative_var || !stats->second_diff) {
-        ssa_opt_component_stats_free(stats);
-        return -1;
-    }
+    stats->singular_values = blk;
+    stats->log_sv = blk + n;
+    stats->gaps = blk + 2 * n;
+    stats->cumulative_var = blk + 3 * n;
+    stats->second_diff = blk + 4 * n;
+    memcpy(stats->singular_values, ssa->sigma, sizeof(double) * (size_t)n);
+    double run = 0.0;
     for (int i = 0; i < n; i++) {
-        stats->singular_values[i] = ssa->sigma[i];
-        stats->log_sv[i] = log(ssa->sigma[i] + 1e-300);
+        stats->log_sv[i] = guarded_log(ssa->sigma[i]);
+        run += ssa->eigenvalues[i];
+        stats->cumulative_var[i] = run / ssa->total_variance;
+        if (i + 1 < n) stats->gaps[i] = ssa->sigma[i] / (ssa->sigma[i + 1] + 1e-300);
     }
-    double max_gap = 0.0;
-    int max_idx = 0;
-    for (int i = 0; i < n - 1; i++) {
-        double g = ssa->sigma[i] / (ssa->sigma[i + 1] + 1e-300);
-        stats->gaps[i] = g;
-        if (g > max_gap) { max_gap = g; max_idx = i; }
-    }
-    double cs = 0.0;
-    for (int i = 0; i < n; i++) {
-        cs += ssa->eigenvalues[i];
-        stats->cumulative_var[i] = cs / ssa->total_variance;
-    }
-    stats->second_diff[0] = 0.0;
-    stats->second_diff[n - 1] = 0.0;
-    for (int i = 1; i < n - 1; i++)
-        stats->second_diff[i] = stats->log_sv[i - 1] - 2.0 * stats->log_sv[i] + stats->log_sv[i + 1];
-    stats->suggested_signal = max_idx + 1;
-    stats->gap_threshold = max_gap;
+    for (int i = 1; i + 1 < n; i++)
+        stats->second_diff[i] = (stats->log_sv[i - 1] + stats->log_sv[i + 1]) - 2.0 * stats->log_sv[i];
+    const int widest = first_max(stats->gaps, n - 1);
+    stats->gap_threshold = stats->gaps[widest];
+    stats->suggested_signal = widest + 1;
     return 0;
 }
 
 void ssa_opt_component_stats_free(SSA_ComponentStats *stats)
 {
     if (!stats) return;
-    free(stats->singular_values); free(stats->log_sv); free(stats->gaps);
-    free(stats->cumulative_var); free(stats->second_diff);
+    free(stats->singular_values);   /* the block holding all five arrays */
     memset(stats, 0, sizeof(SSA_ComponentStats));
 }
 
@@ -1012,6 +1032,7 @@ int ssa_amd_set_options(SSA_Opt *ssa, int q, int precision, unsigned long long s
         ssa->prepared = false;
         ssa->prepared_kp = 0;
         ssa->decomposed = false;
+        ssa->fft_cached = false;   /* the cache went with the engine */
         ssa->n_components = 0;
         if (build_engine(ssa, c)) return -1;
     }

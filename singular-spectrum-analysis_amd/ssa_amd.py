@@ -1,7 +1,10 @@
-"""Python front end of the device SSA library (libssa.so), mirroring the
-reference's ctypes wrapper MKL/py/ssa_wrapper.py: the same classes (SSA,
-MSSA), method names, argument meaning and error behaviour (ValueError on a
-failed init, RuntimeError on a failed call), on top of the same C symbols.
+"""Python front end of the device SSA library (libssa.so).  Its SSA and MSSA
+classes offer the interface of the reference's ctypes wrapper
+MKL/py/ssa_wrapper.py -- the same class and method names, argument meaning and
+error types (ValueError when the library refuses a series, RuntimeError when a
+call fails) -- over the same C symbols; they hold the typed C context and read
+the library's state from it.  Batch (many series per device step) and the
+module functions have no counterpart there.
 
 The library is loaded from this directory; a missing or unloadable library
 raises immediately -- there is no CPU fallback.
@@ -21,13 +24,35 @@ c_float_p = ctypes.POINTER(ctypes.c_float)
 c_ll_p = ctypes.POINTER(ctypes.c_longlong)
 
 
+BMAX = 80   # widest block k + p the device path takes (SSA_BMAX)
+
+_dpp = ctypes.POINTER(ctypes.c_double)
+
+
 class _SSA_Opt(ctypes.Structure):
-    # opaque block, as MKL/py/ssa_wrapper.py:92-94
-    _fields_ = [("_opaque", ctypes.c_char * 1024)]
+    """SSA_Opt as include/ssa_opt.h lays it out (sizeof below the 1024 bytes the
+    reference wrapper reserves, MKL/py/ssa_wrapper.py:92-94)."""
+    _fields_ = [("N", ctypes.c_int), ("L", ctypes.c_int), ("K", ctypes.c_int),
+                ("fft_len", ctypes.c_int), ("r2c_len", ctypes.c_int), ("batch_size", ctypes.c_int),
+                ("fft_x", _dpp), ("ws_real", _dpp), ("ws_complex", _dpp), ("ws_real2", _dpp), ("ws_proj", _dpp),
+                ("ws_batch_real", _dpp), ("ws_batch_complex", _dpp),
+                ("U", _dpp), ("V", _dpp), ("sigma", _dpp), ("eigenvalues", _dpp), ("n_components", ctypes.c_int),
+                ("inv_diag_count", _dpp), ("U_fft", _dpp), ("V_fft", _dpp), ("fft_cached", ctypes.c_bool),
+                ("prepared_kp", ctypes.c_int), ("prepared_lwork", ctypes.c_int),
+                ("decomp_Omega", _dpp), ("decomp_Y", _dpp),
+                ("prepared", ctypes.c_bool), ("initialized", ctypes.c_bool), ("decomposed", ctypes.c_bool),
+                ("total_variance", ctypes.c_double), ("dev", ctypes.c_void_p), ("q", ctypes.c_int),
+                ("precision", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("calls", ctypes.c_ulonglong)]
 
 
 class _MSSA_Opt(ctypes.Structure):
-    _fields_ = [("_opaque", ctypes.c_char * 1024)]
+    """MSSA_Opt as include/ssa_opt.h lays it out."""
+    _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("L", ctypes.c_int), ("K", ctypes.c_int),
+                ("fft_len", ctypes.c_int), ("r2c_len", ctypes.c_int),
+                ("U", _dpp), ("V", _dpp), ("sigma", _dpp), ("eigenvalues", _dpp), ("n_components", ctypes.c_int),
+                ("inv_diag_count", _dpp), ("initialized", ctypes.c_bool), ("decomposed", ctypes.c_bool),
+                ("total_variance", ctypes.c_double), ("dev", ctypes.c_void_p), ("q", ctypes.c_int),
+                ("precision", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("calls", ctypes.c_ulonglong)]
 
 
 class _SSA_LRF(ctypes.Structure):
@@ -232,335 +257,361 @@ def gapfill(x, L, rank, max_iter=50, tol=1e-6, simple=False):
     return y, r.iterations, r.final_diff, bool(r.converged), r.n_gaps
 
 
-class SSA:
-    """Univariate SSA on the device (MKL/py/ssa_wrapper.py:600-1202)."""
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _checked(rc, what):
+    """Raise RuntimeError when a C call returned a failure code."""
+    if rc != 0:
+        raise RuntimeError(f"{what} (libssa returned {rc})")
+
+
+# defaults of the wrapper's non-randomized methods (MKL/py/ssa_wrapper.py:704-745):
+# block width min(k, BLOCK_CAP), MAX_ITER iterations
+BLOCK_CAP = 32
+MAX_ITER = 100
+
+
+class _Device:
+    """One C context (SSA_Opt or MSSA_Opt) owned by a Python object: the typed
+    struct itself, so the library's own state (prepared capacity, component
+    count, flags) is read from it rather than duplicated here."""
+
+    _struct = None
+    _free = None
+
+    def __init__(self):
+        self.lib = lib()
+        self.ctx = self._struct()
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.ctx)
+
+    def __del__(self):
+        if getattr(self, "ctx", None) is not None and getattr(self, "lib", None) is not None:
+            getattr(self.lib, self._free)(ctypes.byref(self.ctx))
+            self.ctx = None
+
+    def _require(self):
+        if not self.ctx.decomposed:
+            raise RuntimeError("no decomposition yet: call decompose() first")
+
+    def _out(self, n, fn, *args, what):
+        out = np.zeros(n)
+        _checked(fn(self.ref, *args, _dp(out)), what)
+        return out
+
+
+class SSA(_Device):
+    """Univariate SSA on the device, with the interface of the reference
+    wrapper's SSA class (MKL/py/ssa_wrapper.py:600-1202): method names,
+    arguments, ValueError for a series the library refuses, RuntimeError for a
+    failed call.  q, precision and seed (no counterpart there) set the power
+    iterations, the plain / mixed form and the test matrix's Philox key."""
+
+    _struct = _SSA_Opt
+    _free = "ssa_opt_free"
 
     def __init__(self, x, L: int, q: int = None, precision: int = None, seed: int = None):
-        self._l = lib()
-        self._ctx = _SSA_Opt()
-        self._x = np.ascontiguousarray(x, dtype=np.float64)
-        self.N = len(self._x)
-        self.L = L
-        self.K = self.N - L + 1
-        self.n_components = 0
-        self._decomposed = False
-        self._prepared = False
-        self._prepared_k = 0
-        self._prepared_oversampling = 0
-        if self._l.ssa_opt_init(ctypes.byref(self._ctx), _dp(self._x), self.N, L) != 0:
-            raise ValueError(f"SSA initialization failed (N={self.N}, L={L})")
-        if q is not None or precision is not None or seed is not None:
+        super().__init__()
+        series = _f64(x)
+        if self.lib.ssa_opt_init(self.ref, _dp(series), series.size, int(L)) != 0:
+            raise ValueError(f"ssa_opt_init refused {series.size} samples with window {L} (needs 2 <= L <= N - 1)")
+        if (q, precision, seed) != (None, None, None):
             self.set_options(2 if q is None else q, 0 if precision is None else precision,
                              42 if seed is None else seed)
 
-    def __del__(self):
-        if hasattr(self, "_ctx") and getattr(self, "_l", None) is not None:
-            self._l.ssa_opt_free(ctypes.byref(self._ctx))
+    # dimensions and state, read from the context
+    N = property(lambda self: self.ctx.N)
+    L = property(lambda self: self.ctx.L)
+    K = property(lambda self: self.ctx.K)
+    n_components = property(lambda self: self.ctx.n_components if self.ctx.decomposed else 0)
 
-    # --- extensions -------------------------------------------------------
+    # --- extensions --------------------------------------------------------
     def set_options(self, q=2, precision=0, seed=42):
-        if self._l.ssa_amd_set_options(ctypes.byref(self._ctx), q, precision, seed) != 0:
-            raise RuntimeError("set_options failed")
-        self._prepared = False
-        self._decomposed = False
+        _checked(self.lib.ssa_amd_set_options(self.ref, int(q), int(precision), int(seed)),
+                 f"options q={q} precision={precision} refused")
         return self
 
     def inject_omega(self, omega):
         om = np.asfortranarray(omega, dtype=np.float64)
-        if self._l.ssa_amd_inject_omega(ctypes.byref(self._ctx), _dp(om), om.shape[0], om.shape[1]) != 0:
-            raise RuntimeError("inject_omega failed")
+        _checked(self.lib.ssa_amd_inject_omega(self.ref, _dp(om), om.shape[0], om.shape[1]),
+                 f"test matrix of shape {om.shape} refused")
         return self
 
     def plan_info(self):
         out = np.zeros(6, dtype=np.int32)
-        self._l.ssa_amd_plan_info(ctypes.byref(self._ctx), _ip(out))
+        self.lib.ssa_amd_plan_info(self.ref, _ip(out))
         return dict(zip(("n", "CL", "CK", "AL", "SK", "sweeps"), (int(v) for v in out)))
 
     def reconstruct_many(self, groups: List[List[int]]):
-        gl = np.ascontiguousarray([i for g in groups for i in g], dtype=np.int32)
-        gc = np.ascontiguousarray([len(g) for g in groups], dtype=np.int32)
+        flat = _i32([i for g in groups for i in g])
+        sizes = _i32([len(g) for g in groups])
+        self._require()
         out = np.zeros(len(groups) * self.N)
-        if self._l.ssa_amd_reconstruct_many(ctypes.byref(self._ctx), _ip(gl), _ip(gc), len(groups), _dp(out)) != 0:
-            raise RuntimeError("reconstruct_many failed")
+        _checked(self.lib.ssa_amd_reconstruct_many(self.ref, _ip(flat), _ip(sizes), len(groups), _dp(out)),
+                 f"reconstruction of {len(groups)} groups failed")
         return out.reshape(len(groups), self.N)
 
     def hankel_matvec(self, v):
-        v = np.ascontiguousarray(v, dtype=np.float64)
         y = np.zeros(self.L)
-        self._l.ssa_opt_hankel_matvec(ctypes.byref(self._ctx), _dp(v), _dp(y))
+        self.lib.ssa_opt_hankel_matvec(self.ref, _dp(_f64(v)), _dp(y))
         return y
 
     def hankel_matvec_T(self, u):
-        u = np.ascontiguousarray(u, dtype=np.float64)
         z = np.zeros(self.K)
-        self._l.ssa_opt_hankel_matvec_T(ctypes.byref(self._ctx), _dp(u), _dp(z))
+        self.lib.ssa_opt_hankel_matvec_T(self.ref, _dp(_f64(u)), _dp(z))
         return z
-
-    # --- the reference wrapper's interface ----------------------------------
-    def prepare(self, max_k: int, oversampling: int = 8) -> "SSA":
-        if self._l.ssa_opt_prepare(ctypes.byref(self._ctx), max_k, oversampling) != 0:
-            raise RuntimeError(f"prepare() failed (max_k={max_k})")
-        self._prepared = True
-        self._prepared_k = max_k
-        self._prepared_oversampling = oversampling
-        return self
-
-    def update_signal(self, new_x) -> "SSA":
-        new_x = np.ascontiguousarray(new_x, dtype=np.float64)
-        if len(new_x) != self.N:
-            raise ValueError(f"Signal length mismatch: {len(new_x)} vs {self.N}")
-        if self._l.ssa_opt_update_signal(ctypes.byref(self._ctx), _dp(new_x)) != 0:
-            raise RuntimeError("update_signal() failed")
-        self._x = new_x
-        return self
 
     def set_mirror(self, on: bool) -> "SSA":
         """Host U/V mirrors refreshed by every decomposition (on) or on demand (off)."""
-        if self._l.ssa_amd_set_mirror(ctypes.byref(self._ctx), int(bool(on))) != 0:
-            raise RuntimeError("set_mirror failed")
+        _checked(self.lib.ssa_amd_set_mirror(self.ref, int(bool(on))), "set_mirror refused")
         return self
 
     def push(self, samples) -> "SSA":
         """Slide the window by len(samples) new samples (ssa_amd_push_samples)."""
-        a = np.ascontiguousarray(samples, dtype=np.float64)
-        if self._l.ssa_amd_push_samples(ctypes.byref(self._ctx), _dp(a), len(a)) != 0:
-            raise RuntimeError("push_samples failed")
-        self._x = np.concatenate([self._x[len(a):], a]) if len(a) < self.N else a[-self.N:].copy()
-        self._decomposed = False
+        new = _f64(samples)
+        _checked(self.lib.ssa_amd_push_samples(self.ref, _dp(new), new.size),
+                 f"push of {new.size} samples refused (1 <= m <= N)")
         return self
+
+    def factors(self):
+        """(U, V, sigma) copies of the current decomposition: U is L x k and V
+        K x k (column-major host mirrors, refreshed first when stale)."""
+        self._require()
+        _checked(self.lib.ssa_amd_sync_mirrors(self.ref), "mirror refresh failed")
+        k = self.ctx.n_components
+        U = np.ctypeslib.as_array(self.ctx.U, shape=(k, self.L)).T.copy()
+        V = np.ctypeslib.as_array(self.ctx.V, shape=(k, self.K)).T.copy()
+        return U, V, np.ctypeslib.as_array(self.ctx.sigma, shape=(k,)).copy()
+
+    def cache_ffts(self) -> "SSA":
+        """Keep the component spectra on the device (ssa_opt_cache_ffts) until
+        the next decomposition or update."""
+        self._require()
+        _checked(self.lib.ssa_opt_cache_ffts(self.ref), "component spectra cache failed")
+        return self
+
+    def free_cached_ffts(self) -> "SSA":
+        self.lib.ssa_opt_free_cached_ffts(self.ref)
+        return self
+
+    fft_cached = property(lambda self: bool(self.ctx.fft_cached))
+
+    def wcorr_matrix_fast(self) -> np.ndarray:
+        """W-correlation from the cached spectra (the slow path without a cache)."""
+        self._require()
+        k = self.ctx.n_components
+        return self._out(k * k, self.lib.ssa_opt_wcorr_matrix_fast, what="W-correlation failed").reshape(k, k)
+
+    # --- the reference wrapper's interface ------------------------------------
+    def prepare(self, max_k: int, oversampling: int = 8) -> "SSA":
+        _checked(self.lib.ssa_opt_prepare(self.ref, int(max_k), int(oversampling)),
+                 f"prepare for {max_k} components + {oversampling} refused (k + p <= {BMAX})")
+        return self
+
+    def update_signal(self, new_x) -> "SSA":
+        series = _f64(new_x)
+        if series.size != self.N:
+            raise ValueError(f"update_signal needs {self.N} samples, got {series.size}")
+        _checked(self.lib.ssa_opt_update_signal(self.ref, _dp(series)), "update_signal failed")
+        return self
+
+    def _randomized(self, k, opts):
+        p = opts.get("oversampling", 8)
+        width = min(k + (p if p > 0 else 8), self.L, self.K)
+        # prepare on demand, as the wrapper does, from the context's own capacity
+        if not self.ctx.prepared or width > self.ctx.prepared_kp:
+            self.prepare(k, p)
+        return self.lib.ssa_opt_decompose_randomized(self.ref, k, p)
+
+    def _block(self, k, opts):
+        return self.lib.ssa_opt_decompose_block(self.ref, k, opts.get("block_size", min(k, BLOCK_CAP)),
+                                                opts.get("max_iter", MAX_ITER))
+
+    def _sequential(self, k, opts):
+        return self.lib.ssa_opt_decompose(self.ref, k, opts.get("max_iter", MAX_ITER))
+
+    _METHODS = {"randomized": _randomized, "block": _block, "sequential": _sequential}
 
     def decompose(self, k: int, method: str = "randomized", **kwargs) -> "SSA":
-        if method == "randomized":
-            oversampling = kwargs.get("oversampling", 8)
-            kp = k + oversampling
-            if not self._prepared or kp > self._prepared_k + self._prepared_oversampling:
-                self.prepare(k, oversampling)
-            ret = self._l.ssa_opt_decompose_randomized(ctypes.byref(self._ctx), k, oversampling)
-        elif method == "block":
-            ret = self._l.ssa_opt_decompose_block(ctypes.byref(self._ctx), k, kwargs.get("block_size", min(k, 32)),
-                                                  kwargs.get("max_iter", 100))
-        elif method == "sequential":
-            ret = self._l.ssa_opt_decompose(ctypes.byref(self._ctx), k, kwargs.get("max_iter", 100))
-        else:
-            raise ValueError(f"Unknown method: {method}")
-        if ret != 0:
-            raise RuntimeError(f"Decomposition failed (method={method}, k={k})")
-        self.n_components = k
-        self._decomposed = True
+        run = self._METHODS.get(method)
+        if run is None:
+            raise ValueError(f"method {method!r} is not one of {sorted(self._METHODS)}")
+        _checked(run(self, int(k), kwargs), f"{method} decomposition into {k} components failed")
         return self
-
-    def _check(self):
-        if not self._decomposed:
-            raise RuntimeError("Call decompose() first")
 
     @property
     def singular_values(self) -> np.ndarray:
-        self._check()
-        out = np.zeros(self.n_components)
-        n = self._l.ssa_opt_get_singular_values(ctypes.byref(self._ctx), _dp(out), self.n_components)
-        return out[:n]
+        self._require()
+        return self._getter(self.lib.ssa_opt_get_singular_values)
 
     @property
     def eigenvalues(self) -> np.ndarray:
-        self._check()
-        out = np.zeros(self.n_components)
-        n = self._l.ssa_opt_get_eigenvalues(ctypes.byref(self._ctx), _dp(out), self.n_components)
-        return out[:n]
+        self._require()
+        return self._getter(self.lib.ssa_opt_get_eigenvalues)
+
+    def _getter(self, fn):
+        out = np.zeros(self.ctx.n_components)
+        n = fn(self.ref, _dp(out), out.size)
+        return out[:max(n, 0)]
 
     @property
     def total_variance(self) -> float:
-        self._check()
-        return self._l.ssa_opt_get_total_variance(ctypes.byref(self._ctx))
+        self._require()
+        return self.lib.ssa_opt_get_total_variance(self.ref)
+
+    def _grouped(self, fn, group, n, *extra, what):
+        self._require()
+        g = _i32(group)
+        return self._out(n, fn, _ip(g), g.size, *extra, what=what)
 
     def reconstruct(self, group: List[int]) -> np.ndarray:
-        self._check()
-        out = np.zeros(self.N)
-        g = np.ascontiguousarray(group, dtype=np.int32)
-        if self._l.ssa_opt_reconstruct(ctypes.byref(self._ctx), _ip(g), len(group), _dp(out)) != 0:
-            raise RuntimeError("Reconstruction failed")
-        return out
+        return self._grouped(self.lib.ssa_opt_reconstruct, group, self.N, what=f"reconstruction of {list(group)} failed")
 
     def forecast(self, group: List[int], n_forecast: int) -> np.ndarray:
-        self._check()
-        out = np.zeros(n_forecast)
-        g = np.ascontiguousarray(group, dtype=np.int32)
-        if self._l.ssa_opt_forecast(ctypes.byref(self._ctx), _ip(g), len(group), n_forecast, _dp(out)) != 0:
-            raise RuntimeError("Forecast failed (verticality >= 1?)")
-        return out
+        return self._grouped(self.lib.ssa_opt_forecast, group, n_forecast, int(n_forecast),
+                             what="R-forecast failed (a group index out of range, or verticality >= 1)")
 
     def forecast_full(self, group: List[int], n_forecast: int) -> np.ndarray:
-        self._check()
-        out = np.zeros(self.N + n_forecast)
-        g = np.ascontiguousarray(group, dtype=np.int32)
-        if self._l.ssa_opt_forecast_full(ctypes.byref(self._ctx), _ip(g), len(group), n_forecast, _dp(out)) != 0:
-            raise RuntimeError("Forecast failed")
-        return out
+        return self._grouped(self.lib.ssa_opt_forecast_full, group, self.N + n_forecast, int(n_forecast),
+                             what="R-forecast failed (a group index out of range, or verticality >= 1)")
 
     def vforecast(self, group: List[int], n_forecast: int) -> np.ndarray:
         """V-forecast (MKL/py/ssa_wrapper.py:880-900, MKL/ssa_opt.h:2906-2965)."""
-        self._check()
-        out = np.zeros(n_forecast)
-        g = np.ascontiguousarray(group, dtype=np.int32)
-        if self._l.ssa_opt_vforecast(ctypes.byref(self._ctx), _ip(g), len(group), n_forecast, _dp(out)) != 0:
-            raise RuntimeError("V-forecast failed")
-        return out
+        return self._grouped(self.lib.ssa_opt_vforecast, group, n_forecast, int(n_forecast),
+                             what="V-forecast failed (a group index out of range, or verticality >= 1)")
 
     def vforecast_full(self, group: List[int], n_forecast: int) -> np.ndarray:
-        self._check()
-        out = np.zeros(self.N + n_forecast)
-        g = np.ascontiguousarray(group, dtype=np.int32)
-        if self._l.ssa_opt_vforecast_full(ctypes.byref(self._ctx), _ip(g), len(group), n_forecast, _dp(out)) != 0:
-            raise RuntimeError("V-forecast failed")
-        return out
+        return self._grouped(self.lib.ssa_opt_vforecast_full, group, self.N + n_forecast, int(n_forecast),
+                             what="V-forecast failed (a group index out of range, or verticality >= 1)")
 
     def vforecast_fast(self, group: List[int], base_signal, n_forecast: int) -> np.ndarray:
         """V-forecast continuing base_signal (its last L-1 values)."""
-        self._check()
-        out = np.zeros(n_forecast)
-        g = np.ascontiguousarray(group, dtype=np.int32)
-        base = np.ascontiguousarray(base_signal, dtype=np.float64)
-        if self._l.ssa_opt_vforecast_fast(ctypes.byref(self._ctx), _ip(g), len(group), _dp(base), len(base),
-                                          n_forecast, _dp(out)) != 0:
-            raise RuntimeError("V-forecast failed")
-        return out
+        base = _f64(base_signal)
+        return self._grouped(self.lib.ssa_opt_vforecast_fast, group, n_forecast, _dp(base), base.size,
+                             int(n_forecast), what="V-forecast failed (base shorter than L - 1, or bad group)")
 
     def cadzow_inplace(self, rank: int, max_iter: int = 20, tol: float = 1e-9):
         """Cadzow on this context's signal (MKL/ssa_opt.h:3319-3440):
         (iterations, final_diff, converged); the context keeps the result."""
         r = _SSA_CadzowResult()
-        it = self._l.ssa_opt_cadzow_inplace(ctypes.byref(self._ctx), rank, max_iter, tol, ctypes.byref(r))
-        if it < 0:
-            raise RuntimeError("Cadzow iterations failed")
-        self.n_components = rank
-        self._decomposed = True
+        if self.lib.ssa_opt_cadzow_inplace(self.ref, int(rank), int(max_iter), float(tol), ctypes.byref(r)) < 0:
+            raise RuntimeError(f"Cadzow iterations at rank {rank} failed")
         return r.iterations, r.final_diff, r.converged > 0.5
 
     def parestimate(self, group: List[int] = None) -> dict:
         """ESPRIT periods, frequencies, moduli and rates of the group's
         eigenvectors (all computed components when group is None;
         MKL/ssa_opt.h:4304-4558)."""
-        self._check()
+        self._require()
         r = _SSA_ParEstimate()
-        if group is None:
-            rc = self._l.ssa_opt_parestimate(ctypes.byref(self._ctx), None, 0, ctypes.byref(r))
-        else:
-            g = np.ascontiguousarray(group, dtype=np.int32)
-            rc = self._l.ssa_opt_parestimate(ctypes.byref(self._ctx), _ip(g), len(g), ctypes.byref(r))
-        if rc != 0:
-            raise RuntimeError("ESPRIT failed")
-        n = r.n_components
-        out = {f: np.ctypeslib.as_array(getattr(r, f), shape=(n,)).copy()
-               for f in ("periods", "frequencies", "moduli", "rates")}
-        self._l.ssa_opt_parestimate_free(ctypes.byref(r))
-        return out
+        g = None if group is None else _i32(group)
+        rc = self.lib.ssa_opt_parestimate(self.ref, None if g is None else _ip(g), 0 if g is None else g.size,
+                                          ctypes.byref(r))
+        _checked(rc, "ESPRIT estimation failed")
+        fields = ("periods", "frequencies", "moduli", "rates")
+        est = {f: np.ctypeslib.as_array(getattr(r, f), shape=(r.n_components,)).copy() for f in fields}
+        self.lib.ssa_opt_parestimate_free(ctypes.byref(r))
+        return est
 
     def lrf(self, group: List[int]):
-        self._check()
-        g = np.ascontiguousarray(group, dtype=np.int32)
-        lrf = _SSA_LRF()
-        rc = self._l.ssa_opt_compute_lrf(ctypes.byref(self._ctx), _ip(g), len(group), ctypes.byref(lrf))
-        R = np.ctypeslib.as_array(lrf.R, shape=(self.L - 1,)).copy() if rc == 0 else None
-        nu = lrf.verticality
-        self._l.ssa_opt_lrf_free(ctypes.byref(lrf))
-        return rc, R, nu
+        """(status, R coefficients or None, verticality) of ssa_opt_compute_lrf."""
+        self._require()
+        g = _i32(group)
+        f = _SSA_LRF()
+        rc = self.lib.ssa_opt_compute_lrf(self.ref, _ip(g), g.size, ctypes.byref(f))
+        coef = np.ctypeslib.as_array(f.R, shape=(self.L - 1,)).copy() if rc == 0 else None
+        nu = f.verticality
+        self.lib.ssa_opt_lrf_free(ctypes.byref(f))
+        return rc, coef, nu
 
     def get_trend(self) -> np.ndarray:
-        self._check()
-        out = np.zeros(self.N)
-        if self._l.ssa_opt_get_trend(ctypes.byref(self._ctx), _dp(out)) != 0:
-            raise RuntimeError("get_trend failed")
-        return out
+        self._require()
+        return self._out(self.N, self.lib.ssa_opt_get_trend, what="trend reconstruction failed")
 
     def get_noise(self, noise_start: int) -> np.ndarray:
-        self._check()
-        out = np.zeros(self.N)
-        if self._l.ssa_opt_get_noise(ctypes.byref(self._ctx), noise_start, _dp(out)) != 0:
-            raise RuntimeError("get_noise failed")
-        return out
+        self._require()
+        return self._out(self.N, self.lib.ssa_opt_get_noise, int(noise_start), what="noise reconstruction failed")
 
     def wcorr_matrix(self) -> np.ndarray:
-        self._check()
-        W = np.zeros((self.n_components, self.n_components))
-        if self._l.ssa_opt_wcorr_matrix(ctypes.byref(self._ctx), _dp(W)) != 0:
-            raise RuntimeError("W-correlation computation failed")
-        return W
+        self._require()
+        k = self.ctx.n_components
+        return self._out(k * k, self.lib.ssa_opt_wcorr_matrix, what="W-correlation failed").reshape(k, k)
 
     def wcorr(self, i: int, j: int) -> float:
-        self._check()
-        return self._l.ssa_opt_wcorr_pair(ctypes.byref(self._ctx), i, j)
+        self._require()
+        return self.lib.ssa_opt_wcorr_pair(self.ref, int(i), int(j))
 
     def variance_explained(self, start: int = 0, end: int = -1) -> float:
-        self._check()
-        return self._l.ssa_opt_variance_explained(ctypes.byref(self._ctx), start, end)
+        self._require()
+        return self.lib.ssa_opt_variance_explained(self.ref, int(start), int(end))
 
     def find_periodic_pairs(self, max_pairs: int = 10, sv_tol: float = 0.1, wcorr_thresh: float = 0.5):
-        self._check()
+        self._require()
         pairs = np.zeros(2 * max_pairs, dtype=np.int32)
-        n = self._l.ssa_opt_find_periodic_pairs(ctypes.byref(self._ctx), _ip(pairs), max_pairs, sv_tol, wcorr_thresh)
-        return [(int(pairs[2 * i]), int(pairs[2 * i + 1])) for i in range(n)]
+        n = self.lib.ssa_opt_find_periodic_pairs(self.ref, _ip(pairs), int(max_pairs), float(sv_tol),
+                                                 float(wcorr_thresh))
+        return [tuple(int(v) for v in pairs[2 * i:2 * i + 2]) for i in range(n)]
 
 
-class MSSA:
-    """Multivariate SSA on the device (MKL/py/ssa_wrapper.py:1208-1371)."""
+class MSSA(_Device):
+    """Multivariate SSA on the device, with the interface of the reference
+    wrapper's MSSA class (MKL/py/ssa_wrapper.py:1208-1371)."""
+
+    _struct = _MSSA_Opt
+    _free = "mssa_opt_free"
 
     def __init__(self, X, L: int, q: int = None, precision: int = None, seed: int = None):
-        self._l = lib()
-        self._ctx = _MSSA_Opt()
-        self._X = np.ascontiguousarray(X, dtype=np.float64)
-        if self._X.ndim != 2:
-            raise ValueError("X must be 2D array (M, N)")
-        self.M, self.N = self._X.shape
-        self.L = L
-        self.K = self.N - L + 1
-        self.n_components = 0
-        self._decomposed = False
-        if self._l.mssa_opt_init(ctypes.byref(self._ctx), _dp(self._X), self.M, self.N, L) != 0:
-            raise ValueError(f"MSSA initialization failed (M={self.M}, N={self.N}, L={L})")
-        if q is not None or precision is not None or seed is not None:
-            if self._l.mssa_amd_set_options(ctypes.byref(self._ctx), 2 if q is None else q,
-                                            0 if precision is None else precision,
-                                            42 if seed is None else seed) != 0:
-                raise RuntimeError("set_options failed")
+        super().__init__()
+        series = _f64(X)
+        if series.ndim != 2:
+            raise ValueError(f"MSSA takes an (M, N) array of channels, got {series.ndim} dimensions")
+        M, N = series.shape
+        if self.lib.mssa_opt_init(self.ref, _dp(series), M, N, int(L)) != 0:
+            raise ValueError(f"mssa_opt_init refused {M} channels of {N} samples with window {L}")
+        if (q, precision, seed) != (None, None, None):
+            _checked(self.lib.mssa_amd_set_options(self.ref, 2 if q is None else q, 0 if precision is None else precision,
+                                                   42 if seed is None else seed), "MSSA options refused")
 
-    def __del__(self):
-        if hasattr(self, "_ctx") and getattr(self, "_l", None) is not None:
-            self._l.mssa_opt_free(ctypes.byref(self._ctx))
+    M = property(lambda self: self.ctx.M)
+    N = property(lambda self: self.ctx.N)
+    L = property(lambda self: self.ctx.L)
+    K = property(lambda self: self.ctx.K)
+    n_components = property(lambda self: self.ctx.n_components if self.ctx.decomposed else 0)
 
     def decompose(self, k: int, oversampling: int = 8) -> "MSSA":
-        if self._l.mssa_opt_decompose(ctypes.byref(self._ctx), k, oversampling) != 0:
-            raise RuntimeError(f"MSSA decomposition failed (k={k})")
-        self.n_components = k
-        self._decomposed = True
+        _checked(self.lib.mssa_opt_decompose(self.ref, int(k), int(oversampling)),
+                 f"MSSA decomposition into {k} components failed")
         return self
 
     def reconstruct(self, series_idx: int, group: List[int]) -> np.ndarray:
-        if not self._decomposed:
-            raise RuntimeError("Call decompose() first")
-        out = np.zeros(self.N)
-        g = np.ascontiguousarray(group, dtype=np.int32)
-        if self._l.mssa_opt_reconstruct(ctypes.byref(self._ctx), series_idx, _ip(g), len(group), _dp(out)) != 0:
-            raise RuntimeError("MSSA reconstruction failed")
-        return out
+        self._require()
+        g = _i32(group)
+        return self._out(self.N, self.lib.mssa_opt_reconstruct, int(series_idx), _ip(g), g.size,
+                         what=f"reconstruction of channel {series_idx} failed")
 
     def reconstruct_all(self, group: List[int]) -> np.ndarray:
-        if not self._decomposed:
-            raise RuntimeError("Call decompose() first")
-        out = np.zeros(self.M * self.N)
-        g = np.ascontiguousarray(group, dtype=np.int32)
-        if self._l.mssa_opt_reconstruct_all(ctypes.byref(self._ctx), _ip(g), len(group), _dp(out)) != 0:
-            raise RuntimeError("MSSA reconstruction failed")
+        self._require()
+        g = _i32(group)
+        out = self._out(self.M * self.N, self.lib.mssa_opt_reconstruct_all, _ip(g), g.size,
+                        what="reconstruction of every channel failed")
         return out.reshape(self.M, self.N)
 
     def series_contributions(self) -> np.ndarray:
-        if not self._decomposed:
-            raise RuntimeError("Call decompose() first")
-        out = np.zeros(self.M * self.n_components)
-        if self._l.mssa_opt_series_contributions(ctypes.byref(self._ctx), _dp(out)) != 0:
-            raise RuntimeError("series_contributions failed")
-        return out.reshape(self.M, self.n_components)
+        self._require()
+        k = self.ctx.n_components
+        return self._out(self.M * k, self.lib.mssa_opt_series_contributions,
+                         what="channel contributions failed").reshape(self.M, k)
 
     def variance_explained(self, start: int = 0, end: int = -1) -> float:
-        return self._l.mssa_opt_variance_explained(ctypes.byref(self._ctx), start, end)
+        return self.lib.mssa_opt_variance_explained(self.ref, int(start), int(end))
 
 
 class Batch:

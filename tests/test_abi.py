@@ -112,6 +112,30 @@ int main(void) {
     assert s1 <= 1024 and s2 <= 1024 and s3 <= 256
 
 
+def test_python_context_mirrors_match_header():
+    """The typed ctypes mirrors of SSA_Opt and MSSA_Opt in ssa_amd.py have the
+    header's size and every field at the header's offset."""
+    import ssa_amd
+    lines = []
+    for cname, py in (("SSA_Opt", ssa_amd._SSA_Opt), ("MSSA_Opt", ssa_amd._MSSA_Opt)):
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'    printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    prog = '#include <stdio.h>\n#include <stddef.h>\n#include "ssa_opt.h"\nint main(void) {\n' + \
+        "\n".join(lines) + "\n    return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        open(c, "w").write(prog)
+        exe = os.path.join(d, "t")
+        subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(HDR), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.splitlines()
+    got = {(a, b): int(v) for a, b, v in (ln.split() for ln in out)}
+    for cname, py in (("SSA_Opt", ssa_amd._SSA_Opt), ("MSSA_Opt", ssa_amd._MSSA_Opt)):
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
 class Plan(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("lgn", ctypes.c_int), ("CL", ctypes.c_int), ("CK", ctypes.c_int),
                 ("AL", ctypes.c_int), ("SK", ctypes.c_int), ("cost", ctypes.c_double)]

@@ -33,7 +33,7 @@ def rms(x):
 
 def _sv(s, n=80):
     out = np.zeros(n)
-    m = s._l.ssa_opt_get_singular_values(ctypes.byref(s._ctx), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+    m = s.lib.ssa_opt_get_singular_values(s.ref, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
     return out[:m]
 
 
@@ -59,15 +59,14 @@ def test_subspace_entry_points_match_oracle():
     o = _oracle(x, L, 6, 20, 2, call=1)
     assert np.max(np.abs(_sv(s) - o.sigma) / o.sigma) < 1e-10
     # extend by 4: a sequential decomposition of k = 10 (kp = 18, q = 10)
-    assert s._l.ssa_opt_extend(ctypes.byref(s._ctx), 4, 40) == 0
-    s.n_components = 10
+    assert s.lib.ssa_opt_extend(s.ref, 4, 40) == 0
     o = _oracle(x, L, 10, 18, 10, call=2)
     sv = _sv(s)
     assert len(sv) == 10
     assert np.max(np.abs(sv - o.sigma) / o.sigma) < 1e-10
     assert np.abs(s.reconstruct([3, 4]) - o.reconstruct([3, 4])).max() / r < 1e-9
     # extend requires a decomposition; invalid arguments are refused
-    assert s._l.ssa_opt_extend(ctypes.byref(s._ctx), 0, 40) == -1
+    assert s.lib.ssa_opt_extend(s.ref, 0, 40) == -1
 
 
 def _pair_signal(N=2000):
@@ -104,7 +103,7 @@ def test_wcorr_pair_trend_noise_and_stats():
                     ("gap_threshold", ctypes.c_double)]
 
     st = Stats()
-    assert s._l.ssa_opt_component_stats(ctypes.byref(s._ctx), ctypes.byref(st)) == 0
+    assert s.lib.ssa_opt_component_stats(s.ref, ctypes.byref(st)) == 0
     try:
         so = o.sigma
         n = st.n
@@ -123,7 +122,7 @@ def test_wcorr_pair_trend_noise_and_stats():
         assert sd[0] == sd[-1] == 0.0
         assert np.allclose(sd[1:-1], ls[:-2] - 2 * ls[1:-1] + ls[2:], rtol=0, atol=1e-9)
     finally:
-        s._l.ssa_opt_component_stats_free(ctypes.byref(st))
+        s.lib.ssa_opt_component_stats_free(ctypes.byref(st))
 
 
 def test_periodic_pairs_on_two_sinusoids():

@@ -134,7 +134,7 @@ def test_state_machine_refusals():
     S = _ssa()
     x = workloads.single(1000, seed=1)
     s = S.SSA(x, 250)
-    lib, ctx = s._l, ctypes.byref(s._ctx)
+    lib, ctx = s.lib, s.ref
     assert lib.ssa_opt_decompose_randomized(ctx, 10, 10) == -1          # not prepared
     assert lib.ssa_opt_prepare(ctx, 10, 10) == 0
     assert lib.ssa_opt_decompose_randomized(ctx, 20, 10) == -1          # kp 30 > prepared 20
@@ -156,7 +156,7 @@ def test_block_width_limit():
     S = _ssa()
     x = workloads.single(2000, seed=1)
     s = S.SSA(x, 500)
-    lib, ctx = s._l, ctypes.byref(s._ctx)
+    lib, ctx = s.lib, s.ref
     assert lib.ssa_opt_prepare(ctx, 75, 10) == -1
     assert lib.ssa_opt_decompose(ctx, 80, 20) == -1
     assert lib.ssa_opt_decompose_block(ctx, 81, 16, 20) == -1

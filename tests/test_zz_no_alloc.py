@@ -18,29 +18,13 @@ import workloads
 pytestmark = pytest.mark.gpu
 
 
-class _Head(ctypes.Structure):
-    # the leading fields of SSA_Opt (include/ssa_opt.h) up to sigma
-    _fields_ = [("N", ctypes.c_int), ("L", ctypes.c_int), ("K", ctypes.c_int), ("fft_len", ctypes.c_int),
-                ("r2c_len", ctypes.c_int), ("batch_size", ctypes.c_int), ("fft_x", ctypes.c_void_p),
-                ("ws_real", ctypes.c_void_p), ("ws_complex", ctypes.c_void_p), ("ws_real2", ctypes.c_void_p),
-                ("ws_proj", ctypes.c_void_p), ("ws_batch_real", ctypes.c_void_p),
-                ("ws_batch_complex", ctypes.c_void_p), ("U", ctypes.c_void_p), ("V", ctypes.c_void_p),
-                ("sigma", ctypes.c_void_p)]
-
-
-def _host(addr, n):
-    return np.ctypeslib.as_array((ctypes.c_double * n).from_address(addr)).copy()
-
-
 def _context(N, L, k):
     import ssa_amd
     x = workloads.single(N, seed=N)
     s = ssa_amd.SSA(x, L, q=2, precision=0, seed=N)
     s.decompose(k, oversampling=10)
-    h = _Head.from_buffer(s._ctx)
-    assert (h.N, h.L) == (N, L)
-    U, V, sig = _host(h.U, L * k), _host(h.V, (N - L + 1) * k), _host(h.sigma, k)
-    return x, s, U, V, sig
+    U, V, sig = s.factors()
+    return x, s, np.asfortranarray(U).ravel(order="F"), np.asfortranarray(V).ravel(order="F"), sig
 
 
 def _call(lib, dU, dV, ds, N, L, k, group, out):
