@@ -376,6 +376,27 @@ def single_configs(reps=3):
     return out
 
 
+def wcorr_workload(reps=10):
+    """A workload for counter runs of the W-correlation Gram (the weighted
+    SYRK of MKL/ssa_opt.h:2473): cfg2 (N = 20000, L = 5000, k = 50, p = 10,
+    q = 2), one decomposition per form followed by `reps` W-correlations.  The
+    decomposition's own Gram launches come first (8 plain, 6 mixed); each
+    W-correlation then launches one Gram over the N x k weighted panel."""
+    global ssa_amd
+    import ssa_amd as _ssa_amd
+    ssa_amd = _ssa_amd
+    x = workloads.single(20000, seed=2)
+    for prec, name in ((0, "plain"), (1, "mixed")):
+        s = ssa_amd.SSA(x, 5000, q=Q_POWER, precision=prec, seed=12)
+        s.decompose(50, oversampling=P_OVERSAMPLE)
+        t = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            s.wcorr_matrix()
+            t.append(time.perf_counter() - a)
+        print(f"[bench] wcorr {name}: median {1e3 * float(np.median(t)):.3f} ms over {reps}", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -390,11 +411,17 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="plan with the cost model instead of measured costs")
+    ap.add_argument("--wcorr-only", action="store_true",
+                    help="profiling workload: one cfg2 decomposition per form, then 10 W-correlations each")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this SSA path runs on one GPU; --gpus {args.gpus} is not supported "
               f"(series of a batch would shard across GPUs with no exchange)", file=sys.stderr)
         sys.exit(2)
+
+    if args.wcorr_only:
+        wcorr_workload()
+        return
 
     xs, Ns, Ls, ks = workloads.ragged_batch(count=args.count, seed=4, dtype=np.float32)
     print(f"[bench] batch: {len(xs)} series, sum N = {int(Ns.sum())}", flush=True)
