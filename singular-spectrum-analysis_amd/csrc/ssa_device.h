@@ -56,6 +56,9 @@ typedef struct ssa_desc {
      * c - M*AL of V), the n/2 + 1 complex values of T-(chunk) at
      * off_cache + 2 ((g (M AL + SK) + c) (n/2 + 1)) */
     long long off_cache;
+    /* the batched step's output (N + h values) gathered here, every series'
+     * slot after the previous one, so that one copy moves all of them */
+    long long off_gather;
 } ssa_desc;
 
 /* FP64 small-matrix slots, each SSA_BMAX*SSA_BMAX doubles, relative to off_small.
@@ -139,6 +142,8 @@ int ssa_dev_cache_build(int prec, int lgn, const ssa_desc *D, const ssa_item *it
 int ssa_dev_recon_cached(int prec, int lgn, int scale_sigma, const ssa_desc *D, const ssa_item *items, int n_items,
                          int n_pieces, const int *gdesc, const int *glist, void *bulk, const double *small,
                          const void *tw, void *stream);
+/* copy every series' step output (N + h values at off_out) to its off_gather slot; items (s) */
+int ssa_dev_gather(int prec, int h, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, void *stream);
 /* the first and last SSA_EDGE samples of every group's reconstruction by direct
  * anti-diagonal sums (FP64), overwriting the overlap-add's values there; items
  * (s, gid); scale_sigma, weight, limit_gsize and groups as in ssa_dev_recon / ssa_dev_olap */

@@ -397,8 +397,12 @@ static int engine_layout(ssa_engine *e, int count, const int *N, const int *L, c
         d->tilesY = ceil_div(d->ML, SSA_GRAM_ROWS);
         d->tilesZ = ceil_div(d->K, SSA_GRAM_ROWS);
         d->seed = 42;
+        /* every series' signal one after the other, so that one copy uploads them all */
         d->off_x = off; off = align64(off + (long long)m * d->N);
-        d->off_spec = off; off = align64(off + (long long)m * d->AL * d->SK * 2 * d->n);
+    }
+    for (int s = 0; s < count; s++) {
+        ssa_desc *d = &e->hd[s];
+        d->off_spec = off; off = align64(off + (long long)d->M * d->AL * d->SK * 2 * d->n);
     }
     e->a_elems = off;
     if (off > e->a_cap || !e->arena_a) {
@@ -530,20 +534,19 @@ int ssa_engine_upload_signal(ssa_engine *e, int s, const double *x)
 
 int ssa_engine_upload_signals_f32(ssa_engine *e, const float *cat, const long long *offsets)
 {
-    /* all signals through one pinned staging buffer, one synchronisation */
-    size_t total = 0;
-    for (int s = 0; s < e->count; s++) total += (size_t)e->hd[s].M * e->hd[s].N;
+    /* the signals sit one after the other at the start of the arena
+     * (engine_layout): staged in that layout, they go up in one copy */
+    const ssa_desc *last = &e->hd[e->count - 1];
+    const size_t total = (size_t)(last->off_x + (long long)last->M * last->N);
     DEVCHK(pin_reserve(e, total * e->esz));
-    size_t at = 0;
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
-        size_t n = (size_t)d->M * d->N;
-        char *dst = (char *)e->hpin + at * e->esz;
+        const size_t n = (size_t)d->M * d->N;
+        char *dst = (char *)e->hpin + (size_t)d->off_x * e->esz;
         if (e->prec == SSA_PREC_F32) memcpy(dst, cat + offsets[s], n * 4);
         else for (size_t i = 0; i < n; i++) ((double *)dst)[i] = cat[offsets[s] + i];
-        HIPCHK(hipMemcpyAsync((char *)e->arena_a + e->esz * d->off_x, dst, n * e->esz, hipMemcpyHostToDevice, e->st));
-        at += n;
     }
+    HIPCHK(hipMemcpyAsync(e->arena_a, e->hpin, total * e->esz, hipMemcpyHostToDevice, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
     return 0;
 }
@@ -587,6 +590,7 @@ long long ssa_engine_series_bytes(int prec, int N, int L, int b, int k, int max_
     long long bulk = align64(ML * b) + align64(K * b) + align64(ML * k) + align64(K * k);
     bulk += align64((long long)gmax * pl.AL * pl.SK * pl.n);
     bulk += align64((long long)gmax * N + max_h + 64);
+    bulk += align64((long long)N + max_h);   /* gathered step output */
     int tilesY = ceil_div((int)ML, SSA_GRAM_ROWS), tilesZ = ceil_div((int)K, SSA_GRAM_ROWS);
     int tiles = tilesY > tilesZ ? tilesY : tilesZ, tout = ceil_div(N, SSA_GRAM_ROWS);
     if (tout > tiles) tiles = tout;
@@ -627,6 +631,11 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
         d->off_part = soff; soff = align64(soff + part);
         d->off_small = soff; soff = align64(soff + (long long)SSA_NSLOTS * SSA_BMAX * SSA_BMAX);
     }
+    /* the step outputs gathered one after the other (one copy to the host) */
+    for (int s = 0; s < e->count; s++) {
+        ssa_desc *d = &e->hd[s];
+        d->off_gather = off; off = align64(off + d->N + max_h);
+    }
     if (off > e->b_cap || !e->arena_b) {
         if (e->arena_b) { hipFree(e->arena_b); e->arena_b = NULL; e->b_cap = 0; }
         const long long cap = off > 64 ? off + off / 4 : 64;
@@ -648,7 +657,7 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
     for (int s = 0; s < e->count; s++) {
         ssa_desc *d = &e->hd[s];
         d->off_Y += delta; d->off_Z += delta; d->off_U += delta; d->off_V += delta;
-        d->off_stage += delta; d->off_out += delta;
+        d->off_stage += delta; d->off_out += delta; d->off_gather += delta;
         if (d->b == 0) { d->b = e->bmax[s]; d->k = e->kmax[s]; }
         if (d->gsize <= 0 || d->gsize > d->k) d->gsize = d->k;
     }
@@ -1561,29 +1570,29 @@ int ssa_engine_profile_step(ssa_engine *e, int q, int max_rec, int *phase, int *
 int ssa_engine_step_outputs_all(ssa_engine *e, double *out, const long long *offsets, int *status)
 {
     if (!e->prepared || !out || !offsets) return -1;
-    long long total = 0;
-    for (int s = 0; s < e->count; s++) total += e->hd[s].N + e->step_h;
+    DEVCHK(sync_desc(e));
+    /* every series' output into its gather slot (one after the other in the
+     * panel arena), then one copy of the whole gather region */
+    DEVCHK(ssa_dev_gather(e->prec, e->step_h, e->dd, it_ptr(e, e->ser), e->ser.n, e->arena_a, e->st));
+    const ssa_desc *first = &e->hd[0], *last = &e->hd[e->count - 1];
+    const long long span_n = last->off_gather + last->N + e->step_h - first->off_gather;
     const size_t dbytes = status ? sizeof(ssa_desc) * (size_t)e->count : 0;
-    DEVCHK(pin_reserve(e, (size_t)total * e->esz + dbytes));
-    long long at = 0;
-    for (int s = 0; s < e->count; s++) {
-        const ssa_desc *d = &e->hd[s];
-        const long long n = d->N + e->step_h;
-        HIPCHK(hipMemcpyAsync((char *)e->hpin + at * e->esz, (char *)e->arena_a + e->esz * d->off_out, n * e->esz,
-                              hipMemcpyDeviceToHost, e->st));
-        at += n;
-    }
-    ssa_desc *hd = (ssa_desc *)((char *)e->hpin + (size_t)total * e->esz);
+    const size_t obytes = ((size_t)span_n * e->esz + 255) & ~(size_t)255;
+    DEVCHK(pin_reserve(e, obytes + dbytes));
+    HIPCHK(hipMemcpyAsync(e->hpin, (char *)e->arena_a + e->esz * first->off_gather, (size_t)span_n * e->esz,
+                          hipMemcpyDeviceToHost, e->st));
+    ssa_desc *hd = (ssa_desc *)((char *)e->hpin + obytes);
     if (status) HIPCHK(hipMemcpyAsync(hd, e->dd, dbytes, hipMemcpyDeviceToHost, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
-    at = 0;
     for (int s = 0; s < e->count; s++) {
-        const long long n = e->hd[s].N + e->step_h;
+        const long long n = e->hd[s].N + e->step_h, at = e->hd[s].off_gather - first->off_gather;
         double *o = out + offsets[s];
-        if (e->prec == SSA_PREC_F64) memcpy(o, (double *)e->hpin + at, n * 8);
-        else for (long long i = 0; i < n; i++) o[i] = ((float *)e->hpin)[at + i];
+        if (e->prec == SSA_PREC_F64) memcpy(o, (const double *)e->hpin + at, n * 8);
+        else {
+            const float *src = (const float *)e->hpin + at;
+            for (long long i = 0; i < n; i++) o[i] = src[i];
+        }
         if (status) status[s] = hd[s].status;
-        at += n;
     }
     return 0;
 }

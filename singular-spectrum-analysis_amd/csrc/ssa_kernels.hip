@@ -1604,6 +1604,17 @@ __global__ __launch_bounds__(2 * SSA_EDGE) void k_edge(const ssa_desc *__restric
     bulk[d.off_out + (long long)it.x * N + t] = (T)(acc * (weight ? sqrt(w) : w));
 }
 
+// every series' step output (reconstruction + h forecasts) into its gather slot
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items,
+                                                T *bulk, int h)
+{
+    const ssa_desc d = D[items[blockIdx.x].s];
+    const T *src = bulk + d.off_out;
+    T *dst = bulk + d.off_gather;
+    for (int t = threadIdx.x; t < d.N + h; t += 256) dst[t] = src[t];
+}
+
 // W-correlation normalisation: W_ij = G_ij / sqrt(G_ii G_jj), rows whose
 // sigma * ||h||_w <= 1e-12 are zero (MKL/ssa_opt.h:2466-2471); row-major k x k.
 __global__ __launch_bounds__(256) void k_wnorm(const ssa_desc *D, const ssa_item *items, double *small)
@@ -2092,6 +2103,18 @@ int ssa_dev_edge(int prec, int scale_sigma, int weight, int limit_gsize, const s
     else
         hipLaunchKernelGGL(k_edge<float>, dim3(n_items), dim3(2 * SSA_EDGE), 0, (hipStream_t)stream, D, items,
                            (const int4 *)gdesc, glist, (float *)bulk, small, scale_sigma, weight, limit_gsize);
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_gather(int prec, int h, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (prec == SSA_PREC_F64)
+        hipLaunchKernelGGL(k_gather<double>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
+                           (double *)bulk, h);
+    else
+        hipLaunchKernelGGL(k_gather<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
+                           (float *)bulk, h);
     return (int)hipGetLastError();
 }
 
