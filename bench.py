@@ -231,6 +231,48 @@ def cpu_reference(xs, Ls, ks, budget_s, workers):
 SERVE_BUDGET_GIB = 16
 
 
+def _oracle_request(args):
+    """One serving request by the FP64 oracle: decompose (q = 2, p = 10, the
+    request's Philox key), reconstruct the group 0..g-1 and R-forecast h = 50."""
+    x, L, k, g, key = args
+    import oracle as O
+    x = np.asarray(x, dtype=np.float64)
+    K = len(x) - L + 1
+    b = min(k + P_OVERSAMPLE, L, K)
+    o = O.OracleSSA(x, L).decompose(k, P_OVERSAMPLE, Q_POWER, O.philox_gaussian(key, 0, K, b))
+    o.reconstruct(list(range(g)))
+    if o.lrf(list(range(g)))[0] == 0:
+        o.forecast(list(range(g)), 50)
+    return len(x)
+
+
+def _serve_worker(w, workers, reqs, budget_s):
+    done = 0
+    t0 = time.perf_counter()
+    for i in range(w, len(reqs), workers):
+        _oracle_request(reqs[i])
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    return done, time.perf_counter() - t0
+
+
+def cpu_serving_baseline(budget_s, workers):
+    """The serving workload's CPU line: the FP64 oracle over the same request
+    distribution (N log-uniform in [500, 50000]), one process per core, each
+    over its share of the requests until budget_s.  The CPU cannot keep up
+    with the device's arrival rate, so this is its saturation throughput."""
+    import multiprocessing as mp
+    import serve
+    reqs = [(r.x, r.L, r.k, r.g, serve.request_key(r.rid, 6)) for r in serve.synthetic_requests(2048, rate=np.inf, seed=6)]
+    with mp.get_context("fork").Pool(workers) as pool:
+        res = pool.starmap(_serve_worker, [(w, workers, reqs, budget_s) for w in range(workers)])
+    done = sum(r[0] for r in res)
+    el = max(r[1] for r in res)
+    return {"capacity_req_s": round(done / el, 2), "cores": workers, "kind": "port",
+            "sample": f"{done} requests of the serving distribution over {workers} processes, {el:.1f} s"}
+
+
 def serving(seconds):
     """cfg5: requests with N log-uniform in [500, 50000], L = N // 4,
     k ~ U{10..50}, reconstruct + forecast h = 50 of components
@@ -253,12 +295,72 @@ def serving(seconds):
     for r in serve.synthetic_requests(48, rate=np.inf, seed=62):
         alone.extend(one.run([r]))
     mix, solo = serve.latency_stats(stream), serve.latency_stats(alone)
+    srv_wave_bytes = list(srv.wave_bytes)
+    del srv, one
+    # the plain FP64 form on the same requests: its capacity, then an open-loop
+    # stream at the mixed form's rate when it can sustain that, else at 70 %
+    # of its own capacity
+    psrv = serve.Server(max_wave=512, inflight=2, precision=0, hbm_budget=SERVE_BUDGET_GIB * 2 ** 30)
+    psrv.warm(sorted(serve.synthetic_requests(4096, rate=np.inf, seed=63), key=lambda r: -len(r.x))[:512])
+    pdone = psrv.run(serve.synthetic_requests(2048, rate=np.inf, seed=6))
+    pcap = len(pdone) / max(r.done for r in pdone)
+    prate = rate if rate <= 0.7 * pcap else 0.7 * pcap
+    pstream = psrv.run(serve.synthetic_requests(max(64, int(prate * seconds)), rate=prate, seed=61))
+    plain = {"capacity_req_s": round(pcap, 1), "rate_req_s": round(prate, 1),
+             "rate_is": "the mixed form's rate" if prate == rate else "70 % of its own capacity",
+             "in_mix": serve.latency_stats(pstream)}
     return {"workload": "N log-uniform in [500, 50000], L = N // 4, k ~ U{10..50}, group 0..floor(2k/5)-1, "
-                        "h = 50, Poisson arrivals, mixed form",
-            "hbm_budget_GiB": SERVE_BUDGET_GIB, "max_wave_GB": round(max(srv.wave_bytes) / 1e9, 2),
+                        "h = 50, Poisson arrivals, mixed form (plain FP64 beside it)",
+            "hbm_budget_GiB": SERVE_BUDGET_GIB, "max_wave_GB": round(max(srv_wave_bytes) / 1e9, 2),
             "capacity_req_s": round(capacity, 1), "rate_req_s": round(rate, 1),
             "in_mix": mix, "alone": solo,
-            "p50_in_mix_over_alone": round(mix["p50_ms"] / solo["p50_ms"], 3)}
+            "p50_in_mix_over_alone": round(mix["p50_ms"] / solo["p50_ms"], 3),
+            "plain_fp64": plain}
+
+
+def dense_errors():
+    """Error of both forms against a dense FP64 LAPACK SVD of the explicit
+    Hankel matrix (numpy) at N = 4000, L = 1000, k = 50, p = 10, q = 2: every
+    singular value; the reconstructions of the groups [0], [1, 2], [3, 4],
+    0..9 and 0..49 (each dense one by diagonal averaging of sum sigma_i u_i v_i^T,
+    as the convolution of u_i and v_i over the anti-diagonal counts); the
+    50-step R-forecast of the group 0..4 (LRF from the dense U,
+    MKL/ssa_opt.h:2766-2841).  Relative to RMS(x) for reconstructions and
+    forecasts.  The q = 2 randomized SVD itself differs from the dense one on
+    the noise-level components, equally in both forms."""
+    N, L, k = 4000, 1000, 50
+    K = N - L + 1
+    xd = workloads.single(N, seed=3)
+    H = np.lib.stride_tricks.sliding_window_view(xd, K)[:L]
+    Ud, sd, Vtd = np.linalg.svd(H, full_matrices=False)
+    cnt = np.convolve(np.ones(L), np.ones(K))
+    rms = float(np.sqrt(np.mean(xd ** 2)))
+
+    def rec(g):
+        return sum(sd[i] * np.convolve(Ud[:, i], Vtd[i]) for i in g) / cnt
+
+    def lrf_forecast(U, base, g, h):
+        pi = U[L - 1, g]
+        R = U[:L - 1, g] @ pi / (1.0 - float(pi @ pi))
+        buf = list(base[N - (L - 1):])
+        for _ in range(h):
+            buf.append(float(np.dot(R, buf[-(L - 1):])))
+        return np.array(buf[L - 1:])
+
+    groups = {"0": [0], "1-2": [1, 2], "3-4": [3, 4], "0-9": list(range(10)), "0-49": list(range(k))}
+    g5 = [0, 1, 2, 3, 4]
+    ref_f = lrf_forecast(Ud, rec(g5), g5, 50)
+    res = {}
+    for prec, name in ((0, "plain"), (1, "mixed")):
+        s = ssa_amd.SSA(xd, L, q=Q_POWER, precision=prec, seed=13).decompose(k, oversampling=P_OVERSAMPLE)
+        sv = s.singular_values
+        rel = np.abs(sv - sd[:k]) / sd[:k]
+        res[f"dense_sigma_err_{name}"] = float(rel.max())
+        res[f"dense_sigma_err_{name}_top10"] = float(rel[:10].max())
+        res[f"dense_recon_err_{name}"] = {gn: float(np.abs(s.reconstruct(g) - rec(g)).max() / rms)
+                                          for gn, g in groups.items()}
+        res[f"dense_forecast_err_{name}"] = float(np.abs(s.forecast(g5, 50) - ref_f).max() / rms)
+    return res
 
 
 def single_configs(reps=3):
@@ -312,12 +414,7 @@ def single_configs(reps=3):
             if r:
                 t.append(time.perf_counter() - a)
         out[f"cfg2_{name}_cached_ms"] = round(1e3 * float(np.median(t)), 2)
-    xd = workloads.single(4000, seed=3)
-    H = np.lib.stride_tricks.sliding_window_view(xd, 3001)[:1000]
-    exact = np.linalg.svd(H, compute_uv=False)[:50]
-    for prec, name in ((0, "plain"), (1, "mixed")):
-        s = ssa_amd.SSA(xd, 1000, q=Q_POWER, precision=prec, seed=13).decompose(50, oversampling=P_OVERSAMPLE)
-        out[f"dense_sigma_err_{name}"] = float(np.max(np.abs(s.singular_values - exact) / exact))
+    out.update(dense_errors())
     X = workloads.mssa_channels(M=16, N=10000, seed=5)
     for prec, name in ((0, "plain"), (1, "mixed")):
         m = ssa_amd.MSSA(X, 2500, q=Q_POWER, precision=prec)
@@ -435,6 +532,9 @@ def main():
     if cpu_ref:
         print(f"[bench] reference cpu (ssa_opt_ref.h, q=0): {cpu_ref['value']} SSA/s on {workers} cores, "
               f"{cpu_ref['value_1_core']} on one", flush=True)
+    cpu_serve = None if (args.no_cpu or args.no_serve) else cpu_serving_baseline(args.cpu_seconds, workers)
+    if cpu_serve:
+        print(f"[bench] cpu serving: {cpu_serve['capacity_req_s']} req/s on {workers} cores", flush=True)
 
     global ssa_amd
     import ssa_amd as _ssa_amd
@@ -471,6 +571,9 @@ def main():
         print(f"[bench] serving: capacity {serving_out['capacity_req_s']} req/s, at {serving_out['rate_req_s']} "
               f"req/s p50 {serving_out['in_mix']['p50_ms']} ms p99 {serving_out['in_mix']['p99_ms']} ms; alone "
               f"p50 {serving_out['alone']['p50_ms']} ms", flush=True)
+        pl = serving_out["plain_fp64"]
+        print(f"[bench] serving plain FP64: capacity {pl['capacity_req_s']} req/s, at {pl['rate_req_s']} req/s "
+              f"({pl['rate_is']}) p50 {pl['in_mix']['p50_ms']} ms p99 {pl['in_mix']['p99_ms']} ms", flush=True)
 
 
     configs_out = None
@@ -507,6 +610,8 @@ def main():
     out["output_check"] = {"mixed": checked, "plain": checked_plain}
     out["autotune"] = tuned
     out["serving"] = serving_out
+    if serving_out is not None:
+        serving_out["cpu_fp64"] = cpu_serve
     out["single_series"] = configs_out
     out["cpu_baseline"] = cpu
     out["cpu_reference"] = cpu_ref
