@@ -108,7 +108,10 @@ int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_item
 /* Gram / apply: every panel of the launch has at most nb*16 columns (1 <= nb <= 5) */
 int ssa_dev_gram(int prec, int panel, int use_m, int nb, const ssa_desc *D, const ssa_item *items, int n_items,
                  void *bulk, double *small, void *stream);
-int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_item *items, int n_items,
+/* Cholesky of the summed Gram; a pivot at or below tol times its column's
+ * original diagonal marks the column dependent (zero row of R, zero column
+ * of R^-1) */
+int ssa_dev_chol(int panel, int pass, int bmax, double tol, const ssa_desc *D, const ssa_item *items, int n_items,
                  double *small, void *stream);
 int ssa_dev_apply(int prec, int src, int dst, int slot, int ncols_k, int nb, const ssa_desc *D,
                   const ssa_item *items, int n_items, void *bulk, double *small, void *stream);

@@ -82,6 +82,7 @@ struct ssa_engine {
     int cholqr2_all;           /* CholeskyQR2 for every orthonormalisation (default: last one only) */
     int final2;                /* CholeskyQR2 for the last orthonormalisation of Y and for B */
     int mv_wide_min;           /* block-matvec launches of at least this many items take the wide geometry */
+    double dead_tol;           /* Cholesky pivot / original diagonal at or below which a column is dependent */
     /* component spectra cache (ssa_engine_cache_build): its own allocation,
      * addressed from arena_a like the panel arena */
     void *cache;
@@ -458,6 +459,17 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
          * when a launch fills the device; a launch of few items is bound by
          * each item's latency, which the narrow geometry's extra waves hide.
          * SSA_AMD_MV_WIDE_MIN moves the threshold (1: always wide). */
+        /* A panel column whose residual, after the earlier columns are
+         * projected out, is at the rounding floor of the panel's storage is
+         * dropped (its Q column is zero).  FP64 panels: pivot / diagonal
+         * 1e-13.  FP32 panels carry transform rounding of about 2.5e-7 of a
+         * column's norm, so a rank-deficient panel (a noise-free signal of
+         * rank below k + p) leaves pivots near 1e-13: kept, such a column is
+         * normalised rounding noise, R^-1 scales it by ~1e6 and the FP32
+         * apply's cancellation leans it into the signal span (measured: U
+         * orthogonal to 1e-3, singular values off by 3e-4).  The FP32 floor
+         * is 1e-10 (a residual of 1e-5, about 40x the rounding). */
+        e->dead_tol = prec == SSA_PREC_F64 ? 1e-13 : 1e-10;
         const char *w = getenv("SSA_AMD_MV_WIDE_MIN");
         e->mv_wide_min = (w && *w) ? atoi(w) : MV_WIDE_MIN_ITEMS;
         if (e->mv_wide_min < 1) e->mv_wide_min = 1;
@@ -851,7 +863,7 @@ static int apply_all(ssa_engine *e, int phase, int src, int dst, int slot, int n
 static int chol_all(ssa_engine *e, int panel, int pass, int bm)
 {
     prof_begin(e, PH_CHOL, panel);
-    DEVCHK(ssa_dev_chol(panel, pass, bm, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
+    DEVCHK(ssa_dev_chol(panel, pass, bm, e->dead_tol, e->dd, it_ptr(e, e->ser), e->ser.n, e->small, e->st));
     prof_end(e);
     return 0;
 }

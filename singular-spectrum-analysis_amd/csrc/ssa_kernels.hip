@@ -1907,8 +1907,8 @@ int ssa_dev_gram(int prec, int panel, int use_m, int nb, const ssa_desc *D, cons
     return (int)hipGetLastError();
 }
 
-int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_item *items, int n_items, double *small,
-                 void *stream)
+int ssa_dev_chol(int panel, int pass, int bmax, double tol, const ssa_desc *D, const ssa_item *items, int n_items,
+                 double *small, void *stream)
 {
     if (n_items <= 0) return 0;
     const size_t lds = 2 * (size_t)bmax * (bmax + 1) * sizeof(double);
@@ -1923,15 +1923,15 @@ int ssa_dev_chol(int panel, int pass, int bmax, const ssa_desc *D, const ssa_ite
         // one wave per series, the matrix in registers
         hipStream_t st = (hipStream_t)stream;
         switch (SSA_NB(bmax)) {
-        case 1: hipLaunchKernelGGL(k_chol_w<1>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, 1e-13); break;
-        case 2: hipLaunchKernelGGL(k_chol_w<2>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, 1e-13); break;
-        case 3: hipLaunchKernelGGL(k_chol_w<3>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, 1e-13); break;
-        default: hipLaunchKernelGGL(k_chol_w<4>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, 1e-13); break;
+        case 1: hipLaunchKernelGGL(k_chol_w<1>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, tol); break;
+        case 2: hipLaunchKernelGGL(k_chol_w<2>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, tol); break;
+        case 3: hipLaunchKernelGGL(k_chol_w<3>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, tol); break;
+        default: hipLaunchKernelGGL(k_chol_w<4>, dim3(n_items), dim3(64), 0, st, D, items, small, panel, pass, tol); break;
         }
         return (int)hipGetLastError();
     }
     hipLaunchKernelGGL(k_chol, dim3(n_items), dim3(256), lds, (hipStream_t)stream, D, items, small, panel, pass,
-                       1e-13);
+                       tol);
     return (int)hipGetLastError();
 }
 

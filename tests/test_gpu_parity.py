@@ -392,3 +392,49 @@ def test_batch_matches_single():
     b.sync()
     for i in range(6):
         assert np.array_equal(b.output(i), out0[i])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("case", ["offset", "rank2", "trend"])
+def test_hard_inputs(case, prec):
+    """Inputs with a wide dynamic range against the FP64 oracle: a large
+    constant offset over small noise (sigma_0 / sigma_1 = 1.7e6), a noise-free
+    sinusoid (rank 2) decomposed with k = 10 (18 of the panels' 20 columns
+    numerically dependent), and a steep trend over two weak sinusoids
+    (sigma_0 / sigma_5 = 1.6e3).
+
+    A group's rounding scales with sigma_0 over the group's smallest singular
+    value (the panels carry the group as a fraction of the leading
+    component): each group is held to max(t, 20 eps sigma_0 / sigma_min)
+    relative (2-norm), sigma_min over the group's components above 1e-9
+    sigma_0, with t = 1e-10 and eps = eps64 plain, t = 1e-5 and eps = eps32
+    mixed; the leading singular values to t.  The rank-2 case in the mixed
+    form needs the FP32 panels' dependent-column floor: with the FP64 floor
+    its 18 rounding-noise columns stay in Q and its singular values were off
+    by 3e-4."""
+    N, L, k, p = 3000, 750, 10, 10
+    t = np.arange(N, dtype=np.float64)
+    rng = np.random.default_rng(17)
+    if case == "offset":
+        x = 1000.0 + 0.01 * rng.standard_normal(N)
+        groups, top = ([0], list(range(k))), 1
+    elif case == "rank2":
+        x = np.sin(2 * np.pi * t / 37.0)
+        groups, top = ([0, 1], list(range(k))), 2
+    else:
+        x = 0.05 * t + 0.2 * np.sin(2 * np.pi * t / 20.0) + 0.1 * np.sin(2 * np.pi * t / 7.0) \
+            + 0.01 * rng.standard_normal(N)
+        groups, top = ([0], [0, 1], [2, 3], [4, 5], list(range(6))), 6
+    seed = {"offset": 31, "rank2": 32, "trend": 33}[case]
+    x, dev, orc = _pair(N, L, k, p, 2, seed=seed, x=x, precision=prec)
+    so = orc.sigma
+    t0, eps = (1e-10, 1.1e-16) if prec == 0 else (1e-5, 6e-8)
+    d_sig = float(np.max(np.abs(dev.singular_values[:top] - so[:top]) / so[:top]))
+    print(f"{case} prec={prec}: singular values {d_sig:.2e}")
+    for g in groups:
+        a, b = dev.reconstruct(g), orc.reconstruct(g)
+        sig = [so[i] for i in g if so[i] > 1e-9 * so[0]]   # numerically zero components carry no signal
+        tol = max(t0, 20 * eps * so[0] / min(sig))
+        print(f"{case} prec={prec}: group {g} rel2 {rel2(a, b):.2e} (bound {tol:.1e})")
+        assert rel2(a, b) < tol, g
+    assert d_sig < t0
