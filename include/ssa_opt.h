@@ -286,6 +286,13 @@ int ssa_amd_batch_set_signals_f32(ssa_amd_batch *b, const float *cat, const long
 /* Philox key of series i (default: seed ^ 0x9E3779B97F4A7C15 * (i + 1)); a
  * request keyed by its own id draws the same test matrix in any batch */
 int ssa_amd_batch_set_key(ssa_amd_batch *b, int i, unsigned long long key);
+/* No counterpart in the reference: block-matvec launches of at least n items
+ * (1: every launch) take the wide transform geometry, which rounds
+ * differently from the narrow one (FP32 at n = 8192, FP64 at 1024..4096).  A
+ * server pins n so that a request's result does not depend on the size of
+ * the wave it lands in; the default chooses by launch size.  Returns -1 for
+ * n < 1. */
+int ssa_amd_batch_set_wide_min(ssa_amd_batch *b, int n);
 /* one SSA of every series: spectra, decomposition, reconstruction of all
  * components; use_graph captures the launch sequence in a HIP graph */
 int ssa_amd_batch_run(ssa_amd_batch *b, int use_graph);

@@ -770,6 +770,13 @@ int ssa_engine_status(ssa_engine *e, int s)
     return st;
 }
 
+void ssa_engine_set_wide_min(ssa_engine *e, int n)
+{
+    if (!e) return;
+    e->mv_wide_min = n < 1 ? 1 : n;
+    if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+}
+
 int ssa_engine_set_rng(ssa_engine *e, int s, unsigned long long seed, unsigned long long call)
 {
     if (s < 0 || s >= e->count) return -1;

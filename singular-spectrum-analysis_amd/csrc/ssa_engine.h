@@ -66,6 +66,10 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h);
  * (b <= bmax, k <= kmax of create); seed and call number of its test matrix */
 int ssa_engine_set_bk(ssa_engine *e, int s, int b, int k);
 int ssa_engine_set_rng(ssa_engine *e, int s, unsigned long long seed, unsigned long long call);
+/* block-matvec launches of at least n items take the wide transform geometry
+ * (default MV_WIDE_MIN_ITEMS or SSA_AMD_MV_WIDE_MIN; 1: every launch); drops a
+ * captured step graph */
+void ssa_engine_set_wide_min(ssa_engine *e, int n);
 /* copy a K x b column-major test matrix into the Z panel; the next
  * decomposition of the engine uses injected matrices for every series */
 int ssa_engine_inject_omega(ssa_engine *e, int s, const double *Omega);

@@ -1224,6 +1224,7 @@ struct ssa_amd_batch {
     ssa_engine *eng[BATCH_MAX_BRANCHES];
     ssa_multi *multi;
     int Gmax, G, count, cap, q, p, h, precision;
+    int wide_min;              /* ssa_amd_batch_set_wide_min, 0: the engines' default */
     unsigned long long seed;
     int *gid, *lid;            /* series i -> branch, index within that branch */
 };
@@ -1317,6 +1318,7 @@ static int batch_bind(ssa_amd_batch *b, int count, const int *N, const int *L, c
             rc = -1;
             break;
         }
+        if (b->wide_min > 0) ssa_engine_set_wide_min(b->eng[br], b->wide_min);
         rc = ssa_engine_prepare(b->eng[br], 1, b->h);
         for (int i = 0; rc == 0 && i < count; i++) {
             if (b->gid[i] != br) continue;
@@ -1399,6 +1401,19 @@ int ssa_amd_autotune(int precision, double *ns8)
 {
     if (precision != 0 && precision != 1) return -1;
     return ssa_engine_autotune(precision == 1 ? SSA_PREC_F32 : SSA_PREC_F64, ns8);
+}
+
+int ssa_amd_batch_set_wide_min(ssa_amd_batch *b, int n)
+{
+    if (!b || n < 1) return -1;
+    b->wide_min = n;
+    for (int br = 0; br < b->G; br++) ssa_engine_set_wide_min(b->eng[br], n);
+    if (b->multi) {   /* a captured graph holds the old launches */
+        ssa_multi_destroy(b->multi);
+        b->multi = ssa_multi_create(b->eng, b->G);
+        if (!b->multi) return -1;
+    }
+    return 0;
 }
 
 int ssa_amd_batch_set_key(ssa_amd_batch *b, int i, unsigned long long key)

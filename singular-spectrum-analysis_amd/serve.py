@@ -113,6 +113,9 @@ class Server:
         else:
             b = ssa_amd.Batch(xs, Ls, ks, p=self.p, q=self.q, precision=self.precision, groups=gs, h=self.h,
                               keys=keys)
+            # one transform geometry for every wave size: a request's result
+            # then does not depend on the wave it lands in
+            b.set_wide_min(1)
         b.run(use_graph=False)
         self.waves.append((b, reqs))
         self.wave_sizes.append(len(reqs))

@@ -143,6 +143,7 @@ SIGNATURES = [
     ("ssa_amd_batch_create", _VP, [_I, c_int_p, c_int_p, c_int_p, _I, _I, _I, _U64]),
     ("ssa_amd_batch_set_signals_f32", _I, [_VP, c_float_p, c_ll_p]),
     ("ssa_amd_batch_set_key", _I, [_VP, _I, _U64]),
+    ("ssa_amd_batch_set_wide_min", _I, [_VP, _I]),
     ("ssa_amd_autotune", _I, [_I, c_double_p]),
     ("ssa_amd_batch_rebind", _I, [_VP, _I, c_int_p, c_int_p, c_int_p, c_int_p]),
     ("ssa_amd_batch_outputs", _I, [_VP, c_double_p, c_ll_p, c_int_p]),
@@ -648,6 +649,13 @@ class Batch:
             for i, key in enumerate(keys):
                 if self._l.ssa_amd_batch_set_key(self._h, i, int(key) & ((1 << 64) - 1)) != 0:
                     raise RuntimeError("batch key failed")
+
+    def set_wide_min(self, n):
+        """Block-matvec launches of at least n items take the wide transform
+        geometry (1: every launch), for this batch and every rebind of it."""
+        if self._l.ssa_amd_batch_set_wide_min(self._h, int(n)) != 0:
+            raise RuntimeError("batch geometry setting failed")
+        return self
 
     def rebind(self, xs, Ls, ks, groups=None, keys=None):
         """Re-target this (idle) batch to new series with the same p, q, h,

@@ -39,6 +39,23 @@ def test_alone_vs_a_wave_of_700_bit_identical():
         assert big[r.rid].status == alone.status
 
 
+@pytest.mark.parametrize("prec,count,n_lo,n_hi", [(1, 48, 25000, 50000), (0, 300, 3000, 8000)])
+def test_alone_vs_a_wave_on_the_wide_geometry_bit_identical(prec, count, n_lo, n_hi):
+    """Waves whose block-matvec launches hold thousands of items: FP32 series
+    at n = 8192 chunked on both sides, FP64 series at n = 1024..4096 whose K
+    dimension is split.  With the launch-size rule a batch would run these
+    launches on the wide transform geometry and a request alone on the narrow
+    one, which rounds differently; the server pins the wide geometry, so the
+    results are bit-identical."""
+    S = _serve()
+    reqs = S.synthetic_requests(count, rate=np.inf, n_lo=n_lo, n_hi=n_hi, seed=37)
+    big = {r.rid: r for r in S.Server(max_wave=count, inflight=1, precision=prec).run(reqs)}
+    for r in S.synthetic_requests(count, rate=np.inf, n_lo=n_lo, n_hi=n_hi, seed=37)[::count // 3]:
+        alone = S.Server(max_wave=1, inflight=1, precision=prec).run([r])[0]
+        assert np.array_equal(big[r.rid].out, alone.out), (r.rid, len(r.x))
+        assert big[r.rid].status == alone.status
+
+
 def test_open_loop_stream_completes_with_forecasts():
     S = _serve()
     reqs = S.synthetic_requests(40, rate=400.0, n_lo=500, n_hi=8000, seed=32)
