@@ -172,6 +172,72 @@ def _ref_ssa(args):
     return len(x)
 
 
+def _mkl_ssa(args, q=Q_POWER, p=P_OVERSAMPLE):
+    """One SSA of series i through the reference's MKL call sequence
+    (oracle/mkl_baseline.c): decompose with q power iterations and oversampling
+    p, the series' Philox test matrix, reconstruct all k components."""
+    x, L, k = args
+    import oracle as O
+    x = np.asarray(x, dtype=np.float64)
+    K = len(x) - L + 1
+    b = min(k + p, L, K)
+    s = O.MklSSA(x, L).decompose(k, p, q, O.philox_gaussian(4, 0, K, b))
+    s.reconstruct(list(range(k)))
+    return len(x)
+
+
+def _mkl_faithful(args):
+    """The reference's own shape: q = 0 (no power iterations), p = 8."""
+    return _mkl_ssa(args, q=0, p=8)
+
+
+def _mkl_worker(w, workers, xs, Ls, ks, budget_s, fn, threads):
+    """_oracle_worker over the MKL path with `threads` MKL threads, after one
+    untimed call (MKL's first call sets up its thread pool and tables)."""
+    import oracle as O
+    O.mkl_lib().mklb_set_threads(threads)
+    fn((xs[w % len(xs)], int(Ls[w % len(xs)]), int(ks[w % len(xs)])))
+    return _oracle_worker(w, workers, xs, Ls, ks, budget_s, fn)
+
+
+def cpu_mkl(xs, Ls, ks, budget_s, workers):
+    """The reference's MKL call sequence (decompose_randomized + reconstruct,
+    MKL/ssa_opt.h:910-1004, :1685-1763, :2325-2400), linked at run time to
+    the MKL runtime on the host, timed on the same batch:
+      matched:  q = 2, p = 10 (the device's work), one process per core with
+                sequential MKL (series i to process i mod cores), and one
+                process with MKL threaded over every core, one series at a
+                time (how the reference is used);
+      faithful: q = 0, p = 8 (the reference's own shape), one process per core.
+    Every leg runs in forked children, so this process never loads MKL.
+    None when the MKL runtime is absent."""
+    import multiprocessing as mp
+    import oracle as O
+    if not (os.path.exists(O.MKL_SO) and os.path.exists(O.MKL_RT)):
+        return None
+    ctx = mp.get_context("fork")
+    with ctx.Pool(1) as pool:
+        version = pool.apply(O.mkl_version)
+        d1, n1, t1 = pool.apply(_mkl_worker, (0, 1, xs, Ls, ks, budget_s, _mkl_ssa, 1))
+        dt, nt, tt = pool.apply(_mkl_worker, (0, 1, xs, Ls, ks, budget_s, _mkl_ssa, workers))
+    out = {}
+    for name, fn in (("matched", _mkl_ssa), ("faithful", _mkl_faithful)):
+        with ctx.Pool(workers) as pool:
+            res = pool.starmap(_mkl_worker, [(w, workers, xs, Ls, ks, budget_s, fn, 1) for w in range(workers)])
+        out[name] = (sum(r[0] for r in res), sum(r[1] for r in res), max(r[2] for r in res))
+    dm, nm, em = out["matched"]
+    df, nf, ef = out["faithful"]
+    return {"value": round(dm / em, 2), "unit": "SSA/s", "cores": workers, "kind": "port",
+            "sample": f"the reference's MKL call sequence (oracle/mkl_baseline.c, {version}), FP64: {dm} series of "
+                      f"the same batch (sum N = {nm}) over {workers} processes with sequential MKL (series i to "
+                      f"process i mod {workers}), decompose q=2 p=10 + reconstruct all k, {em:.1f} s",
+            "value_1_core": round(d1 / t1, 2),
+            "value_threaded": round(dt / tt, 2),
+            "threaded_sample": f"one process, MKL on {workers} threads, one series at a time: {dt} series in {tt:.1f} s",
+            "faithful": {"value": round(df / ef, 2),
+                         "sample": f"q=0 p=8 (the reference's shape), {df} series over {workers} processes, {ef:.1f} s"}}
+
+
 def _oracle_worker(w, workers, xs, Ls, ks, budget_s, fn=None):
     fn = fn or _oracle_ssa
     done = nsum = 0
@@ -527,6 +593,11 @@ def main():
     workers = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))))
     cpu = None if args.no_cpu else cpu_baseline(xs, Ls, ks, args.cpu_seconds, workers)
     cpu_ref = None if args.no_cpu else cpu_reference(xs, Ls, ks, args.cpu_seconds, workers)
+    cpu_mkl_out = None if args.no_cpu else cpu_mkl(xs, Ls, ks, args.cpu_seconds, workers)
+    if cpu_mkl_out:
+        print(f"[bench] cpu MKL (reference call sequence): {cpu_mkl_out['value']} SSA/s on {workers} cores "
+              f"(q=2), {cpu_mkl_out['value_1_core']} on one, {cpu_mkl_out['value_threaded']} MKL-threaded; "
+              f"q=0 p=8 {cpu_mkl_out['faithful']['value']} SSA/s", flush=True)
     if cpu:
         print(f"[bench] cpu baseline: {cpu['value']} SSA/s on {workers} cores, {cpu['value_1_core']} on one", flush=True)
     if cpu_ref:
@@ -613,7 +684,10 @@ def main():
     if serving_out is not None:
         serving_out["cpu_fp64"] = cpu_serve
     out["single_series"] = configs_out
-    out["cpu_baseline"] = cpu
+    # the MKL-linked reference call sequence where the MKL runtime is on the
+    # host, the FP64 oracle port otherwise; both are reported
+    out["cpu_baseline"] = cpu_mkl_out or cpu
+    out["cpu_oracle"] = cpu
     out["cpu_reference"] = cpu_ref
     print(json.dumps(out), flush=True)
 

@@ -404,3 +404,82 @@ class RefSSA:
         out = np.zeros(self.N + h)
         rc = self._l.ssa_opt_ref_forecast(ctypes.byref(self.s), _i(g), len(g), h, _d(out))
         return rc, out
+
+
+MKL_SO = os.path.join(_HERE, "libmkl_baseline.so")
+MKL_RT = os.environ.get("SSA_MKL_RT", "/opt/conda/lib/libmkl_rt.so.1")
+_mkl = None
+
+
+def mkl_lib():
+    """libmkl_baseline.so with MKL's runtime loaded, or None where either is
+    absent (then the baseline falls back to the oracle port)."""
+    global _mkl
+    if _mkl is not None:
+        return _mkl or None
+    _mkl = False
+    if not (os.path.exists(MKL_SO) and os.path.exists(MKL_RT)):
+        return None
+    lib = ctypes.CDLL(MKL_SO)
+    vp = ctypes.c_void_p
+    lib.mklb_load.argtypes = [ctypes.c_char_p]
+    lib.mklb_set_threads.argtypes = [ctypes.c_int]
+    lib.mklb_version.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    lib.mklb_create.restype = vp
+    lib.mklb_create.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+    lib.mklb_free.argtypes = [vp]
+    lib.mklb_decompose.argtypes = [vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+    lib.mklb_reconstruct.argtypes = [vp, _dp, _dp, _dp, _ip, ctypes.c_int, _dp]
+    if lib.mklb_load(MKL_RT.encode()) != 0:
+        return None
+    _mkl = lib
+    return lib
+
+
+def mkl_version():
+    lib = mkl_lib()
+    if lib is None:
+        return None
+    buf = ctypes.create_string_buffer(256)
+    lib.mklb_version(buf, 256)
+    return buf.value.decode(errors="replace").strip()
+
+
+class MklSSA:
+    """One series through the reference's MKL call sequence (mkl_baseline.c):
+    decompose(k, p, q, omega) with the caller's K x (k+p) test matrix, then
+    reconstructions.  The CPU baseline, checked against the oracle."""
+
+    def __init__(self, x, L):
+        self._l = mkl_lib()
+        if self._l is None:
+            raise RuntimeError("MKL runtime not available")
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.N, self.L = len(self.x), L
+        self.K = self.N - L + 1
+        self._h = self._l.mklb_create(_d(self.x), self.N, L)
+        if not self._h:
+            raise RuntimeError("MKL context failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._l.mklb_free(self._h)
+            self._h = None
+
+    def decompose(self, k, p, q, omega):
+        b = omega.shape[1]
+        om = np.asfortranarray(omega, dtype=np.float64)
+        self.k = k
+        self.sigma = np.zeros(k)
+        self.U = np.zeros((self.L, k), order="F")
+        self.V = np.zeros((self.K, k), order="F")
+        if self._l.mklb_decompose(self._h, _d(om), k, b, q, _d(self.sigma), _d(self.U), _d(self.V)) != 0:
+            raise RuntimeError("MKL decomposition failed")
+        return self
+
+    def reconstruct(self, group):
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        out = np.zeros(self.N)
+        if self._l.mklb_reconstruct(self._h, _d(self.sigma), _d(self.U), _d(self.V), _i(g), len(g), _d(out)) != 0:
+            raise RuntimeError("MKL reconstruction failed")
+        return out
