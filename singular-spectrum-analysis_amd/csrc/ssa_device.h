@@ -20,6 +20,8 @@ extern "C" {
 
 #define SSA_BMAX 80           /* max block width k + p */
 #define SSA_GRAM_ROWS 2048    /* panel rows per Gram / apply work item */
+/* element multiple of the Y and Z panels' leading dimensions (128 bytes of FP32) */
+#define SSA_PANEL_ALIGN 32
 #define SSA_EDGE 64           /* samples at each end of a reconstruction summed directly (power of two) */
 
 enum { SSA_PREC_F64 = 0, SSA_PREC_F32 = 1 };
@@ -32,13 +34,15 @@ typedef struct ssa_desc {
     int n, lgn;               /* correlation length (power of two) */
     int CL, CK, AL, SK;       /* chunk sizes and counts (AL per channel) */
     int tilesY, tilesZ;       /* Gram row tiles of the ML x b and K x b panels */
+    int ldY, ldZ;             /* leading dimensions of the Y and Z panels: ML and K rounded up to
+                               * SSA_PANEL_ALIGN elements, so every column starts on a 128-byte line */
     int status;               /* set non-zero by device code on failure */
     int gsize;                /* batched step: reconstruct / forecast the group {0..gsize-1} */
     /* element offsets into the bulk arena (T = double or float) */
     long long off_x;          /* M*N signal values */
     long long off_spec;       /* M*AL*SK spectra, n complex each (2n T) */
-    long long off_Y;          /* ML x b, ld ML */
-    long long off_Z;          /* K x b, ld K */
+    long long off_Y;          /* ML x b, ld ldY */
+    long long off_Z;          /* K x b, ld ldZ */
     long long off_U;          /* ML x k, ld ML */
     long long off_V;          /* K x k, ld K */
     long long off_stage;      /* reconstruction staging */

@@ -179,6 +179,10 @@ int ssa_plan_series(int prec, int N, int L, int M, int b, int k, int q, int lgn_
 
 static long long align64(long long v) { return (v + 63) & ~63LL; }
 
+/* Y and Z panel columns start on 128-byte lines: the panel kernels' row
+ * segments then never straddle two lines */
+static int panel_ld(int rows) { return (rows + SSA_PANEL_ALIGN - 1) / SSA_PANEL_ALIGN * SSA_PANEL_ALIGN; }
+
 
 static int pin_reserve(ssa_engine *e, size_t bytes)
 {
@@ -397,6 +401,8 @@ static int engine_layout(ssa_engine *e, int count, const int *N, const int *L, c
         d->n = pl.n; d->lgn = pl.lgn; d->CL = pl.CL; d->CK = pl.CK; d->AL = pl.AL; d->SK = pl.SK;
         d->tilesY = ceil_div(d->ML, SSA_GRAM_ROWS);
         d->tilesZ = ceil_div(d->K, SSA_GRAM_ROWS);
+        d->ldY = panel_ld(d->ML);
+        d->ldZ = panel_ld(d->K);
         d->seed = 42;
         /* every series' signal one after the other, so that one copy uploads them all */
         d->off_x = off; off = align64(off + (long long)m * d->N);
@@ -599,7 +605,8 @@ long long ssa_engine_series_bytes(int prec, int N, int L, int b, int k, int max_
     const long long K = N - L + 1, ML = L;
     const int gmax = max_groups > k ? max_groups : k;
     long long a = align64(N) + align64((long long)pl.AL * pl.SK * 2 * pl.n);
-    long long bulk = align64(ML * b) + align64(K * b) + align64(ML * k) + align64(K * k);
+    long long bulk = align64((long long)panel_ld((int)ML) * b) + align64((long long)panel_ld((int)K) * b) +
+                     align64(ML * k) + align64(K * k);
     bulk += align64((long long)gmax * pl.AL * pl.SK * pl.n);
     bulk += align64((long long)gmax * N + max_h + 64);
     bulk += align64((long long)N + max_h);   /* gathered step output */
@@ -626,8 +633,8 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
         int bm = e->bmax[s], km = e->kmax[s];
         if (bm < 2 || km < 1) return -1;
         int gmax = max_groups > km ? max_groups : km;   /* W-correlation needs k pieces */
-        d->off_Y = off; off = align64(off + (long long)d->ML * bm);
-        d->off_Z = off; off = align64(off + (long long)d->K * bm);
+        d->off_Y = off; off = align64(off + (long long)d->ldY * bm);
+        d->off_Z = off; off = align64(off + (long long)d->ldZ * bm);
         d->off_U = off; off = align64(off + (long long)d->ML * km);
         d->off_V = off; off = align64(off + (long long)d->K * km);
         d->stage_cap = (long long)gmax * d->AL * d->SK * d->n;
@@ -794,7 +801,9 @@ int ssa_engine_inject_omega(ssa_engine *e, int s, const double *Omega)
     DEVCHK(pin_reserve(e, n * e->esz));
     if (e->prec == SSA_PREC_F64) memcpy(e->hpin, Omega, n * 8);
     else for (size_t i = 0; i < n; i++) ((float *)e->hpin)[i] = (float)Omega[i];
-    HIPCHK(hipMemcpyAsync((char *)e->arena_a + e->esz * d->off_Z, e->hpin, n * e->esz, hipMemcpyHostToDevice, e->st));
+    /* K x b, column-major, into the Z panel (leading dimension ldZ) */
+    HIPCHK(hipMemcpy2DAsync((char *)e->arena_a + e->esz * d->off_Z, (size_t)d->ldZ * e->esz, e->hpin,
+                            (size_t)d->K * e->esz, (size_t)d->K * e->esz, (size_t)d->b, hipMemcpyHostToDevice, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
     return 0;
 }

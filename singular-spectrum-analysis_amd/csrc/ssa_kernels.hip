@@ -51,8 +51,8 @@ template <typename T> struct Panel {
 template <typename T> __device__ __forceinline__ Panel<T> panel_of(T *bulk, const ssa_desc &d, int which)
 {
     switch (which) {
-    case SSA_PANEL_Y: return {bulk + d.off_Y, d.ML, d.ML, d.b};
-    case SSA_PANEL_Z: return {bulk + d.off_Z, d.K, d.K, d.b};
+    case SSA_PANEL_Y: return {bulk + d.off_Y, d.ML, d.ldY, d.b};
+    case SSA_PANEL_Z: return {bulk + d.off_Z, d.K, d.ldZ, d.b};
     case SSA_PANEL_U: return {bulk + d.off_U, d.ML, d.ML, d.k};
     case SSA_PANEL_V: return {bulk + d.off_V, d.K, d.K, d.k};
     default: return {bulk + d.off_out, d.N, d.N, d.k};   // weighted component panel (W-corr)
