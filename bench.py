@@ -238,6 +238,77 @@ def cpu_mkl(xs, Ls, ks, budget_s, workers):
                          "sample": f"q=0 p=8 (the reference's shape), {df} series over {workers} processes, {ef:.1f} s"}}
 
 
+def _mkl_configs_child(threads):
+    """Per-config latencies of the MKL call sequence (run in a forked child):
+    cfg1 (N = 1000, L = 250, k = 50: decompose q=2 p=10 + reconstructions [0]
+    and [1, 2] + W-correlation), cfg2 (N = 20000, L = 5000, k = 50: decompose
+    + the 50 single-component reconstructions + W-correlation) and the
+    reference's published shape (N = 500, L = 375, k = 15, p = 10, q = 0:
+    decompose, and decompose + reconstruction of all 15).  One context per
+    shape; the test matrix is drawn once, outside the timed region.  p50/p99
+    in ms after 3 untimed calls."""
+    import oracle as O
+    O.mkl_lib().mklb_set_threads(threads)
+    out = {}
+
+    def timed(fn, reps):
+        for _ in range(3):
+            fn()
+        t = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            fn()
+            t.append(time.perf_counter() - a)
+        return round(1e3 * float(np.percentile(t, 50)), 3), round(1e3 * float(np.percentile(t, 99)), 3)
+
+    def shape(N, L, k, p, q, seed, om_seed):
+        x = workloads.single(N, seed=seed)
+        K = N - L + 1
+        return O.MklSSA(x, L), O.philox_gaussian(om_seed, 0, K, min(k + p, L, K))
+
+    m, om = shape(1000, 250, 50, P_OVERSAMPLE, Q_POWER, 1, 11)
+
+    def cfg1():
+        m.decompose(50, P_OVERSAMPLE, Q_POWER, om)
+        m.reconstruct([0])
+        m.reconstruct([1, 2])
+        m.wcorr()
+    out["cfg1_p50_ms"], out["cfg1_p99_ms"] = timed(cfg1, 20)
+    m2, om2 = shape(20000, 5000, 50, P_OVERSAMPLE, Q_POWER, 2, 12)
+
+    def cfg2():
+        m2.decompose(50, P_OVERSAMPLE, Q_POWER, om2)
+        for i in range(50):
+            m2.reconstruct([i])
+        m2.wcorr()
+    out["cfg2_p50_ms"], out["cfg2_p99_ms"] = timed(cfg2, 10 if threads > 1 else 3)
+    m5, om5 = shape(500, 375, 15, P_OVERSAMPLE, 0, 15, 15)
+    out["ref_shape_N500_k15_q0_decompose_p50_ms"], _ = timed(lambda: m5.decompose(15, P_OVERSAMPLE, 0, om5), 200)
+
+    def pipe():
+        m5.decompose(15, P_OVERSAMPLE, 0, om5)
+        m5.reconstruct(list(range(15)))
+    out["ref_shape_N500_k15_q0_pipeline_p50_ms"], _ = timed(pipe, 200)
+    return out
+
+
+def cpu_mkl_configs(workers):
+    """_mkl_configs_child with MKL threaded over every core (how the reference
+    is used) and on one core, each in a forked child; None without MKL."""
+    import multiprocessing as mp
+    import oracle as O
+    if not (os.path.exists(O.MKL_SO) and os.path.exists(O.MKL_RT)):
+        return None
+    ctx = mp.get_context("fork")
+    with ctx.Pool(1) as pool:
+        allc = pool.apply(_mkl_configs_child, (workers,))
+    with ctx.Pool(1) as pool:
+        one = pool.apply(_mkl_configs_child, (1,))
+    return {"threads": workers, "all_cores": allc, "one_core": one,
+            "sample": "the reference's MKL call sequence (oracle/mkl_baseline.c), FP64, one prepared context "
+                      "per shape, test matrix drawn outside the timed region"}
+
+
 def _oracle_worker(w, workers, xs, Ls, ks, budget_s, fn=None):
     fn = fn or _oracle_ssa
     done = nsum = 0
@@ -594,6 +665,10 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(xs, Ls, ks, args.cpu_seconds, workers)
     cpu_ref = None if args.no_cpu else cpu_reference(xs, Ls, ks, args.cpu_seconds, workers)
     cpu_mkl_out = None if args.no_cpu else cpu_mkl(xs, Ls, ks, args.cpu_seconds, workers)
+    cpu_cfg = None if (args.no_cpu or args.no_configs) else cpu_mkl_configs(workers)
+    if cpu_cfg:
+        print(f"[bench] cpu MKL per config ({workers} threads): {cpu_cfg['all_cores']}; one core: "
+              f"{cpu_cfg['one_core']}", flush=True)
     if cpu_mkl_out:
         print(f"[bench] cpu MKL (reference call sequence): {cpu_mkl_out['value']} SSA/s on {workers} cores "
               f"(q=2), {cpu_mkl_out['value_1_core']} on one, {cpu_mkl_out['value_threaded']} MKL-threaded; "
@@ -688,6 +763,7 @@ def main():
     # host, the FP64 oracle port otherwise; both are reported
     out["cpu_baseline"] = cpu_mkl_out or cpu
     out["cpu_oracle"] = cpu
+    out["cpu_configs"] = cpu_cfg
     out["cpu_reference"] = cpu_ref
     print(json.dumps(out), flush=True)
 

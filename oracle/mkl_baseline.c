@@ -18,6 +18,9 @@
  *   reconstruct     per component: transforms of sigma u and v, their product
  *                   accumulated, one inverse transform, times the diagonal
  *                   averaging weights (MKL/ssa_opt.h:2325-2400)
+ *   wcorr           per component h = C2R(FFT(u) FFT(v)), row h[t] / sqrt(c_t)
+ *                   normalised, W = G G^T by dsyrk, mirrored
+ *                   (MKL/ssa_opt.h:2407-2490)
  * The test matrix comes from the caller (the same Philox matrix the oracle
  * and the device use), so the outputs can be checked against the oracle.
  */
@@ -41,7 +44,14 @@ enum {
     DFTI_COMPLEX_COMPLEX_ = 39,
     DFTI_NOT_INPLACE_ = 44,
 };
-enum { LAPACK_COL_MAJOR_ = 102, CBLAS_COL_MAJOR_ = 102, CBLAS_NO_TRANS_ = 111, CBLAS_TRANS_ = 112 };
+enum {
+    LAPACK_COL_MAJOR_ = 102,
+    CBLAS_ROW_MAJOR_ = 101,
+    CBLAS_COL_MAJOR_ = 102,
+    CBLAS_NO_TRANS_ = 111,
+    CBLAS_TRANS_ = 112,
+    CBLAS_UPPER_ = 121,
+};
 
 typedef void *dfti_handle;
 static long (*p_create)(dfti_handle *, int, int, long, ...);
@@ -56,6 +66,7 @@ static int (*p_orgqr)(int, int, int, int, double *, int, const double *);
 static int (*p_gesdd)(int, char, int, int, double *, int, double *, double *, int, double *, int);
 static void (*p_dgemm)(int, int, int, int, int, int, double, const double *, int, const double *, int, double,
                        double *, int);
+static void (*p_dsyrk)(int, int, int, int, int, double, const double *, int, double, double *, int);
 static void (*p_set_threads)(int);
 static void (*p_version)(char *, int);
 static void *g_mkl;
@@ -91,6 +102,7 @@ int mklb_load(const char *path)
     SYM(p_orgqr, "LAPACKE_dorgqr");
     SYM(p_gesdd, "LAPACKE_dgesdd");
     SYM(p_dgemm, "cblas_dgemm");
+    SYM(p_dsyrk, "cblas_dsyrk");
     SYM(p_set_threads, "MKL_Set_Num_Threads");
     SYM(p_version, "MKL_Get_Version_String");
     return 0;
@@ -301,5 +313,43 @@ int mklb_reconstruct(mklb_ctx *c, const double *sigma, const double *U, const do
     p_bwd(c->b1, acc, c->wr);
     for (int t = 0; t < N; t++) out[t] = c->wr[t] * c->w[t];
     free(acc);
+    return 0;
+}
+
+/* W (k x k, row-major) = the W-correlation matrix of the k components: row i
+ * of G is h_i[t] / sqrt(c_t) over its weighted norm, h_i = u_i (*) v_i (sigma
+ * cancels), W = G G^T */
+int mklb_wcorr(mklb_ctx *c, const double *U, const double *V, int k, double *W)
+{
+    const int n = c->n, r = c->r2c, L = c->L, K = c->K, N = c->N;
+    if (k < 1) return -1;
+    double *G = (double *)malloc(sizeof(double) * (size_t)k * N);
+    if (!G) return -1;
+    double *a = c->wr, *bv = c->wr + n, *fa = c->wc, *fb = c->wc + 2 * (size_t)r;
+    for (int i = 0; i < k; i++) {
+        memcpy(a, U + (size_t)i * L, sizeof(double) * L);
+        memset(a + L, 0, sizeof(double) * (n - L));
+        memcpy(bv, V + (size_t)i * K, sizeof(double) * K);
+        memset(bv + K, 0, sizeof(double) * (n - K));
+        p_fwd(c->f1, a, fa);
+        p_fwd(c->f1, bv, fb);
+        for (int j = 0; j < r; j++) {
+            const double ar = fa[2 * j], ai = fa[2 * j + 1], br = fb[2 * j], bi = fb[2 * j + 1];
+            fa[2 * j] = ar * br - ai * bi;
+            fa[2 * j + 1] = ar * bi + ai * br;
+        }
+        p_bwd(c->b1, fa, a);
+        double *g = G + (size_t)i * N, nn = 0.0;
+        for (int t = 0; t < N; t++) {
+            g[t] = a[t] * sqrt(c->w[t]);
+            nn += g[t] * g[t];
+        }
+        const double inv = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+        for (int t = 0; t < N; t++) g[t] *= inv;
+    }
+    p_dsyrk(CBLAS_ROW_MAJOR_, CBLAS_UPPER_, CBLAS_NO_TRANS_, k, N, 1.0, G, N, 0.0, W, k);
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < i; j++) W[(size_t)i * k + j] = W[(size_t)j * k + i];
+    free(G);
     return 0;
 }

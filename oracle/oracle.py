@@ -430,6 +430,7 @@ def mkl_lib():
     lib.mklb_free.argtypes = [vp]
     lib.mklb_decompose.argtypes = [vp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
     lib.mklb_reconstruct.argtypes = [vp, _dp, _dp, _dp, _ip, ctypes.c_int, _dp]
+    lib.mklb_wcorr.argtypes = [vp, _dp, _dp, ctypes.c_int, _dp]
     if lib.mklb_load(MKL_RT.encode()) != 0:
         return None
     _mkl = lib
@@ -483,3 +484,9 @@ class MklSSA:
         if self._l.mklb_reconstruct(self._h, _d(self.sigma), _d(self.U), _d(self.V), _i(g), len(g), _d(out)) != 0:
             raise RuntimeError("MKL reconstruction failed")
         return out
+
+    def wcorr(self):
+        W = np.zeros((self.k, self.k))
+        if self._l.mklb_wcorr(self._h, _d(self.U), _d(self.V), self.k, _d(W)) != 0:
+            raise RuntimeError("MKL W-correlation failed")
+        return W

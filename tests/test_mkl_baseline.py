@@ -46,6 +46,7 @@ for N, L, k, p, q in {cases!r}:
     out[f"u_{{N}}_{{q}}"] = m.U
     for j, g in enumerate(([0], [1, 2], list(range(k)))):
         out[f"r_{{N}}_{{q}}_{{j}}"] = m.reconstruct(g)
+    out[f"w_{{N}}_{{q}}"] = m.wcorr()
 np.savez({path!r}, **out)
 """
 
@@ -74,5 +75,6 @@ def test_mkl_baseline_matches_oracle(mkl_results, N, L, k, p, q):
     r = rms(x)
     for j, g in enumerate(GROUPS(k)):
         assert np.abs(mkl_results[f"r_{N}_{q}_{j}"] - o.reconstruct(g)).max() / r < 1e-11, g
+    assert np.abs(mkl_results[f"w_{N}_{q}"] - o.wcorr()).max() < 1e-10
     # the sign rule: every column of U sums to a non-negative value
     assert np.all(mkl_results[f"u_{N}_{q}"].sum(axis=0) >= 0)
