@@ -273,11 +273,15 @@ __global__ __launch_bounds__((FftGeom<T, LOGN, W>::NT), (FftGeom<T, LOGN, W>::OC
             // spectra are indexed [L chunk][K chunk]
             const int lo = dir == 0 ? o : p, ko = dir == 0 ? p : o;
             const C2<T> *Xo = X + ((long long)lo * d.SK + ko) * G::N;
+            // all spectrum loads first, then the products: interleaved, the
+            // scheduler reused one register pair and waited on every load
+            C2<T> xv[E];
+#pragma unroll
+            for (int e = 0; e < E; e++) xv[e] = Xo[threadIdx.x + e * NT];
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                const C2<T> xv = Xo[threadIdx.x + e * NT];
-                acc[e].x += xv.x * z[e].x - xv.y * z[e].y;
-                acc[e].y += xv.x * z[e].y + xv.y * z[e].x;
+                acc[e].x += xv[e].x * z[e].x - xv[e].y * z[e].y;
+                acc[e].y += xv[e].x * z[e].y + xv[e].y * z[e].x;
             }
         }
         fft<T, LOGN, 1, SPLIT, W>(acc, lre, lim, tw);
@@ -309,6 +313,7 @@ __global__ __launch_bounds__((FftGeom<T, LOGN, W>::NT), (FftGeom<T, LOGN, W>::OC
 // accumulator values of lane l are D[l/16 + 4v][l%16], v = 0..3.
 // ---------------------------------------------------------------------------
 typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ d4v mfma64(double a, double b, d4v c)
 {
@@ -425,11 +430,19 @@ template <int NB> constexpr int n_upper_blocks() { return NB * (NB + 1) / 2; }
 // column block I from the owner of block (I, I)) for launches of few items;
 // each block is summed in the same order either way, so the partial is the
 // same.
+// ACC32 (FP32 panels, USE_M = 0 only): FP32 MFMA (v_mfma_f32_16x16x4f32) with
+// FP32 accumulation inside the tile; the 4 waves' sums and the tiles are then
+// added in FP64.  Only the W-correlation of the mixed form takes it: its
+// entries are normalised inner products, bounded by 1, so the FP32 sum's
+// relative error (about eps32 * sqrt(rows)) is far below what that output
+// needs, unlike the orthonormalisation Grams, whose smallest eigenvalues the
+// Cholesky needs to eps * kappa^2.
 // ---------------------------------------------------------------------------
-template <typename T, int NB, int USE_M, bool SPLIT>
+template <typename T, int NB, int USE_M, bool SPLIT, bool ACC32 = false>
 __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items,
                                               const T *__restrict__ bulk, double *small, int which)
 {
+    static_assert(!ACC32 || (sizeof(T) == 4 && !USE_M), "FP32 accumulation is for FP32 panels without M");
     extern __shared__ double sm[];
     constexpr int NBLK = n_upper_blocks<NB>(), MP = small_ld<NB>();
     const ssa_item it = items[blockIdx.x];
@@ -439,9 +452,12 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63, cl = l & 15, kq = l >> 4;
     const int r_begin = it.x * SSA_GRAM_ROWS;
     const int r_end = min(P.rows, r_begin + SSA_GRAM_ROWS);
-    d4v acc[NBLK];
+    d4v acc[ACC32 ? 1 : NBLK];
+    f4v accf[ACC32 ? NBLK : 1];
 #pragma unroll
-    for (int m = 0; m < NBLK; m++) acc[m] = d4v{0, 0, 0, 0};
+    for (int m = 0; m < (ACC32 ? 1 : NBLK); m++) acc[m] = d4v{0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < (ACC32 ? NBLK : 1); m++) accf[m] = f4v{0, 0, 0, 0};
     double cs[NB];
 #pragma unroll
     for (int I = 0; I < NB; I++) cs[I] = 0.0;
@@ -485,8 +501,13 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
                 for (int I = 0; I < NB; I++)
 #pragma unroll
                     for (int J = I; J < NB; J++, m++)
-                        if (!SPLIT || m % (int)gridDim.y == (int)blockIdx.y)
-                            acc[m] = mfma64((double)cur[I][v], (double)cur[J][v], acc[m]);
+                        if (!SPLIT || m % (int)gridDim.y == (int)blockIdx.y) {
+                            if constexpr (ACC32)
+                                accf[m] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)cur[I][v], (float)cur[J][v],
+                                                                               accf[m], 0, 0, 0);
+                            else
+                                acc[m] = mfma64((double)cur[I][v], (double)cur[J][v], acc[m]);
+                        }
             }
 #pragma unroll
             for (int I = 0; I < NB; I++)
@@ -551,7 +572,10 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     double *p = red + (m * 4 + v) * 64 + l;
-                    *p = (ww == 0 ? 0.0 : *p) + acc[m][v];
+                    double a;
+                    if constexpr (ACC32) a = (double)accf[m][v];
+                    else a = acc[m][v];
+                    *p = (ww == 0 ? 0.0 : *p) + a;
                 }
             if (kq == 0) {
 #pragma unroll
@@ -570,7 +594,9 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
         int I = 0, mm = m;
         while (mm >= NB - I) { mm -= NB - I; I++; }
         const int J = I + mm;
-        const int i = I * 16 + (ll >> 4) + 4 * v, j = J * 16 + (ll & 15);
+        // accumulator value v of lane ll: D[ll/16 + 4v][ll%16] (FP64 MFMA),
+        // D[4 (ll/16) + v][ll%16] (FP32 MFMA)
+        const int i = I * 16 + (ACC32 ? 4 * (ll >> 4) + v : (ll >> 4) + 4 * v), j = J * 16 + (ll & 15);
         if (i < b && j < b) {
             const double g = red[idx];
             out[i * b + j] = g;
@@ -666,11 +692,16 @@ __device__ __forceinline__ void gram_sum_regs(const double *part, int tiles, int
         if (16 * h >= b) break;
         for (int t = 0; t < tiles; t++) {
             const double *pt = part + (long long)t * stride;
+            // the 16 loads first, then the sums (interleaved, the scheduler
+            // reused one register pair and waited on every load)
+            double v[16];
 #pragma unroll
-            for (int i = 16 * h; i < 16 * h + 16; i++) {
-                const double v = pt[(i < b ? i : b - 1) * b + lc];
-                a[i] += (i < b) ? v : 0.0;
+            for (int i = 0; i < 16; i++) {
+                const int ii = 16 * h + i;
+                v[i] = pt[(ii < b ? ii : b - 1) * b + lc];
             }
+#pragma unroll
+            for (int i = 16 * h; i < 16 * h + 16; i++) a[i] += (i < b) ? v[i - 16 * h] : 0.0;
             if (h == 0) {
                 dg += pt[lc * b + lc];
                 cy += pt[b * b + lc];
@@ -1021,8 +1052,6 @@ __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply(const ssa_desc *D, c
 // A[l%16][l/16] and B[l/16][l%16]; accumulator value v of lane l is
 // D[4*(l/16) + v][l%16].
 // ---------------------------------------------------------------------------
-typedef float f4v __attribute__((ext_vector_type(4)));
-
 template <int NB>
 __global__ __launch_bounds__(64 * APPLY_WAVES) void k_apply_f32(const ssa_desc *D, const ssa_item *items,
                                                                 float *bulk, const double *small, int src,
@@ -1331,11 +1360,20 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) vo
         const T sc = (T)(scale_sigma ? sqrt(fabs(sig[idx])) : 1.0);
         const T *u = bulk + d.off_U + (long long)idx * d.ML + urow0;
         const T *v = bulk + d.off_V + (long long)idx * d.K + vrow0;
+        // every load unconditional (positions past the chunk read its last
+        // element, ulen, vlen >= 1) and masked afterwards: a load under a
+        // branch whose result is used inside it waits for that load before
+        // the next one issues, which serialised the 2E loads of a component.
+        // The opaque thread index keeps the clamped offsets from being
+        // hoisted out of the component loop (they would be spilled).
+        int tq = threadIdx.x;
+        asm volatile("" : "+v"(tq));
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const int q = threadIdx.x + e * NT;
-            z[e].x = (q < ulen) ? sc * u[q] : (T)0;
-            z[e].y = (q < vlen) ? sc * v[q] : (T)0;
+            const int q = tq + e * NT;
+            const T uu = u[min(q, ulen - 1)], vv = v[min(q, vlen - 1)];
+            z[e].x = (q < ulen) ? sc * uu : (T)0;
+            z[e].y = (q < vlen) ? sc * vv : (T)0;
         }
         fft<T, LOGN, -1, SPLIT>(z, lre, lim, tw);
 #pragma unroll
@@ -1864,7 +1902,7 @@ int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_item
 extern "C++" {
 template <typename T, int NB>
 static void launch_gram(int panel, int use_m, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk,
-                        double *small, hipStream_t st)
+                        double *small, hipStream_t st, bool acc32 = false)
 {
     constexpr int NBLK = n_upper_blocks<NB>(), B16 = NB * 16, MP = small_ld<NB>();
     const size_t red = (size_t)(NBLK * 256 + NB * 16) * sizeof(double);
@@ -1873,6 +1911,17 @@ static void launch_gram(int panel, int use_m, const ssa_desc *D, const ssa_item 
     // over up to NBLK workgroups
     const int py = n_items >= 512 || NBLK == 1 ? 1 : (1024 / n_items < NBLK ? 1024 / n_items : NBLK);
     const dim3 grid(n_items, py);
+    if constexpr (sizeof(T) == 4) {
+        if (acc32 && !use_m) {
+            if (py > 1)
+                hipLaunchKernelGGL((k_gram<T, NB, 0, true, true>), grid, dim3(256), red, st, D, items,
+                                   (const T *)bulk, small, panel);
+            else
+                hipLaunchKernelGGL((k_gram<T, NB, 0, false, true>), grid, dim3(256), red, st, D, items,
+                                   (const T *)bulk, small, panel);
+            return;
+        }
+    }
     if (py > 1) {
         if (use_m)
             hipLaunchKernelGGL((k_gram<T, NB, 1, true>), grid, dim3(256), red > ms ? red : ms, st, D, items,
@@ -2121,7 +2170,21 @@ int ssa_dev_gather(int prec, int h, const ssa_desc *D, const ssa_item *items, in
 int ssa_dev_wcorr(int prec, int nb, const ssa_desc *D, const ssa_item *tile_items, int n_tiles,
                   const ssa_item *series_items, int n_series, void *bulk, double *small, void *stream)
 {
-    int e = ssa_dev_gram(prec, SSA_PANEL_OUT, 0, nb, D, tile_items, n_tiles, bulk, small, stream);
+    if (n_tiles <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    // the weighted Gram: FP64 MFMA for FP64 panels, FP32 MFMA with FP32
+    // accumulation inside each row tile for the mixed form's FP32 panels
+#define G(NB)                                                                                              \
+    case NB:                                                                                               \
+        if (prec == SSA_PREC_F64) launch_gram<double, NB>(SSA_PANEL_OUT, 0, D, tile_items, n_tiles, bulk, small, st); \
+        else launch_gram<float, NB>(SSA_PANEL_OUT, 0, D, tile_items, n_tiles, bulk, small, st, true);    \
+        break;
+    switch (nb) {
+        G(1) G(2) G(3) G(4) G(5)
+    default: return -1;
+    }
+#undef G
+    int e = (int)hipGetLastError();
     if (e) return e;
     if (n_series <= 0) return 0;
     hipLaunchKernelGGL(k_wnorm, dim3(n_series), dim3(256), 0, (hipStream_t)stream, D, series_items, small);

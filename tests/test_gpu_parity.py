@@ -285,6 +285,40 @@ def test_mixed_precision_error():
         assert np.abs(a - b).max() / r < 1e-4, g
 
 
+def _wcorr_np(rec, L):
+    """W-correlation by definition (MKL/ssa_opt.h:2407-2477) from the rows of
+    rec, in FP64: weights w_t = min(t + 1, L, K, N - t)."""
+    N = rec.shape[1]
+    K = N - L + 1
+    t = np.arange(N)
+    w = np.minimum(np.minimum(t + 1, N - t), min(L, K)).astype(np.float64)
+    G = (rec * w) @ rec.T
+    d = np.sqrt(np.diag(G))
+    return G / np.outer(d, d)
+
+
+def test_mixed_wcorr_fp32_gram():
+    """The mixed form's W-correlation: its weighted Gram runs on FP32 MFMA
+    with FP32 accumulation inside each 2048-row tile.  cfg2's shape (N = 20000,
+    L = 5000, k = 50): against the W of the device's own reconstructions
+    formed in FP64 (isolates the FP32 Gram) within 1e-5, and against the FP64
+    oracle within 1e-3 overall (noise-level components carry the FP32
+    rounding of their small reconstructions) and 1e-5 on the signal block
+    (components 0..4)."""
+    x, dev, orc = _pair(20000, 5000, 50, 10, 2, seed=2, precision=1)
+    W = dev.wcorr_matrix()
+    rec = np.asarray(dev.reconstruct_many([[i] for i in range(50)]))
+    d_self = float(np.abs(W - _wcorr_np(rec, 5000)).max())
+    Wo = orc.wcorr()
+    d_orc = float(np.abs(W - Wo).max())
+    d_sig = float(np.abs(W[:5, :5] - Wo[:5, :5]).max())
+    print(f"mixed W (FP32 Gram): vs FP64 W of own reconstructions {d_self:.1e}, vs oracle {d_orc:.1e}, "
+          f"signal block {d_sig:.1e}")
+    assert d_self < 1e-5
+    assert d_orc < 1e-3
+    assert d_sig < 1e-5
+
+
 def test_mixed_batch_against_oracle():
     """The mixed batched path (FP32 panels and transforms, FP64 Gram,
     Cholesky and SVD, the final factorisations single-pass) against the FP64
