@@ -365,7 +365,10 @@ def cpu_reference(xs, Ls, ks, budget_s, workers):
             "value_1_core": round(d1 / t1, 2)}
 
 
-SERVE_BUDGET_GIB = 16
+SERVE_BUDGET_GIB = 32
+SERVE_INFLIGHT = 4      # waves in flight: one stream each, as many as the device's hardware queues
+SERVE_BACKLOG = 8192    # requests of the saturated backlog the capacity is measured on
+SERVE_FIXED_RATE = 20000.0   # req/s: a second open-loop stream at this rate, for comparison across builds
 
 
 def _oracle_request(args):
@@ -413,19 +416,25 @@ def cpu_serving_baseline(budget_s, workers):
 def serving(seconds):
     """cfg5: requests with N log-uniform in [500, 50000], L = N // 4,
     k ~ U{10..50}, reconstruct + forecast h = 50 of components
-    0..floor(2k/5)-1, mixed form.  Capacity from a saturated backlog; then an
-    open-loop Poisson stream at 70 % of it for `seconds`; then the same kind
-    of request served alone.  Waves are admitted against a device-memory
-    budget of SERVE_BUDGET_GIB (half per in-flight slot)."""
+    0..floor(2k/5)-1, mixed form.  Capacity from a saturated backlog of
+    SERVE_BACKLOG requests; then an open-loop Poisson stream at 70 % of it for
+    `seconds`, and one at the fixed rate SERVE_FIXED_RATE (the rate earlier
+    measurements of this loop were quoted at); then the same kind of request
+    served alone.  SERVE_INFLIGHT
+    waves are in flight, each a one-branch batch on its own stream; they are
+    admitted against a device-memory budget of SERVE_BUDGET_GIB (an equal share
+    per in-flight slot)."""
     import serve
-    srv = serve.Server(max_wave=512, inflight=2, hbm_budget=SERVE_BUDGET_GIB * 2 ** 30)
+    srv = serve.Server(max_wave=512, inflight=SERVE_INFLIGHT, hbm_budget=SERVE_BUDGET_GIB * 2 ** 30)
     # grow the in-flight slots to the largest wave before anything is timed
     srv.warm(sorted(serve.synthetic_requests(4096, rate=np.inf, seed=63), key=lambda r: -len(r.x))[:512])
-    done = srv.run(serve.synthetic_requests(2048, rate=np.inf, seed=6))
+    done = srv.run(serve.synthetic_requests(SERVE_BACKLOG, rate=np.inf, seed=6))
     capacity = len(done) / max(r.done for r in done)
     rate = 0.7 * capacity
     count = max(64, int(rate * seconds))
     stream = srv.run(serve.synthetic_requests(count, rate=rate, seed=61))
+    fixed = srv.run(serve.synthetic_requests(max(64, int(SERVE_FIXED_RATE * seconds)), rate=SERVE_FIXED_RATE,
+                                             seed=65)) if SERVE_FIXED_RATE < capacity else None
     alone = []
     one = serve.Server(max_wave=1, inflight=1)
     one.warm(sorted(serve.synthetic_requests(512, rate=np.inf, seed=64), key=lambda r: -len(r.x))[:1])
@@ -437,9 +446,9 @@ def serving(seconds):
     # the plain FP64 form on the same requests: its capacity, then an open-loop
     # stream at the mixed form's rate when it can sustain that, else at 70 %
     # of its own capacity
-    psrv = serve.Server(max_wave=512, inflight=2, precision=0, hbm_budget=SERVE_BUDGET_GIB * 2 ** 30)
+    psrv = serve.Server(max_wave=512, inflight=SERVE_INFLIGHT, precision=0, hbm_budget=SERVE_BUDGET_GIB * 2 ** 30)
     psrv.warm(sorted(serve.synthetic_requests(4096, rate=np.inf, seed=63), key=lambda r: -len(r.x))[:512])
-    pdone = psrv.run(serve.synthetic_requests(2048, rate=np.inf, seed=6))
+    pdone = psrv.run(serve.synthetic_requests(SERVE_BACKLOG, rate=np.inf, seed=6))
     pcap = len(pdone) / max(r.done for r in pdone)
     prate = rate if rate <= 0.7 * pcap else 0.7 * pcap
     pstream = psrv.run(serve.synthetic_requests(max(64, int(prate * seconds)), rate=prate, seed=61))
@@ -449,8 +458,10 @@ def serving(seconds):
     return {"workload": "N log-uniform in [500, 50000], L = N // 4, k ~ U{10..50}, group 0..floor(2k/5)-1, "
                         "h = 50, Poisson arrivals, mixed form (plain FP64 beside it)",
             "hbm_budget_GiB": SERVE_BUDGET_GIB, "max_wave_GB": round(max(srv_wave_bytes) / 1e9, 2),
+            "inflight_waves": SERVE_INFLIGHT, "branches_per_wave": 1, "capacity_backlog": SERVE_BACKLOG,
             "capacity_req_s": round(capacity, 1), "rate_req_s": round(rate, 1),
             "in_mix": mix, "alone": solo,
+            "at_fixed_rate": None if fixed is None else dict(rate_req_s=SERVE_FIXED_RATE, **serve.latency_stats(fixed)),
             "p50_in_mix_over_alone": round(mix["p50_ms"] / solo["p50_ms"], 3),
             "plain_fp64": plain}
 
@@ -717,6 +728,10 @@ def main():
         print(f"[bench] serving: capacity {serving_out['capacity_req_s']} req/s, at {serving_out['rate_req_s']} "
               f"req/s p50 {serving_out['in_mix']['p50_ms']} ms p99 {serving_out['in_mix']['p99_ms']} ms; alone "
               f"p50 {serving_out['alone']['p50_ms']} ms", flush=True)
+        if serving_out["at_fixed_rate"]:
+            fx = serving_out["at_fixed_rate"]
+            print(f"[bench] serving at {fx['rate_req_s']:.0f} req/s: p50 {fx['p50_ms']} ms p99 {fx['p99_ms']} ms",
+                  flush=True)
         pl = serving_out["plain_fp64"]
         print(f"[bench] serving plain FP64: capacity {pl['capacity_req_s']} req/s, at {pl['rate_req_s']} req/s "
               f"({pl['rate_is']}) p50 {pl['in_mix']['p50_ms']} ms p99 {pl['in_mix']['p99_ms']} ms", flush=True)

@@ -273,6 +273,14 @@ ssa_amd_batch *ssa_amd_batch_create(int count, const int *N, const int *L, const
  * k) and, when h > 0, an h-step LRF forecast of that group appended */
 ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, const int *k, const int *g, int h,
                                        int p, int q, int precision, unsigned long long seed);
+/* No counterpart in the reference: as ssa_amd_batch_create_ex with the number
+ * of parallel graph branches given (1..4; 0: SSA_AMD_BRANCHES or 2).  A
+ * serving loop that keeps several batches in flight takes 1 branch each, so
+ * its streams do not outnumber the device's hardware queues; a series'
+ * result does not depend on the branch count.  NULL for branches outside
+ * 0..4. */
+ssa_amd_batch *ssa_amd_batch_create_g(int count, const int *N, const int *L, const int *k, const int *g, int h,
+                                      int p, int q, int precision, unsigned long long seed, int branches);
 /* re-target an idle batch to a new set of series (same p, q, h, precision,
  * seed); device buffers are reused when large enough, so a serving loop that
  * rebinds a few batches allocates nothing per wave */

@@ -1340,10 +1340,11 @@ out:
     return rc;
 }
 
-ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, const int *k, const int *g, int h,
-                                       int p, int q, int precision, unsigned long long seed)
+ssa_amd_batch *ssa_amd_batch_create_g(int count, const int *N, const int *L, const int *k, const int *g, int h,
+                                      int p, int q, int precision, unsigned long long seed, int branches)
 {
     if (count < 1 || !N || !L || !k || q < 0 || h < 0 || (precision != 0 && precision != 1)) return NULL;
+    if (branches < 0 || branches > BATCH_MAX_BRANCHES) return NULL;
     ssa_amd_batch *b = (ssa_amd_batch *)calloc(1, sizeof(ssa_amd_batch));
     if (!b) return NULL;
     b->p = p <= 0 ? 8 : p;
@@ -1351,9 +1352,15 @@ ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, co
     b->h = h;
     b->precision = precision;
     b->seed = seed;
-    b->Gmax = batch_branches();
+    b->Gmax = branches ? branches : batch_branches();
     if (batch_bind(b, count, N, L, k, g)) { ssa_amd_batch_free(b); return NULL; }
     return b;
+}
+
+ssa_amd_batch *ssa_amd_batch_create_ex(int count, const int *N, const int *L, const int *k, const int *g, int h,
+                                       int p, int q, int precision, unsigned long long seed)
+{
+    return ssa_amd_batch_create_g(count, N, L, k, g, h, p, q, precision, seed, 0);
 }
 
 int ssa_amd_batch_rebind(ssa_amd_batch *b, int count, const int *N, const int *L, const int *k, const int *g)

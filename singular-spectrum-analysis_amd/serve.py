@@ -12,8 +12,10 @@ device bytes by `ssa_amd.series_bytes`), runs the wave as one batched step on
 the device (one `ssa_amd_batch`, mixed or plain form) and hands every request
 its output.  A request larger than a slot's share runs alone; one larger than
 the whole budget is refused (status -3, no output).
-Up to `inflight` waves are outstanding: while one runs on the GPU the next is
-assembled and uploaded.  The server keeps one batch per in-flight slot and
+Up to `inflight` waves are outstanding (default 4): while they run on the GPU
+the next is assembled and uploaded.  Each wave's batch runs on one stream (one
+graph branch), so the in-flight waves' streams do not outnumber the device's
+4 hardware queues and their kernels overlap.  The server keeps one batch per in-flight slot and
 re-targets it to each new wave (`ssa_amd_batch_rebind`), so once the slots
 have grown to the largest wave no device memory is allocated or freed.  Each
 request carries its own Philox key (its id), so its test matrix, and hence its
@@ -55,9 +57,11 @@ class Server:
     """Waves of requests through `ssa_amd.Batch` (all requests of a wave share
     h, p, q and the precision)."""
 
-    def __init__(self, p=10, q=2, h=50, precision=1, max_wave=512, inflight=2, seed=6, hbm_budget=None):
+    def __init__(self, p=10, q=2, h=50, precision=1, max_wave=512, inflight=4, seed=6, hbm_budget=None,
+                 branches=1):
         self.p, self.q, self.h, self.precision = p, q, h, precision
         self.max_wave, self.inflight, self.seed = max_wave, inflight, seed
+        self.branches = branches
         self.hbm_budget = hbm_budget
         self.waves = []          # (batch, requests) in flight
         self.idle = []           # batches ready for the next wave
@@ -112,7 +116,7 @@ class Server:
             b.rebind(xs, Ls, ks, groups=gs, keys=keys)
         else:
             b = ssa_amd.Batch(xs, Ls, ks, p=self.p, q=self.q, precision=self.precision, groups=gs, h=self.h,
-                              keys=keys)
+                              keys=keys, branches=self.branches)
             # one transform geometry for every wave size: a request's result
             # then does not depend on the wave it lands in
             b.set_wide_min(1)
@@ -121,8 +125,10 @@ class Server:
         self.wave_sizes.append(len(reqs))
         self.wave_bytes.append(sum(self.footprint(r) for r in reqs) if self.hbm_budget is not None else 0)
 
-    def _retire(self, block):
-        b, reqs = self.waves[0]
+    def _retire(self, block, j=0):
+        """Hand out the outputs of in-flight wave j once it has finished
+        (block: wait for it); True if it was retired."""
+        b, reqs = self.waves[j]
         if not block and not b.query():
             return False
         b.sync()
@@ -132,9 +138,20 @@ class Server:
             r.out = outs[i]
             r.status = int(status[i])
             r.done = now
-        self.waves.pop(0)
+        self.waves.pop(j)
         self.idle.append(b)
         return True
+
+    def _retire_finished(self):
+        """Retire every in-flight wave that has finished, in any order (a
+        small wave can finish before an older large one); how many."""
+        n, j = 0, 0
+        while j < len(self.waves):
+            if self._retire(block=False, j=j):
+                n += 1
+            else:
+                j += 1
+        return n
 
     def run(self, requests, t0=None):
         """Serve `requests` (arrival times relative to t0, seconds) in real
@@ -157,9 +174,9 @@ class Server:
                     self._launch(wave)
                     finished.extend(wave)
                 continue
-            if self.waves and self._retire(block=False):
+            if self.waves and self._retire_finished():
                 continue
-            # nothing ready: sleep until the next arrival or poll the oldest wave
+            # nothing ready: sleep until the next arrival or poll the waves
             nap = 2e-4
             if pending and not self.waves:
                 nap = max(0.0, min(0.01, pending[0].arrival - now))

@@ -163,6 +163,7 @@ SIGNATURES = [
     ("ssa_amd_batch_free", None, [_VP]),
     ("ssa_amd_batch_profile", _I, [_VP, _I, c_int_p, c_int_p, c_float_p]),
     ("ssa_amd_batch_create_ex", _VP, [_I, c_int_p, c_int_p, c_int_p, c_int_p, _I, _I, _I, _I, _U64]),
+    ("ssa_amd_batch_create_g", _VP, [_I, c_int_p, c_int_p, c_int_p, c_int_p, _I, _I, _I, _I, _U64, _I]),
     ("ssa_amd_batch_query", _I, [_VP]),
     ("ssa_amd_batch_status", _I, [_VP, _I]),
     ("ssa_amd_batch_plan", _I, [_VP, _I, c_int_p]),
@@ -618,16 +619,18 @@ class MSSA(_Device):
 class Batch:
     """Many uneven series in one device engine (no counterpart in the reference)."""
 
-    def __init__(self, xs, Ls, ks, p=10, q=2, precision=1, seed=4, groups=None, h=0, keys=None):
+    def __init__(self, xs, Ls, ks, p=10, q=2, precision=1, seed=4, groups=None, h=0, keys=None, branches=None):
         """groups[i]: reconstruct (and forecast) components 0..groups[i]-1
         of series i (default all k); h > 0 appends an h-step LRF forecast;
-        keys[i]: Philox key of series i (default derived from seed and i)."""
+        keys[i]: Philox key of series i (default derived from seed and i);
+        branches: parallel graph branches (1..4; default SSA_AMD_BRANCHES or 2)."""
         self._l = lib()
         self.horizon = int(h)
         self._dims(xs, Ls, ks)
         g = None if groups is None else np.ascontiguousarray(groups, dtype=np.int32)
-        self._h = self._l.ssa_amd_batch_create_ex(self.count, _ip(self.N), _ip(self.L), _ip(self.k),
-                                                 None if g is None else _ip(g), self.horizon, p, q, precision, seed)
+        self._h = self._l.ssa_amd_batch_create_g(self.count, _ip(self.N), _ip(self.L), _ip(self.k),
+                                                None if g is None else _ip(g), self.horizon, p, q, precision, seed,
+                                                int(branches or 0))
         if not self._h:
             raise RuntimeError("batch creation failed")
         self._upload(xs, keys)

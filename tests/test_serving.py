@@ -25,6 +25,34 @@ def test_alone_vs_in_the_mix_bit_identical():
         assert m.status == alone.status
 
 
+def test_branch_count_and_concurrent_waves_bit_identical():
+    """A wave split over 1, 2 or 4 graph branches, and the same requests served
+    by four one-branch waves in flight at once (their kernels overlap on the
+    device), give the same bits: every per-series computation is independent
+    of the engine and stream it runs on."""
+    S = _serve()
+    import ssa_amd
+    reqs = S.synthetic_requests(24, rate=np.inf, n_lo=500, n_hi=20000, seed=33)
+    xs, Ls, ks = [r.x for r in reqs], [r.L for r in reqs], [r.k for r in reqs]
+    gs, keys = [r.g for r in reqs], [S.request_key(r.rid) for r in reqs]
+    ref = None
+    for br in (1, 2, 4):
+        b = ssa_amd.Batch(xs, Ls, ks, p=10, q=2, precision=1, groups=gs, h=50, keys=keys, branches=br)
+        b.set_wide_min(1)
+        b.run(use_graph=False)
+        b.sync()
+        outs, st = b.outputs()
+        if ref is None:
+            ref = outs
+        else:
+            for i in range(len(reqs)):
+                assert np.array_equal(outs[i], ref[i]), (br, i)
+        del b
+    served = {r.rid: r for r in S.Server(max_wave=6, inflight=4).run(reqs)}
+    for i, r in enumerate(reqs):
+        assert np.array_equal(served[r.rid].out, ref[i]), r.rid
+
+
 def test_alone_vs_a_wave_of_700_bit_identical():
     """A wave of 700 short series launches its Gram, apply and test-matrix
     kernels one workgroup per row tile; a request served alone deals each
