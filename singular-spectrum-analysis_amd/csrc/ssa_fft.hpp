@@ -202,8 +202,30 @@ template <typename T, int LOGN, bool WIDE = false> struct FftGeom {
 
 template <typename T, int LOGN, bool WIDE = false> using FftRegs = C2<T>[FftGeom<T, LOGN, WIDE>::E];
 
+// FP32 twiddle base entries of a pass of stride NS = 2^NSLOG and radix
+// R = 2^RLOG: w^(2^b), b < RLOG, of butterfly m of this thread, sign applied.
+template <typename T, int SIGN, int NSLOG, int RLOG, int BPT, int NT>
+__device__ __forceinline__ void tw_fetch(C2<T> (&wp)[BPT][4], const C2<T> *tw, int tid)
+{
+    constexpr int R = 1 << RLOG, NS = 1 << NSLOG;
+#pragma unroll
+    for (int m = 0; m < BPT; m++) {
+        const int kk = (tid + m * NT) & (NS - 1);
+        const int step = kk << (TW_BASE_LOG - NSLOG - RLOG);
+#pragma unroll
+        for (int b = 0; (1 << b) < R; b++) {
+            wp[m][b] = tw[(1 << b) * step];
+            if constexpr (SIGN < 0) wp[m][b].y = -wp[m][b].y;
+        }
+    }
+}
+
+// One pass.  wp: for FP32 passes with NS > 1, this pass's twiddle base entries,
+// fetched by the previous pass before its LDS exchange so that their latency
+// overlaps the exchange (unused otherwise).
 template <typename T, int LOGN, int SIGN, bool SPLIT, int NSLOG, int RLOG, bool WIDE = false>
-__device__ __forceinline__ void fft_pass(FftRegs<T, LOGN, WIDE> &c, T *lre, T *lim, const C2<T> *tw, int tid)
+__device__ __forceinline__ void fft_pass(FftRegs<T, LOGN, WIDE> &c, T *lre, T *lim, const C2<T> *tw, int tid,
+                                         const C2<T> (&wp)[FftGeom<T, LOGN, WIDE>::E / (1 << RLOG)][4])
 {
     using G = FftGeom<T, LOGN, WIDE>;
     constexpr int N = G::N, NT = G::NT, E = G::E;
@@ -226,14 +248,6 @@ __device__ __forceinline__ void fft_pass(FftRegs<T, LOGN, WIDE> &c, T *lre, T *l
             }
         } else if constexpr (NS > 1) {
             // w^r for r < R from the base-table entries w^1, w^2, w^4, w^8
-            const int kk = j & (NS - 1);
-            const int step = kk << (TW_BASE_LOG - NSLOG - RLOG);
-            C2<T> wp[5];
-#pragma unroll
-            for (int b = 0; (1 << b) < R; b++) {
-                wp[b] = tw[(1 << b) * step];
-                if constexpr (SIGN < 0) wp[b].y = -wp[b].y;
-            }
 #pragma unroll
             for (int r = 1; r < R; r++) {
                 C2<T> w = C2<T>{1, 0};
@@ -241,7 +255,7 @@ __device__ __forceinline__ void fft_pass(FftRegs<T, LOGN, WIDE> &c, T *lre, T *l
 #pragma unroll
                 for (int b = 0; (1 << b) < R; b++)
                     if (r & (1 << b)) {
-                        w = first ? wp[b] : cmul(w, wp[b]);
+                        w = first ? wp[m][b] : cmul(w, wp[m][b]);
                         first = false;
                     }
                 v[r] = cmul(v[r], w);
@@ -252,6 +266,9 @@ __device__ __forceinline__ void fft_pass(FftRegs<T, LOGN, WIDE> &c, T *lre, T *l
         for (int r = 0; r < R; r++) c[m + r * BPT] = v[r];
     }
     if constexpr (NSLOG + RLOG < LOGN) {
+        constexpr int NEXT = G::NEXT;
+        C2<T> wn[E >> NEXT][4];
+        if constexpr (sizeof(T) == 4) tw_fetch<T, SIGN, NSLOG + RLOG, NEXT, (E >> NEXT), NT>(wn, tw, tid);
         // scatter to Stockham destinations, gather natural layout
         constexpr int PLANES = SPLIT ? 2 : 1;
 #pragma unroll
@@ -286,8 +303,7 @@ __device__ __forceinline__ void fft_pass(FftRegs<T, LOGN, WIDE> &c, T *lre, T *l
             }
             __syncthreads();
         }
-        constexpr int NEXT = G::NEXT;
-        fft_pass<T, LOGN, SIGN, SPLIT, NSLOG + RLOG, NEXT, WIDE>(c, lre, lim, tw, tid);
+        fft_pass<T, LOGN, SIGN, SPLIT, NSLOG + RLOG, NEXT, WIDE>(c, lre, lim, tw, tid, wn);
     }
 }
 
@@ -297,13 +313,16 @@ __device__ __forceinline__ void fft(FftRegs<T, LOGN, WIDE> &c, T *lre, T *lim, c
 {
     // The twiddles and LDS addresses depend only on the thread index, so the
     // compiler would hoist those of every pass out of the callers' loops and
-    // keep them all live (register spills).  Opaque copies of the thread index
-    // and the table pointer make each transform recompute its own.
+    // keep them all live (register spills).  An opaque copy of the thread
+    // index makes each transform recompute its own.  (The table pointer stays
+    // the kernel argument: an opaque copy of it would lose its global address
+    // space, and flat loads also count against the LDS wait counter.)
     int tid = threadIdx.x;
     const C2<T> *t = tw;
     asm volatile("" : "+v"(tid));
-    asm volatile("" : "+s"(t));
-    fft_pass<T, LOGN, SIGN, SPLIT, 0, FftGeom<T, LOGN, WIDE>::FIRST, WIDE>(c, lre, lim, t, tid);
+    using G = FftGeom<T, LOGN, WIDE>;
+    const C2<T> none[G::E >> G::FIRST][4] = {};   // the first pass (stride 1) has no twiddles
+    fft_pass<T, LOGN, SIGN, SPLIT, 0, G::FIRST, WIDE>(c, lre, lim, t, tid, none);
 }
 
 }  // namespace ssa
