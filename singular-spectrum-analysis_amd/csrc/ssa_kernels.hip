@@ -1221,10 +1221,18 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
                 const double ab = al * be, g2 = ga * ga;
                 if (ga != 0.0 && g2 > early * early * ab) big_local = 1;
                 if (ga != 0.0 && g2 > 1e-30 * ab) {
-                    const double zeta = (be - al) * rcp_nr(2.0 * ga);
-                    const double u = fma(zeta, zeta, 1.0);
-                    const double t = (zeta >= 0 ? 1.0 : -1.0) * rcp_nr(fabs(zeta) + u * rsq_nr(u));
-                    const double c = rsq_nr(fma(t, t, 1.0)), s = (sw ? -c : c) * t;
+                    // The rotation of tan(2 theta) = 2 ga / (be - al), |theta| <= pi/4
+                    // (Rutishauser's t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),
+                    // zeta = (be - al) / (2 ga)), from two refined reciprocal square
+                    // roots in series instead of four reciprocals and roots:
+                    //   r = sqrt(d^2 + 4 ga^2), cos 2theta = |d| / r,
+                    //   c = sqrt((1 + cos 2theta) / 2), s = sign(zeta) |ga| / (r c).
+                    const double d = be - al;
+                    const double rinv = rsq_nr(fma(d, d, 4.0 * g2));
+                    const double hc = fma(0.5 * fabs(d), rinv, 0.5);
+                    const double ic = rsq_nr(hc);
+                    const double sg = (d == 0.0 || ((d > 0.0) == (ga > 0.0))) ? 1.0 : -1.0;
+                    const double c = hc * ic, s = (sw ? -sg : sg) * fabs(ga) * rinv * ic;
                     // (min, max) <- (c min - s max, s min + c max), written for (p, q)
 #pragma unroll
                     for (int m = 0; m < MB; m++) {
