@@ -72,6 +72,12 @@ struct ssa_engine {
     /* pinned host staging */
     void *hpin;
     size_t hpin_bytes;
+    /* pinned staging of the descriptor upload, which is not waited for: the
+     * event marks when the copy has read it, before the next upload refills it */
+    void *dpin;
+    size_t dpin_bytes;
+    hipEvent_t dpin_ev;
+    int dpin_busy;
     /* graph of the batched step */
     hipGraphExec_t gexec;
     int graph_q;
@@ -352,14 +358,32 @@ static int rebuild_lists(ssa_engine *e)
     return 0;
 }
 
+/* Upload the descriptors when they changed.  The copy is ordered on the
+ * engine stream before every kernel that reads them, so the host does not
+ * wait for it (one host round trip less per call: every decomposition
+ * changes its series' test-matrix counter); its staging buffer is refilled
+ * only once the event recorded after the copy has completed. */
 static int sync_desc(ssa_engine *e)
 {
     if (e->lists_dirty) DEVCHK(rebuild_lists(e));
     if (e->desc_dirty) {
-        DEVCHK(pin_reserve(e, sizeof(ssa_desc) * e->count));
-        memcpy(e->hpin, e->hd, sizeof(ssa_desc) * e->count);
-        HIPCHK(hipMemcpyAsync(e->dd, e->hpin, sizeof(ssa_desc) * e->count, hipMemcpyHostToDevice, e->st));
-        HIPCHK(hipStreamSynchronize(e->st));
+        const size_t bytes = sizeof(ssa_desc) * (size_t)e->count;
+        if (!e->dpin_ev) HIPCHK(hipEventCreateWithFlags(&e->dpin_ev, hipEventDisableTiming));
+        if (e->dpin_busy) {
+            HIPCHK(hipEventSynchronize(e->dpin_ev));
+            e->dpin_busy = 0;
+        }
+        if (e->dpin_bytes < bytes) {
+            if (e->dpin) hipHostFree(e->dpin);
+            e->dpin = NULL;
+            e->dpin_bytes = 0;
+            HIPCHK(hipHostMalloc(&e->dpin, bytes, hipHostMallocDefault));
+            e->dpin_bytes = bytes;
+        }
+        memcpy(e->dpin, e->hd, bytes);
+        HIPCHK(hipMemcpyAsync(e->dd, e->dpin, bytes, hipMemcpyHostToDevice, e->st));
+        HIPCHK(hipEventRecord(e->dpin_ev, e->st));
+        e->dpin_busy = 1;
         e->desc_dirty = 0;
     }
     return 0;
@@ -510,6 +534,8 @@ void ssa_engine_destroy(ssa_engine *e)
     free(e->dyn_last);
     free(e->grp_last);
     if (e->hpin) hipHostFree(e->hpin);
+    if (e->dpin) hipHostFree(e->dpin);
+    if (e->dpin_ev) hipEventDestroy(e->dpin_ev);
     for (int i = 0; i < 2 * e->prof_cap; i++) hipEventDestroy(e->prof_ev[i]);
     free(e->prof_ev); free(e->prof_phase); free(e->prof_lgn);
     if (e->st) hipStreamDestroy(e->st);
@@ -1542,6 +1568,11 @@ int ssa_multi_step(ssa_multi *m, int q, int use_graph)
         DEVCHK(sync_desc(e));
         DEVCHK(prepare_step_groups(e));
     }
+    /* a graph launched on the first stream is not ordered after the other
+     * branches' streams: their descriptor uploads (not waited for on the
+     * host) must complete first */
+    for (int g = 1; g < m->G; g++)
+        if (m->es[g]->dpin_busy) HIPCHK(hipStreamWaitEvent(m->es[0]->st, m->es[g]->dpin_ev, 0));
     if (!use_graph) return multi_launch(m, q);
     if (!m->has_graph || m->graph_q != q) {
         if (m->has_graph) { hipGraphExecDestroy(m->gexec); m->has_graph = 0; }
