@@ -63,6 +63,9 @@ typedef struct ssa_desc {
     /* the batched step's output (N + h values) gathered here, every series'
      * slot after the previous one, so that one copy moves all of them */
     long long off_gather;
+    /* staged block-matvec scratch (series chunked in both dimensions): the
+     * input-chunk spectra of every column pair, 2 n T elements each */
+    long long off_mvs;
 } ssa_desc;
 
 /* FP64 small-matrix slots, each SSA_BMAX*SSA_BMAX doubles, relative to off_small.
@@ -107,6 +110,13 @@ int ssa_dev_spectra(int prec, int lgn, const ssa_desc *D, const ssa_item *items,
 int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel, int wide,
                    const ssa_desc *D, const ssa_item *items, int n_items,
                    void *bulk, const void *tw, void *stream);
+/* staged block matvec for series chunked in both dimensions: items (s, pair,
+ * input chunk) transform each input chunk once into the scratch at off_mvs;
+ * then items (s, pair, output chunk) sum the stored spectra and transform back */
+int ssa_dev_mv_spec(int prec, int lgn, int dir, int in_panel, int wide, const ssa_desc *D, const ssa_item *items,
+                    int n_items, void *bulk, const void *tw, void *stream);
+int ssa_dev_mv_acc(int prec, int lgn, int dir, int out_panel, int wide, const ssa_desc *D, const ssa_item *items,
+                   int n_items, void *bulk, const void *tw, void *stream);
 /* Philox Gaussian test matrix into panel Z; items (s, row tile) of the K-dim panel */
 int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, void *stream);
 /* Gram / apply: every panel of the launch has at most nb*16 columns (1 <= nb <= 5) */

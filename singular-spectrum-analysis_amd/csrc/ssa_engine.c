@@ -55,6 +55,7 @@ struct ssa_engine {
     ssa_item *hitems;
     long long hitems_n, hitems_cap;
     span spec[NLG], mvf[NLG], mva[NLG], tY[NBW], tZ[NBW], ser, ser_nb[NBW];   /* tY/tZ/ser_nb by nb */
+    span stgS[2][NLG], stgA[2][NLG];   /* staged block matvec (by direction): spectra, sums */
     span rec_step[NLG], olap_step;
     int lists_dirty, desc_dirty;
     int dyn_base;              /* first item of the per-call region */
@@ -132,6 +133,13 @@ static int g_force_lgn = -1;   /* autotune: every series planned at this length 
 
 static double plan_cost(int prec, int n, int lgn, int M, int AL, int SK, int b, int k, int q)
 {
+    /* transforms per column pair and block matvec, as the fused form does
+     * them: n_out (n_in + 1), or 1 + n_out when one input chunk is shared.
+     * The staged form does n_in + n_out, but writes the input spectra and
+     * reads them back once per output chunk; priced by its transform count
+     * alone it moved series to smaller lengths with more chunks, and ran
+     * slower, so a plan is priced as the fused form and the staged form only
+     * takes work off the plans chosen this way. */
     int nLc = M * AL;
     double fwd = (SK == 1) ? 1.0 + nLc : (double)nLc * (SK + 1);
     double adj = (nLc == 1) ? 1.0 + SK : (double)SK * (nLc + 1);
@@ -265,18 +273,65 @@ enum { MV_MIN_ITEMS = 1024 };
 enum { MV_WIDE_MIN_ITEMS = 2048 };
 /* component-loop splits of the batched step's reconstruction items */
 enum { RECON_SPLIT = 3 };
+/* A block matvec direction with more than one input chunk and at least
+ * MV_STAGE_MIN_OUT output chunks takes the staged form: the input chunks are
+ * transformed once into a scratch of the input-chunk spectra of every column
+ * pair, then every output chunk sums them. */
+#define MV_STAGE_MIN_OUT 2
+static int mv_staged(const ssa_desc *d, int dir)
+{
+    const int nL = d->M * d->AL;
+    const int n_in = dir == 0 ? d->SK : nL, n_out = dir == 0 ? nL : d->SK;
+    return n_in > 1 && n_out >= MV_STAGE_MIN_OUT;
+}
+
+/* T elements of the staged scratch of a series of block width b (either direction) */
+static long long mvs_elems(const ssa_desc *d, int b)
+{
+    if (!mv_staged(d, 0) && !mv_staged(d, 1)) return 0;
+    const int n_in = d->SK > d->M * d->AL ? d->SK : d->M * d->AL;
+    return 2LL * d->n * ((b + 1) / 2) * n_in;
+}
+
+/* the staged lists of a direction and bucket: (s, pair, input chunk) and (s, pair, output chunk) */
+static int build_staged(ssa_engine *e, int dir, int lgn)
+{
+    span *sp = &e->stgS[dir][lgn], *sa = &e->stgA[dir][lgn];
+    sp->off = e->hitems_n;
+    for (int s = 0; s < e->count; s++) {
+        const ssa_desc *d = &e->hd[s];
+        if (d->lgn != lgn || d->b <= 0 || !mv_staged(d, dir)) continue;
+        const int n_in = dir == 0 ? d->SK : d->M * d->AL;
+        for (int p = 0; p < (d->b + 1) / 2; p++)
+            for (int c = 0; c < n_in; c++)
+                if (items_push(e, s, p, c, 0)) return -1;
+    }
+    sp->n = (int)(e->hitems_n - sp->off);
+    sa->off = e->hitems_n;
+    for (int s = 0; s < e->count; s++) {
+        const ssa_desc *d = &e->hd[s];
+        if (d->lgn != lgn || d->b <= 0 || !mv_staged(d, dir)) continue;
+        const int n_out = dir == 0 ? d->M * d->AL : d->SK;
+        for (int p = 0; p < (d->b + 1) / 2; p++)
+            for (int o = 0; o < n_out; o++)
+                if (items_push(e, s, p, o, o + 1)) return -1;
+    }
+    sa->n = (int)(e->hitems_n - sa->off);
+    return 0;
+}
+
 static int build_matvec(ssa_engine *e, int dir, int lgn, span *sp)
 {
     long long units = 0;   /* (pair, output chunk) units of the launch */
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
-        if (d->lgn != lgn || d->b <= 0) continue;
+        if (d->lgn != lgn || d->b <= 0 || mv_staged(d, dir)) continue;
         units += (long long)((d->b + 1) / 2) * (dir == 0 ? d->M * d->AL : d->SK);
     }
     sp->off = e->hitems_n;
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
-        if (d->lgn != lgn || d->b <= 0) continue;
+        if (d->lgn != lgn || d->b <= 0 || mv_staged(d, dir)) continue;
         int nLc = d->M * d->AL;
         int n_in = dir == 0 ? d->SK : nLc, n_out = dir == 0 ? nLc : d->SK;
         long long g = n_in == 1 ? units / MV_MIN_ITEMS : 1;
@@ -305,6 +360,7 @@ static int rebuild_lists(ssa_engine *e)
     for (int lg = 0; lg < NLG; lg++) {
         if (build_matvec(e, 0, lg, &e->mvf[lg])) return -1;
         if (build_matvec(e, 1, lg, &e->mva[lg])) return -1;
+        if (build_staged(e, 0, lg) || build_staged(e, 1, lg)) return -1;
     }
     /* Gram / apply row tiles, one list per column-block count of the series */
     for (int nb = 0; nb < NBW; nb++) {
@@ -635,6 +691,12 @@ long long ssa_engine_series_bytes(int prec, int N, int L, int b, int k, int max_
                      align64(ML * k) + align64(K * k);
     bulk += align64((long long)gmax * pl.AL * pl.SK * pl.n);
     bulk += align64((long long)gmax * N + max_h + 64);
+    {
+        ssa_desc d1;
+        memset(&d1, 0, sizeof(d1));
+        d1.M = 1; d1.n = pl.n; d1.AL = pl.AL; d1.SK = pl.SK;
+        bulk += align64(mvs_elems(&d1, b));   /* staged block-matvec scratch */
+    }
     bulk += align64((long long)N + max_h);   /* gathered step output */
     int tilesY = ceil_div((int)ML, SSA_GRAM_ROWS), tilesZ = ceil_div((int)K, SSA_GRAM_ROWS);
     int tiles = tilesY > tilesZ ? tilesY : tilesZ, tout = ceil_div(N, SSA_GRAM_ROWS);
@@ -667,6 +729,7 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
         d->off_stage = off; off = align64(off + d->stage_cap);
         d->out_cap = (long long)gmax * d->N + max_h + 64;
         d->off_out = off; off = align64(off + d->out_cap);
+        d->off_mvs = off; off = align64(off + mvs_elems(d, bm));
         int tiles = d->tilesY > d->tilesZ ? d->tilesY : d->tilesZ;
         int tout = ceil_div(d->N, SSA_GRAM_ROWS);
         if (tout > tiles) tiles = tout;
@@ -702,7 +765,7 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
     for (int s = 0; s < e->count; s++) {
         ssa_desc *d = &e->hd[s];
         d->off_Y += delta; d->off_Z += delta; d->off_U += delta; d->off_V += delta;
-        d->off_stage += delta; d->off_out += delta; d->off_gather += delta;
+        d->off_stage += delta; d->off_out += delta; d->off_gather += delta; d->off_mvs += delta;
         if (d->b == 0) { d->b = e->bmax[s]; d->k = e->kmax[s]; }
         if (d->gsize <= 0 || d->gsize > d->k) d->gsize = d->k;
     }
@@ -847,13 +910,28 @@ static int bmax_all(const ssa_engine *e)
 
 static int matvec_all(ssa_engine *e, int dir, int in_panel, int out_panel)
 {
+    const int ph = dir == 0 ? PH_MV_FWD : PH_MV_ADJ;
     for (int lg = 6; lg < NLG; lg++) {
         span sp = dir == 0 ? e->mvf[lg] : e->mva[lg];
         if (!sp.n) continue;
-        prof_begin(e, dir == 0 ? PH_MV_FWD : PH_MV_ADJ, lg);
+        prof_begin(e, ph, lg);
         DEVCHK(ssa_dev_matvec(e->prec, lg, dir, in_panel, out_panel, sp.n >= e->mv_wide_min, e->dd,
                               it_ptr(e, sp), sp.n, e->arena_a,
                               e->tw, e->st));
+        prof_end(e);
+    }
+    for (int lg = 6; lg < NLG; lg++) {
+        const span ss = e->stgS[dir][lg], sa = e->stgA[dir][lg];
+        if (!ss.n) continue;
+        /* one transform geometry for both launches of a bucket */
+        const int wide = (ss.n > sa.n ? ss.n : sa.n) >= e->mv_wide_min;
+        prof_begin(e, ph, lg);
+        DEVCHK(ssa_dev_mv_spec(e->prec, lg, dir, in_panel, wide, e->dd, it_ptr(e, ss), ss.n, e->arena_a, e->tw,
+                               e->st));
+        prof_end(e);
+        prof_begin(e, ph, lg);
+        DEVCHK(ssa_dev_mv_acc(e->prec, lg, dir, out_panel, wide, e->dd, it_ptr(e, sa), sa.n, e->arena_a, e->tw,
+                              e->st));
         prof_end(e);
     }
     return 0;

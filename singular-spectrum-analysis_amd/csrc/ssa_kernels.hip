@@ -308,6 +308,121 @@ __global__ __launch_bounds__((FftGeom<T, LOGN, W>::NT), (FftGeom<T, LOGN, W>::OC
 }
 
 // ---------------------------------------------------------------------------
+// staged block matvec, for series chunked in both dimensions (more than one
+// input and more than one output chunk): k_matvec would transform every input
+// chunk again for every output chunk (n_in * n_out + n_out transforms per
+// column pair); here each input chunk is transformed once into the series'
+// scratch (k_mv_spec), and each output chunk sums the stored spectra
+// (k_mv_acc): n_in + n_out transforms.  The stored spectrum is the transform's
+// register values, so the sums are formed from the same values in the same
+// order as in k_matvec.  Scratch layout: spectrum of input chunk p of column
+// pair c at off_mvs + 2 N (c n_in + p), N complex values in natural order.
+// ---------------------------------------------------------------------------
+template <typename T, int LOGN, bool W>
+__global__ __launch_bounds__((FftGeom<T, LOGN, W>::NT), (FftGeom<T, LOGN, W>::OCC)) void k_mv_spec(
+    const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items, T *bulk, const C2<T> *__restrict__ tw,
+    int dir, int in_panel, int n_items)
+{
+    using G = FftGeom<T, LOGN, W>;
+    constexpr int E = G::E, NT = G::NT;
+    constexpr bool SPLIT = G::SPLIT;
+    __shared__ T lre[G::LDS];
+    __shared__ T lim[SPLIT ? 1 : G::LDS];
+    const int item = xcd_item(n_items);
+    if (item < 0) return;
+    const ssa_item it = items[item];   // (s, pair, input chunk p)
+    const ssa_desc d = D[it.s];
+    const int c0 = 2 * it.x, c1 = c0 + 1, p = it.y;
+    const bool two = c1 < d.b;
+    const Panel<T> pin = panel_of<T>(bulk, d, in_panel);
+    const int n_in = dir == 0 ? d.SK : d.M * d.AL;
+    int row0, len;
+    if (dir == 0) {
+        row0 = p * d.CK;
+        len = min(d.CK, d.K - row0);
+    } else {
+        const int m = p / d.AL, a = p - m * d.AL;
+        row0 = m * d.L + a * d.CL;
+        len = min(d.CL, d.L - a * d.CL);
+    }
+    const T *a0 = pin.p + (long long)c0 * pin.ld + row0;
+    const T *a1 = pin.p + (long long)(two ? c1 : c0) * pin.ld + row0;
+    C2<T> z[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int q = threadIdx.x + e * NT;
+        z[e].x = (q < len) ? a0[q] : (T)0;
+        z[e].y = (two && q < len) ? a1[q] : (T)0;
+    }
+    fft<T, LOGN, 1, SPLIT, W>(z, lre, lim, tw);
+    C2<T> *S = reinterpret_cast<C2<T> *>(bulk + d.off_mvs) + ((long long)it.x * n_in + p) * G::N;
+#pragma unroll
+    for (int e = 0; e < E; e++) S[threadIdx.x + e * NT] = z[e];
+}
+
+template <typename T, int LOGN, bool W>
+__global__ __launch_bounds__((FftGeom<T, LOGN, W>::NT), (FftGeom<T, LOGN, W>::OCC)) void k_mv_acc(
+    const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items, T *bulk, const C2<T> *__restrict__ tw,
+    int dir, int out_panel, int n_items)
+{
+    using G = FftGeom<T, LOGN, W>;
+    constexpr int E = G::E, NT = G::NT;
+    constexpr bool SPLIT = G::SPLIT;
+    __shared__ T lre[G::LDS];
+    __shared__ T lim[SPLIT ? 1 : G::LDS];
+    const int item = xcd_item(n_items);
+    if (item < 0) return;
+    const ssa_item it = items[item];   // (s, pair, output chunk o)
+    const ssa_desc d = D[it.s];
+    const int c0 = 2 * it.x, c1 = c0 + 1, o = it.y;
+    const bool two = c1 < d.b;
+    const Panel<T> pout = panel_of<T>(bulk, d, out_panel);
+    const int n_in = dir == 0 ? d.SK : d.M * d.AL;
+    const C2<T> *X = reinterpret_cast<const C2<T> *>(bulk + d.off_spec);
+    const C2<T> *S = reinterpret_cast<const C2<T> *>(bulk + d.off_mvs) + (long long)it.x * n_in * G::N;
+    C2<T> acc[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) acc[e] = C2<T>{0, 0};
+    for (int p = 0; p < n_in; p++) {
+        // spectra are indexed [L chunk][K chunk]
+        const int lo = dir == 0 ? o : p, ko = dir == 0 ? p : o;
+        const C2<T> *Xo = X + ((long long)lo * d.SK + ko) * G::N;
+        const C2<T> *Sp = S + (long long)p * G::N;
+        C2<T> xv[E], zv[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            xv[e] = Xo[threadIdx.x + e * NT];
+            zv[e] = Sp[threadIdx.x + e * NT];
+        }
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            acc[e].x += xv[e].x * zv[e].x - xv[e].y * zv[e].y;
+            acc[e].y += xv[e].x * zv[e].y + xv[e].y * zv[e].x;
+        }
+    }
+    fft<T, LOGN, 1, SPLIT, W>(acc, lre, lim, tw);
+    int row0, len;
+    if (dir == 0) {
+        const int m = o / d.AL, a = o - m * d.AL;
+        row0 = m * d.L + a * d.CL;
+        len = min(d.CL, d.L - a * d.CL);
+    } else {
+        row0 = o * d.CK;
+        len = min(d.CK, d.K - row0);
+    }
+    T *b0 = pout.p + (long long)c0 * pout.ld + row0;
+    T *b1 = pout.p + (long long)c1 * pout.ld + row0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int q = threadIdx.x + e * NT;
+        if (q < len) {
+            b0[q] = acc[e].x;
+            if (two) b1[q] = acc[e].y;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 MFMA 16x16x4 (v_mfma_f64_16x16x4f64).  Operand layout: lane l holds
 // A[l%16][l/16] of the 16x4 A and B[l/16][l%16] of the 4x16 B; the 4
 // accumulator values of lane l are D[l/16 + 4v][l%16], v = 0..3.
@@ -1877,6 +1992,56 @@ int ssa_dev_matvec(int prec, int lgn, int dir, int in_panel, int out_panel, int 
 #define LAUNCH(T, L, WD)                                                                                      \
     hipLaunchKernelGGL((k_matvec<T, L, WD>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L, WD>::NT), 0, st, D,  \
                        items, (T *)bulk, (const C2<T> *)tw, dir, in_panel, out_panel, n_items)
+#define CASE(L)                                                                                               \
+    case L:                                                                                                   \
+        if (prec == SSA_PREC_F64) {                                                                           \
+            if (wide) LAUNCH(double, L, true);                                                                \
+            else LAUNCH(double, L, false);                                                                    \
+        } else {                                                                                              \
+            if (wide) LAUNCH(float, L, true);                                                                 \
+            else LAUNCH(float, L, false);                                                                     \
+        }                                                                                                     \
+        break;
+    switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
+#undef CASE
+#undef LAUNCH
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_mv_spec(int prec, int lgn, int dir, int in_panel, int wide, const ssa_desc *D, const ssa_item *items,
+                    int n_items, void *bulk, const void *tw, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (!lgn_ok(prec, lgn)) return -2;
+    hipStream_t st = (hipStream_t)stream;
+#define LAUNCH(T, L, WD)                                                                                      \
+    hipLaunchKernelGGL((k_mv_spec<T, L, WD>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L, WD>::NT), 0, st, D, \
+                       items, (T *)bulk, (const C2<T> *)tw, dir, in_panel, n_items)
+#define CASE(L)                                                                                               \
+    case L:                                                                                                   \
+        if (prec == SSA_PREC_F64) {                                                                           \
+            if (wide) LAUNCH(double, L, true);                                                                \
+            else LAUNCH(double, L, false);                                                                    \
+        } else {                                                                                              \
+            if (wide) LAUNCH(float, L, true);                                                                 \
+            else LAUNCH(float, L, false);                                                                     \
+        }                                                                                                     \
+        break;
+    switch (lgn) { SSA_LGN_CASES(CASE) default: return -2; }
+#undef CASE
+#undef LAUNCH
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_mv_acc(int prec, int lgn, int dir, int out_panel, int wide, const ssa_desc *D, const ssa_item *items,
+                   int n_items, void *bulk, const void *tw, void *stream)
+{
+    if (n_items <= 0) return 0;
+    if (!lgn_ok(prec, lgn)) return -2;
+    hipStream_t st = (hipStream_t)stream;
+#define LAUNCH(T, L, WD)                                                                                      \
+    hipLaunchKernelGGL((k_mv_acc<T, L, WD>), dim3(xcd_grid(n_items)), dim3(FftGeom<T, L, WD>::NT), 0, st, D,  \
+                       items, (T *)bulk, (const C2<T> *)tw, dir, out_panel, n_items)
 #define CASE(L)                                                                                               \
     case L:                                                                                                   \
         if (prec == SSA_PREC_F64) {                                                                           \
