@@ -1376,9 +1376,15 @@ __global__ __launch_bounds__(256) void k_svd(const ssa_desc *D, const ssa_item *
     }
     __syncthreads();
     if (threadIdx.x < b) {
+        // a NaN norm (non-finite input) ranks below every number, so the
+        // ranks stay a permutation of 0..b-1 and perm indexes A in bounds
         const int i = threadIdx.x;
+        const double si = S[i] == S[i] ? S[i] : -1.0;
         int rank = 0;
-        for (int j = 0; j < b; j++) rank += (S[j] > S[i]) || (S[j] == S[i] && j < i);
+        for (int j = 0; j < b; j++) {
+            const double sj = S[j] == S[j] ? S[j] : -1.0;
+            rank += (sj > si) || (sj == si && j < i);
+        }
         perm[rank] = i;
     }
     __syncthreads();

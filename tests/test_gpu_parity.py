@@ -306,6 +306,37 @@ def test_staged_block_matvec(precision, N, L, k, tol):
         assert rel2(a, b) < tol
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_nonfinite_input_leaves_context_usable(precision):
+    """A series holding a NaN decomposes without a valid spectrum: a NaN
+    Gram fails every Cholesky pivot test, so the columns are marked dead and
+    sigma comes out zero (the Jacobi ranking would put NaN norms last, so its
+    indices stay in range either way).  The same context, given a finite
+    series by update_signal, then gives the bits a fresh context gives: no
+    NaN survives in padding or scratch."""
+    S = _ssa()
+    N, L, k, p = 3000, 750, 10, 10
+    x = workloads.single(N, seed=77)
+    bad = x.copy()
+    bad[1234] = np.nan
+    K = N - L + 1
+    om = O.philox_gaussian(4242, 0, K, k + p)
+    used = S.SSA(bad, L, q=2, precision=precision)
+    used.decompose(k, oversampling=p)
+    sv = used.singular_values
+    assert not (np.all(np.isfinite(sv)) and np.all(sv > 0))
+    used.update_signal(x)
+    used.prepare(k, p)
+    used.inject_omega(om)
+    used.decompose(k, oversampling=p)
+    fresh = S.SSA(x, L, q=2, precision=precision)
+    fresh.prepare(k, p)
+    fresh.inject_omega(om)
+    fresh.decompose(k, oversampling=p)
+    assert np.array_equal(used.singular_values, fresh.singular_values)
+    assert np.array_equal(used.reconstruct(list(range(k))), fresh.reconstruct(list(range(k))))
+
+
 def _wcorr_np(rec, L):
     """W-correlation by definition (MKL/ssa_opt.h:2407-2477) from the rows of
     rec, in FP64: weights w_t = min(t + 1, L, K, N - t)."""
