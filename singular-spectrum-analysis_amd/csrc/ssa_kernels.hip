@@ -1714,8 +1714,14 @@ __global__ __launch_bounds__(256) void k_olap(const ssa_desc *D, const ssa_item 
 // count min(t + 1, L, K, N - t), which is small only near the two ends, so the
 // ends carry almost all of the error of an FP32 reconstruction.  There the sums
 // have at most SSA_EDGE terms each.  Item (s, gid); products and sums in FP64.
+// The group's components are staged EDGE_CG at a time: every load of a stage
+// is issued before its one barrier, so a group pays one memory latency per
+// EDGE_CG components, not one per component.  Each sample is summed in
+// component order, terms in index order, whatever the staging.
 // scale_sigma, weight, limit_gsize and the group layout as in k_recon/k_olap.
 // ---------------------------------------------------------------------------
+constexpr int EDGE_CG = 8;
+
 template <typename T>
 __global__ __launch_bounds__(2 * SSA_EDGE) void k_edge(const ssa_desc *__restrict__ D,
                                                        const ssa_item *__restrict__ items,
@@ -1724,8 +1730,9 @@ __global__ __launch_bounds__(2 * SSA_EDGE) void k_edge(const ssa_desc *__restric
                                                        const double *__restrict__ small, int scale_sigma,
                                                        int weight, int limit_gsize)
 {
-    constexpr int E = SSA_EDGE;
-    __shared__ double uh[E], vh[E], ut[E], vt[E];
+    constexpr int E = SSA_EDGE, CG = EDGE_CG;
+    __shared__ double uh[CG][E], vh[CG][E], ut[CG][E], vt[CG][E];
+    __shared__ int gl[2 * E];
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const int4 gd = gdesc[it.x];
@@ -1738,32 +1745,104 @@ __global__ __launch_bounds__(2 * SSA_EDGE) void k_edge(const ssa_desc *__restric
     const bool mine = t >= 0 && t < N && (!tail || t >= E);
     const int lo = max(0, t - K + 1), hi = min(t, L - 1);
     const double *sig = small + d.off_small + (long long)SSA_SLOT_MISC * SSA_BMAX * SSA_BMAX + SSA_MISC_SIGMA;
+    // rows of u and v this thread stages: head rows [0, E) and tail rows
+    // [L - E, L) of u, [0, E) and [K - E, K) of v (clamped into the columns;
+    // rows outside them are staged as zero)
+    const int ur = tail ? L - E + j : j, vr = tail ? K - E + j : j;
+    const bool uin = ur >= 0 && ur < L, vin = vr >= 0 && vr < K;
+    const int urc = min(max(ur, 0), L - 1), vrc = min(max(vr, 0), K - 1);
+    const T *ubase = bulk + d.off_U + (long long)m * L + urc;
+    const T *vbase = bulk + d.off_V + vrc;
     double acc = 0.0;
-    for (int g = 0; g < gd.y; g++) {
-        const int idx = glist[gd.x + g];
-        if (idx < 0 || idx >= d.k || (limit_gsize && idx >= d.gsize)) continue;   // uniform over the block
-        const T *u = bulk + d.off_U + (long long)idx * d.ML + (long long)m * L;
-        const T *v = bulk + d.off_V + (long long)idx * K;
-        __syncthreads();   // the previous component's values are no longer read
-        // head rows [0, E) and tail rows [L - E, L) of u, [0, E) and [K - E, K) of v
-        if (!tail) {
-            uh[j] = j < L ? (double)u[j] : 0.0;
-            vh[j] = j < K ? (double)v[j] : 0.0;
-        } else {
-            ut[j] = L - E + j >= 0 ? (double)u[L - E + j] : 0.0;
-            vt[j] = K - E + j >= 0 ? (double)v[K - E + j] : 0.0;
+    // the group's list in blocks of 2E entries: component indices in gl, -1
+    // for one that is skipped
+    for (int b0 = 0; b0 < gd.y; b0 += 2 * E) {
+        const int ng = min(2 * E, gd.y - b0);
+        __syncthreads();   // the previous block's indices are no longer read
+        if ((int)threadIdx.x < ng) {
+            const int idx = glist[gd.x + b0 + threadIdx.x];
+            gl[threadIdx.x] = (idx < 0 || idx >= d.k || (limit_gsize && idx >= d.gsize)) ? -1 : idx;
         }
-        __syncthreads();
-        if (!mine) continue;
-        double s = 0.0;
-        if (!tail) {
-            // t < E: both indices below E
-            for (int i = lo; i <= hi; i++) s = fma(uh[i], vh[t - i], s);
-        } else {
-            // t >= N - E: i >= L - E and t - i >= K - E
-            for (int i = lo; i <= hi; i++) s = fma(ut[i - (L - E)], vt[t - i - (K - E)], s);
+        for (int g0 = 0; g0 < ng; g0 += CG) {
+            __syncthreads();   // gl is written; the previous stage's values are no longer read
+            int cid[CG];
+            double a[CG], bv[CG], sc[CG];
+            // every load of the stage is issued before any of them is used
+#pragma unroll
+            for (int c = 0; c < CG; c++) {
+                cid[c] = g0 + c < ng ? gl[g0 + c] : -1;
+                const int ci = max(cid[c], 0);
+                a[c] = (double)ubase[(long long)ci * d.ML];
+                bv[c] = (double)vbase[(long long)ci * K];
+                sc[c] = scale_sigma ? sig[ci] : 1.0;
+            }
+#pragma unroll
+            for (int c = 0; c < CG; c++) {
+                const bool ok = cid[c] >= 0;
+                const double av = ok && uin ? a[c] : 0.0, vv = ok && vin ? bv[c] : 0.0;
+                if (!tail) {
+                    uh[c][j] = av;
+                    vh[c][j] = vv;
+                } else {
+                    ut[c][j] = av;
+                    vt[c][j] = vv;
+                }
+            }
+            __syncthreads();
+            if (!mine) continue;
+            // head (t < E): both indices below E; tail (t >= N - E): i >= L - E
+            // and t - i >= K - E.  Terms i = lo..hi in order, 8 at a time so
+            // that their LDS loads are in flight together: term lo + ii reads
+            // up[ii] and vq[-ii] (constant offsets within a group of 8); the
+            // last, partial group reads clamped positions and adds only its
+            // terms.  Two components' sums run side by side (independent
+            // chains); each is added to acc in component order.
+            const int uo = tail ? L - E : 0, vo = tail ? K - E : 0, nt = hi - lo + 1;
+#pragma unroll
+            for (int c = 0; c < CG; c += 2) {
+                if (cid[c] < 0 && cid[c + 1] < 0) continue;
+                const double *up0 = (tail ? ut[c] : uh[c]) + (lo - uo);
+                const double *vq0 = (tail ? vt[c] : vh[c]) + (t - lo - vo);
+                const double *up1 = (tail ? ut[c + 1] : uh[c + 1]) + (lo - uo);
+                const double *vq1 = (tail ? vt[c + 1] : vh[c + 1]) + (t - lo - vo);
+                double s0 = 0.0, s1 = 0.0;
+                int ii = 0;
+                for (; ii + 8 <= nt; ii += 8) {
+                    double x0[8], y0[8], x1[8], y1[8];
+#pragma unroll
+                    for (int r = 0; r < 8; r++) {
+                        x0[r] = up0[ii + r];
+                        y0[r] = (vq0 - ii - 7)[7 - r];
+                        x1[r] = up1[ii + r];
+                        y1[r] = (vq1 - ii - 7)[7 - r];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 8; r++) {
+                        s0 = fma(x0[r], y0[r], s0);
+                        s1 = fma(x1[r], y1[r], s1);
+                    }
+                }
+                if (ii < nt) {
+                    double x0[8], y0[8], x1[8], y1[8];
+#pragma unroll
+                    for (int r = 0; r < 8; r++) {
+                        const int q = min(ii + r, nt - 1);
+                        x0[r] = up0[q];
+                        y0[r] = vq0[-q];
+                        x1[r] = up1[q];
+                        y1[r] = vq1[-q];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 8; r++)
+                        if (ii + r < nt) {
+                            s0 = fma(x0[r], y0[r], s0);
+                            s1 = fma(x1[r], y1[r], s1);
+                        }
+                }
+                if (cid[c] >= 0) acc = fma(sc[c], s0, acc);
+                if (cid[c + 1] >= 0) acc = fma(sc[c + 1], s1, acc);
+            }
         }
-        acc = fma(scale_sigma ? sig[idx] : 1.0, s, acc);
     }
     if (!mine) return;
     const int c = min(min(t + 1, L), min(K, N - t));
