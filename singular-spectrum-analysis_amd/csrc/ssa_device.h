@@ -23,6 +23,7 @@ extern "C" {
 /* element multiple of the Y and Z panels' leading dimensions (128 bytes of FP32) */
 #define SSA_PANEL_ALIGN 32
 #define SSA_EDGE 64           /* samples at each end of a reconstruction summed directly (power of two) */
+#define SSA_OLAP_TILE 256     /* reconstruction samples per overlap-add work item of a short launch */
 
 enum { SSA_PREC_F64 = 0, SSA_PREC_F32 = 1 };
 
@@ -143,10 +144,11 @@ int ssa_dev_svd(int bmax, int r_slot, int rinv_slot, double early, const ssa_des
 int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, int nsplit, const ssa_desc *D,
                   const ssa_item *items, int n_items, const int *gdesc, const int *glist, void *bulk,
                   const double *small, const void *tw, void *stream);
-/* overlap-add of the staging pieces: items (s, gid, t tile); weight 0: w_t, 1: sqrt(w_t);
- * channel index taken from gdesc[gid].z.  Output of gid goes to out + gid*N. */
-int ssa_dev_olap(int prec, int weight, int nsplit, const ssa_desc *D, const ssa_item *items, int n_items,
-                 const int *gdesc, void *bulk, void *stream);
+/* overlap-add of the staging pieces: items (s, gid, t tile of `tile` samples, a multiple
+ * of 256); weight 0: w_t, 1: sqrt(w_t); channel index taken from gdesc[gid].z.  Output of
+ * gid goes to out + gid*N. */
+int ssa_dev_olap(int prec, int weight, int nsplit, int tile, const ssa_desc *D, const ssa_item *items,
+                 int n_items, const int *gdesc, void *bulk, void *stream);
 /* component spectra cache: items (s, component pair p, chunk c); components 2p and
  * 2p + 1 (< k) of U's row chunk c (c < M AL) or V's column chunk c - M AL, transformed
  * together and separated; the n/2 + 1 values of each land at off_cache (ssa_desc) */

@@ -28,7 +28,7 @@
 
 #define NLG 14                 /* lgn buckets 0..13 (6..13 used) */
 #define NBW 6                  /* Gram / apply lists by column blocks nb = 1..5 */
-#define OLAP_TILE 1024
+#define OLAP_TILE_STEP 1024   /* overlap-add samples per work item in the batched step */
 
 typedef struct { long long off; int n; } span;
 
@@ -273,6 +273,9 @@ enum { MV_MIN_ITEMS = 1024 };
 enum { MV_WIDE_MIN_ITEMS = 2048 };
 /* component-loop splits of the batched step's reconstruction items */
 enum { RECON_SPLIT = 3 };
+/* context reconstructions: split a group's list up to this many ways while
+ * the launch holds at most RECON_CTX_ITEMS pieces */
+enum { RECON_SPLIT_CTX = 8, RECON_CTX_ITEMS = 64 };
 /* A block matvec direction with more than one input chunk and at least
  * MV_STAGE_MIN_OUT output chunks takes the staged form: the input chunks are
  * transformed once into a scratch of the input-chunk spectra of every column
@@ -405,7 +408,7 @@ static int rebuild_lists(ssa_engine *e)
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
         if (d->k <= 0) continue;
-        for (int t = 0; t < ceil_div(d->N, OLAP_TILE); t++) if (items_push(e, s, 0, t, 0)) return -1;
+        for (int t = 0; t < ceil_div(d->N, OLAP_TILE_STEP); t++) if (items_push(e, s, 0, t, 0)) return -1;
     }
     e->olap_step.n = (int)(e->hitems_n - e->olap_step.off);
     e->dyn_base = (int)e->hitems_n;
@@ -1175,6 +1178,15 @@ static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weigh
     const ssa_desc *d = &e->hd[s];
     const int cached = use_cache && e->cache_valid;
     const int n_pieces = ng * d->AL * d->SK;
+    /* A launch of few pieces splits each group's component list S ways (the
+     * overlap-add sums the S pieces), so that more workgroups share a short
+     * reconstruction; as many as the staging slots hold for ng groups. */
+    int S = 1;
+    if (!cached) {
+        const long long slots = d->stage_cap / ((long long)d->AL * d->SK * d->n);
+        while (2 * S <= RECON_SPLIT_CTX && (long long)ng * 2 * S <= slots && n_pieces * 2 * S <= RECON_CTX_ITEMS)
+            S *= 2;
+    }
     e->hitems_n = e->dyn_base;
     span rec, ol, edge;
     rec.off = e->hitems_n;
@@ -1182,14 +1194,14 @@ static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weigh
         /* two pieces per item (one inverse transform) */
         for (int p = 0; p < n_pieces; p += 2) DEVCHK(items_push(e, s, p, 0, 0));
     } else {
-        for (int g = 0; g < ng; g++)
+        for (int g = 0; g < ng * S; g++)
             for (int a = 0; a < d->AL; a++)
                 for (int sk = 0; sk < d->SK; sk++) DEVCHK(items_push(e, s, g, a, sk));
     }
     rec.n = (int)(e->hitems_n - rec.off);
     ol.off = e->hitems_n;
     for (int g = 0; g < ng; g++)
-        for (int t = 0; t < ceil_div(d->N, OLAP_TILE); t++) DEVCHK(items_push(e, s, g, t, 0));
+        for (int t = 0; t < ceil_div(d->N, SSA_OLAP_TILE); t++) DEVCHK(items_push(e, s, g, t, 0));
     ol.n = (int)(e->hitems_n - ol.off);
     edge.off = e->hitems_n;
     for (int g = 0; g < ng; g++) DEVCHK(items_push(e, s, g, 0, 0));
@@ -1199,9 +1211,9 @@ static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weigh
         DEVCHK(ssa_dev_recon_cached(e->prec, d->lgn, scale_sigma, e->dd, it_ptr(e, rec), rec.n, n_pieces, e->dgroups,
                                     e->dgroups + 4 * ng, e->arena_a, e->small, e->tw, e->st));
     else
-        DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, 0, 1, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
+        DEVCHK(ssa_dev_recon(e->prec, d->lgn, scale_sigma, 0, S, e->dd, it_ptr(e, rec), rec.n, e->dgroups,
                              e->dgroups + 4 * ng, e->arena_a, e->small, e->tw, e->st));
-    DEVCHK(ssa_dev_olap(e->prec, weight, 1, e->dd, it_ptr(e, ol), ol.n, e->dgroups, e->arena_a, e->st));
+    DEVCHK(ssa_dev_olap(e->prec, weight, S, SSA_OLAP_TILE, e->dd, it_ptr(e, ol), ol.n, e->dgroups, e->arena_a, e->st));
     DEVCHK(ssa_dev_edge(e->prec, scale_sigma, weight, 0, e->dd, it_ptr(e, edge), edge.n, e->dgroups,
                         e->dgroups + 4 * ng, e->arena_a, e->small, e->st));
     return 0;
@@ -1487,7 +1499,7 @@ static int launch_step(ssa_engine *e, int q)
         prof_end(e);
     }
     prof_begin(e, PH_OLAP, 0);
-    DEVCHK(ssa_dev_olap(e->prec, 0, RECON_SPLIT, e->dd, it_ptr(e, e->olap_step), e->olap_step.n, e->dgroups,
+    DEVCHK(ssa_dev_olap(e->prec, 0, RECON_SPLIT, OLAP_TILE_STEP, e->dd, it_ptr(e, e->olap_step), e->olap_step.n, e->dgroups,
                         e->arena_a, e->st));
     DEVCHK(ssa_dev_edge(e->prec, 1, 0, 1, e->dd, it_ptr(e, e->ser), e->ser.n, e->dgroups, e->dgroups + 4,
                         e->arena_a, e->small, e->st));

@@ -1463,16 +1463,17 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) vo
     if (item < 0) return;
     const ssa_item it = items[item];
     const ssa_desc d = D[it.s];
-    // item.x = group * S + split: split j of S sums components [ce j / S,
-    // ce (j+1) / S) of the group's list into its own staged piece (the
+    // item.x = group * S + split: split j of S sums entries [ce j / S,
+    // ce (j+1) / S) of the group's list (ce entries: the whole list, or with
+    // limit_gsize at most min(k, gsize)) into its own staged piece (the
     // overlap-add sums the pieces); S = 1 sums the whole list
     const int S = recon_splits(d, nsplit);
     const int4 gd = gdesc[it.x / S];
     const int a = it.y, sk = it.z, m = gd.z;
     int g0 = 0, g1 = gd.y;
     if (S > 1) {
-        int ce = min(gd.y, d.k);
-        if (limit_gsize) ce = min(ce, d.gsize);
+        int ce = gd.y;
+        if (limit_gsize) ce = min(min(ce, d.k), d.gsize);
         const int j = it.x % S;
         g0 = ce * j / S;
         g1 = ce * (j + 1) / S;
@@ -1676,28 +1677,53 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), (FftGeom<T, LOGN>::OCC)) vo
 }
 
 // overlap-add of the pieces of group gid and diagonal-averaging weights.
-// item (s, gid, t tile of 1024); weight 0: 1/c_t, 1: sqrt(1/c_t)
-template <typename T>
+// item (s, gid, t tile of `tile` samples); weight 0: 1/c_t, 1: sqrt(1/c_t).
+// A sample sums its pieces in (split, L chunk, K chunk) order.  ALL: the
+// loads of all pieces, eight at a time (clamped positions, masked
+// afterwards), for short launches, whose time is the latency of one item;
+// otherwise only the pieces that cover the sample are read (less traffic
+// when a launch fills the device).
+template <typename T, bool ALL>
 __global__ __launch_bounds__(256) void k_olap(const ssa_desc *D, const ssa_item *items, const int4 *gdesc, T *bulk,
-                                              int weight, int nsplit)
+                                              int weight, int nsplit, int tile)
 {
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
-    const int n = d.n, S = recon_splits(d, nsplit);
-    for (int t = it.y * 1024 + threadIdx.x; t < min(d.N, (it.y + 1) * 1024); t += 256) {
+    const int n = d.n, S = recon_splits(d, nsplit), AL = d.AL, SK = d.SK, np = S * AL * SK;
+    const T *stage = bulk + d.off_stage + (long long)it.x * S * AL * SK * n;
+    for (int t = it.y * tile + threadIdx.x; t < min(d.N, (it.y + 1) * tile); t += 256) {
         double s = 0;
-        for (int j = 0; j < S; j++) {
-            const long long piece = (long long)it.x * S + j;
-            for (int a = 0; a < d.AL; a++) {
-                const int ulen = min(d.CL, d.L - a * d.CL);
-                for (int sk = 0; sk < d.SK; sk++) {
-                    const int off = a * d.CL + sk * d.CK;
-                    const int vlen = min(d.CK, d.K - sk * d.CK);
-                    const int q = t - off;
-                    if (q >= 0 && q < ulen + vlen - 1)
-                        s += (double)bulk[d.off_stage + ((piece * d.AL + a) * d.SK + sk) * n + q];
+        if constexpr (ALL) {
+            int a = 0, sk = 0;   // chunks of piece p0 + r (the split is p / (AL SK))
+            for (int p0 = 0; p0 < np; p0 += 8) {
+                T v[8];
+                bool ok[8];
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    const int pc = min(p0 + r, np - 1);
+                    const int q = t - (a * d.CL + sk * d.CK);
+                    const int plen = min(d.CL, d.L - a * d.CL) + min(d.CK, d.K - sk * d.CK) - 1;
+                    ok[r] = p0 + r < np && q >= 0 && q < plen;
+                    v[r] = stage[(long long)pc * n + min(max(q, 0), n - 1)];
+                    if (++sk == SK) {
+                        sk = 0;
+                        if (++a == AL) a = 0;
+                    }
                 }
+#pragma unroll
+                for (int r = 0; r < 8; r++)
+                    if (ok[r]) s += (double)v[r];
             }
+        } else {
+            for (int j = 0; j < S; j++)
+                for (int a = 0; a < AL; a++) {
+                    const int ulen = min(d.CL, d.L - a * d.CL);
+                    for (int sk = 0; sk < SK; sk++) {
+                        const int q = t - (a * d.CL + sk * d.CK);
+                        const int vlen = min(d.CK, d.K - sk * d.CK);
+                        if (q >= 0 && q < ulen + vlen - 1) s += (double)stage[((long long)(j * AL + a) * SK + sk) * n + q];
+                    }
+                }
         }
         const int c = min(min(t + 1, d.L), min(d.K, d.N - t));
         const double w = 1.0 / (double)c;
@@ -1715,31 +1741,39 @@ __global__ __launch_bounds__(256) void k_olap(const ssa_desc *D, const ssa_item 
 // ends carry almost all of the error of an FP32 reconstruction.  There the sums
 // have at most SSA_EDGE terms each.  Item (s, gid); products and sums in FP64.
 // The group's components are staged EDGE_CG at a time: every load of a stage
-// is issued before its one barrier, so a group pays one memory latency per
-// EDGE_CG components, not one per component.  Each sample is summed in
-// component order, terms in index order, whatever the staging.
+// is issued before its barrier (index-clamped and masked, not under
+// branches), so a group pays one memory latency per EDGE_CG components.  The
+// EDGE_P thread groups of 2 SSA_EDGE threads each sum EDGE_CG / EDGE_P of a
+// stage's components (two chains side by side, terms read eight at a time at
+// constant LDS offsets); the first group then adds the stage's sums in
+// component order.  Each sample is summed in component order, terms in index
+// order, whatever the staging.
 // scale_sigma, weight, limit_gsize and the group layout as in k_recon/k_olap.
 // ---------------------------------------------------------------------------
-constexpr int EDGE_CG = 8;
+constexpr int EDGE_CG = 8, EDGE_P = 4, EDGE_THREADS = EDGE_P * 2 * SSA_EDGE;
+static_assert(EDGE_CG == 2 * EDGE_P, "each thread group sums two components of a stage");
 
 template <typename T>
-__global__ __launch_bounds__(2 * SSA_EDGE) void k_edge(const ssa_desc *__restrict__ D,
+__global__ __launch_bounds__(EDGE_THREADS) void k_edge(const ssa_desc *__restrict__ D,
                                                        const ssa_item *__restrict__ items,
                                                        const int4 *__restrict__ gdesc,
                                                        const int *__restrict__ glist, T *bulk,
                                                        const double *__restrict__ small, int scale_sigma,
                                                        int weight, int limit_gsize)
 {
-    constexpr int E = SSA_EDGE, CG = EDGE_CG;
+    constexpr int E = SSA_EDGE, CG = EDGE_CG, P = EDGE_P;
     __shared__ double uh[CG][E], vh[CG][E], ut[CG][E], vt[CG][E];
-    __shared__ int gl[2 * E];
+    __shared__ double ss[CG][2 * E];
+    __shared__ int gl[EDGE_THREADS];
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const int4 gd = gdesc[it.x];
     const int L = d.L, K = d.K, N = d.N, m = gd.z;
+    // thread group p sums components p and p + P of a stage; within a group,
     // thread j < E: head sample t = j; thread E + j: tail sample t = N - E + j
-    const int j = threadIdx.x & (E - 1);
-    const bool tail = threadIdx.x >= E;
+    const int p = threadIdx.x / (2 * E), tt = threadIdx.x % (2 * E);
+    const int j = tt & (E - 1);
+    const bool tail = tt >= E;
     const int t = tail ? N - E + j : j;
     // a sample both ranges contain (N < 2E) is written by the head thread
     const bool mine = t >= 0 && t < N && (!tail || t >= E);
@@ -1753,11 +1787,12 @@ __global__ __launch_bounds__(2 * SSA_EDGE) void k_edge(const ssa_desc *__restric
     const int urc = min(max(ur, 0), L - 1), vrc = min(max(vr, 0), K - 1);
     const T *ubase = bulk + d.off_U + (long long)m * L + urc;
     const T *vbase = bulk + d.off_V + vrc;
+    const int uo = tail ? L - E : 0, vo = tail ? K - E : 0, nt = hi - lo + 1;
     double acc = 0.0;
-    // the group's list in blocks of 2E entries: component indices in gl, -1
-    // for one that is skipped
-    for (int b0 = 0; b0 < gd.y; b0 += 2 * E) {
-        const int ng = min(2 * E, gd.y - b0);
+    // the group's list in blocks of EDGE_THREADS entries: component indices
+    // in gl, -1 for one that is skipped
+    for (int b0 = 0; b0 < gd.y; b0 += EDGE_THREADS) {
+        const int ng = min(EDGE_THREADS, gd.y - b0);
         __syncthreads();   // the previous block's indices are no longer read
         if ((int)threadIdx.x < ng) {
             const int idx = glist[gd.x + b0 + threadIdx.x];
@@ -1765,46 +1800,34 @@ __global__ __launch_bounds__(2 * SSA_EDGE) void k_edge(const ssa_desc *__restric
         }
         for (int g0 = 0; g0 < ng; g0 += CG) {
             __syncthreads();   // gl is written; the previous stage's values are no longer read
-            int cid[CG];
-            double a[CG], bv[CG], sc[CG];
-            // every load of the stage is issued before any of them is used
-#pragma unroll
-            for (int c = 0; c < CG; c++) {
-                cid[c] = g0 + c < ng ? gl[g0 + c] : -1;
-                const int ci = max(cid[c], 0);
-                a[c] = (double)ubase[(long long)ci * d.ML];
-                bv[c] = (double)vbase[(long long)ci * K];
-                sc[c] = scale_sigma ? sig[ci] : 1.0;
-            }
-#pragma unroll
-            for (int c = 0; c < CG; c++) {
-                const bool ok = cid[c] >= 0;
-                const double av = ok && uin ? a[c] : 0.0, vv = ok && vin ? bv[c] : 0.0;
+            // this thread's two components of the stage: c0 = p, c1 = p + P
+            const int c0 = p, c1 = p + P;
+            const int id0 = g0 + c0 < ng ? gl[g0 + c0] : -1, id1 = g0 + c1 < ng ? gl[g0 + c1] : -1;
+            {
+                const int i0 = max(id0, 0), i1 = max(id1, 0);
+                const double a0 = (double)ubase[(long long)i0 * d.ML], b0v = (double)vbase[(long long)i0 * K];
+                const double a1 = (double)ubase[(long long)i1 * d.ML], b1v = (double)vbase[(long long)i1 * K];
+                const double ua0 = id0 >= 0 && uin ? a0 : 0.0, va0 = id0 >= 0 && vin ? b0v : 0.0;
+                const double ua1 = id1 >= 0 && uin ? a1 : 0.0, va1 = id1 >= 0 && vin ? b1v : 0.0;
                 if (!tail) {
-                    uh[c][j] = av;
-                    vh[c][j] = vv;
+                    uh[c0][j] = ua0; vh[c0][j] = va0;
+                    uh[c1][j] = ua1; vh[c1][j] = va1;
                 } else {
-                    ut[c][j] = av;
-                    vt[c][j] = vv;
+                    ut[c0][j] = ua0; vt[c0][j] = va0;
+                    ut[c1][j] = ua1; vt[c1][j] = va1;
                 }
             }
             __syncthreads();
-            if (!mine) continue;
-            // head (t < E): both indices below E; tail (t >= N - E): i >= L - E
-            // and t - i >= K - E.  Terms i = lo..hi in order, 8 at a time so
-            // that their LDS loads are in flight together: term lo + ii reads
-            // up[ii] and vq[-ii] (constant offsets within a group of 8); the
-            // last, partial group reads clamped positions and adds only its
-            // terms.  Two components' sums run side by side (independent
-            // chains); each is added to acc in component order.
-            const int uo = tail ? L - E : 0, vo = tail ? K - E : 0, nt = hi - lo + 1;
-#pragma unroll
-            for (int c = 0; c < CG; c += 2) {
-                if (cid[c] < 0 && cid[c + 1] < 0) continue;
-                const double *up0 = (tail ? ut[c] : uh[c]) + (lo - uo);
-                const double *vq0 = (tail ? vt[c] : vh[c]) + (t - lo - vo);
-                const double *up1 = (tail ? ut[c + 1] : uh[c + 1]) + (lo - uo);
-                const double *vq1 = (tail ? vt[c + 1] : vh[c + 1]) + (t - lo - vo);
+            if (mine && (id0 >= 0 || id1 >= 0)) {
+                // head (t < E): both indices below E; tail (t >= N - E): i >= L - E
+                // and t - i >= K - E.  Terms i = lo..hi in order: term lo + ii
+                // reads up[ii] and vq[-ii] (constant offsets within a group of
+                // 8); the last, partial group reads clamped positions and adds
+                // only its terms.
+                const double *up0 = (tail ? ut[c0] : uh[c0]) + (lo - uo);
+                const double *vq0 = (tail ? vt[c0] : vh[c0]) + (t - lo - vo);
+                const double *up1 = (tail ? ut[c1] : uh[c1]) + (lo - uo);
+                const double *vq1 = (tail ? vt[c1] : vh[c1]) + (t - lo - vo);
                 double s0 = 0.0, s1 = 0.0;
                 int ii = 0;
                 for (; ii + 8 <= nt; ii += 8) {
@@ -1839,12 +1862,20 @@ __global__ __launch_bounds__(2 * SSA_EDGE) void k_edge(const ssa_desc *__restric
                             s1 = fma(x1[r], y1[r], s1);
                         }
                 }
-                if (cid[c] >= 0) acc = fma(sc[c], s0, acc);
-                if (cid[c + 1] >= 0) acc = fma(sc[c + 1], s1, acc);
+                ss[c0][tt] = s0;
+                ss[c1][tt] = s1;
+            }
+            __syncthreads();
+            if (p == 0 && mine) {
+#pragma unroll
+                for (int c = 0; c < CG; c++) {
+                    const int id = g0 + c < ng ? gl[g0 + c] : -1;
+                    if (id >= 0) acc = fma(scale_sigma ? sig[id] : 1.0, ss[c][tt], acc);
+                }
             }
         }
     }
-    if (!mine) return;
+    if (p != 0 || !mine) return;
     const int c = min(min(t + 1, L), min(K, N - t));
     const double w = 1.0 / (double)c;
     bulk[d.off_out + (long long)it.x * N + t] = (T)(acc * (weight ? sqrt(w) : w));
@@ -2344,16 +2375,25 @@ int ssa_dev_recon(int prec, int lgn, int scale_sigma, int limit_gsize, int nspli
     return (int)hipGetLastError();
 }
 
-int ssa_dev_olap(int prec, int weight, int nsplit, const ssa_desc *D, const ssa_item *items, int n_items,
-                 const int *gdesc, void *bulk, void *stream)
+int ssa_dev_olap(int prec, int weight, int nsplit, int tile, const ssa_desc *D, const ssa_item *items,
+                 int n_items, const int *gdesc, void *bulk, void *stream)
 {
     if (n_items <= 0) return 0;
-    if (prec == SSA_PREC_F64)
-        hipLaunchKernelGGL(k_olap<double>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
-                           (const int4 *)gdesc, (double *)bulk, weight, nsplit);
-    else
-        hipLaunchKernelGGL(k_olap<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
-                           (const int4 *)gdesc, (float *)bulk, weight, nsplit);
+    if (tile < 256 || tile % 256) return -1;
+    // a launch of few items reads every piece ahead (its time is one item's latency)
+    const bool all = n_items < 1024;
+    const hipStream_t st = (hipStream_t)stream;
+    if (prec == SSA_PREC_F64) {
+        if (all) hipLaunchKernelGGL((k_olap<double, true>), dim3(n_items), dim3(256), 0, st, D, items,
+                                    (const int4 *)gdesc, (double *)bulk, weight, nsplit, tile);
+        else hipLaunchKernelGGL((k_olap<double, false>), dim3(n_items), dim3(256), 0, st, D, items,
+                                (const int4 *)gdesc, (double *)bulk, weight, nsplit, tile);
+    } else {
+        if (all) hipLaunchKernelGGL((k_olap<float, true>), dim3(n_items), dim3(256), 0, st, D, items,
+                                    (const int4 *)gdesc, (float *)bulk, weight, nsplit, tile);
+        else hipLaunchKernelGGL((k_olap<float, false>), dim3(n_items), dim3(256), 0, st, D, items,
+                                (const int4 *)gdesc, (float *)bulk, weight, nsplit, tile);
+    }
     return (int)hipGetLastError();
 }
 
@@ -2405,10 +2445,10 @@ int ssa_dev_edge(int prec, int scale_sigma, int weight, int limit_gsize, const s
 {
     if (n_items <= 0) return 0;
     if (prec == SSA_PREC_F64)
-        hipLaunchKernelGGL(k_edge<double>, dim3(n_items), dim3(2 * SSA_EDGE), 0, (hipStream_t)stream, D, items,
+        hipLaunchKernelGGL(k_edge<double>, dim3(n_items), dim3(EDGE_THREADS), 0, (hipStream_t)stream, D, items,
                            (const int4 *)gdesc, glist, (double *)bulk, small, scale_sigma, weight, limit_gsize);
     else
-        hipLaunchKernelGGL(k_edge<float>, dim3(n_items), dim3(2 * SSA_EDGE), 0, (hipStream_t)stream, D, items,
+        hipLaunchKernelGGL(k_edge<float>, dim3(n_items), dim3(EDGE_THREADS), 0, (hipStream_t)stream, D, items,
                            (const int4 *)gdesc, glist, (float *)bulk, small, scale_sigma, weight, limit_gsize);
     return (int)hipGetLastError();
 }

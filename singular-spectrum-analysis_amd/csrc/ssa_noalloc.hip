@@ -121,7 +121,7 @@ extern "C" int ssa_amd_reconstruct_noalloc(const double *U, const double *V, con
     ssa_plan pl;
     if (ssa_plan_series(SSA_PREC_F64, N, L, 1, 2, k, -1, -1, &pl) != 0) return -1;
     const int K = N - L + 1;
-    const int n_rec = pl.AL * pl.SK, n_olap = (N + 1023) / 1024;
+    const int n_rec = pl.AL * pl.SK, n_olap = (N + SSA_OLAP_TILE - 1) / SSA_OLAP_TILE;
     // scratch layout (doubles): descriptor | items | group | staging
     const long long o_desc = 0;
     const long long o_items = align32(o_desc + (long long)(sizeof(ssa_desc) + 7) / 8);
@@ -166,7 +166,7 @@ extern "C" int ssa_amd_reconstruct_noalloc(const double *U, const double *V, con
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
     rc = ssa_dev_recon(SSA_PREC_F64, pl.lgn, 1, 0, 1, dd, items, n_rec, gdesc, glist, base, base, st.tw, stream);
-    if (rc == 0) rc = ssa_dev_olap(SSA_PREC_F64, 0, 1, dd, items + n_rec, n_olap, gdesc, base, stream);
+    if (rc == 0) rc = ssa_dev_olap(SSA_PREC_F64, 0, 1, SSA_OLAP_TILE, dd, items + n_rec, n_olap, gdesc, base, stream);
     if (rc == 0) rc = ssa_dev_edge(SSA_PREC_F64, 1, 0, 0, dd, items + n_rec + n_olap, 1, gdesc, glist, base, base,
                                    stream);
     return rc;
