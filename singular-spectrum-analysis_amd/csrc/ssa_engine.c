@@ -209,6 +209,13 @@ static int pin_reserve(ssa_engine *e, size_t bytes)
     return 0;
 }
 
+/* The captured step holds kernel arguments (item lists, arenas, list
+ * lengths, transform geometry): every change of those drops it. */
+static void graphs_drop(ssa_engine *e)
+{
+    if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+}
+
 static int items_push(ssa_engine *e, int s, int x, int y, int z)
 {
     if (e->hitems_n == e->hitems_cap) {
@@ -227,6 +234,7 @@ static int items_upload(ssa_engine *e, long long from)
 {
     long long n = e->hitems_n;
     if (n > e->ditems_cap) {
+        graphs_drop(e);   /* the lists move */
         if (e->ditems) hipFree(e->ditems);
         e->ditems = NULL;
         long long cap = n + n / 2 + 1024;
@@ -349,6 +357,7 @@ static int build_matvec(ssa_engine *e, int dir, int lgn, span *sp)
 
 static int rebuild_lists(ssa_engine *e)
 {
+    graphs_drop(e);   /* list lengths and offsets may change */
     e->hitems_n = 0;
     for (int lg = 0; lg < NLG; lg++) {
         e->spec[lg].off = e->hitems_n;
@@ -514,7 +523,7 @@ static int engine_layout(ssa_engine *e, int count, const int *N, const int *L, c
     e->lists_dirty = 1;
     e->desc_dirty = 1;
     e->step_groups_ready = 0;
-    if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+    graphs_drop(e);
     return 0;
 }
 
@@ -779,7 +788,7 @@ int ssa_engine_prepare(ssa_engine *e, int max_groups, int max_h)
     e->lists_dirty = 1;
     e->desc_dirty = 1;
     e->step_groups_ready = 0;
-    if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+    graphs_drop(e);
     return 0;
 }
 
@@ -826,7 +835,7 @@ int ssa_engine_set_bk(ssa_engine *e, int s, int b, int k)
         e->lists_dirty = 1;
         e->desc_dirty = 1;
         e->step_groups_ready = 0;
-        if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+        graphs_drop(e);
     }
     return 0;
 }
@@ -846,7 +855,7 @@ int ssa_engine_set_step_h(ssa_engine *e, int h)
     if (h < 0 || (h > 0 && (!e->prepared || h > e->max_h))) return -1;
     if (h != e->step_h) {
         e->step_h = h;
-        if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+        graphs_drop(e);
     }
     return 0;
 }
@@ -873,7 +882,7 @@ void ssa_engine_set_wide_min(ssa_engine *e, int n)
 {
     if (!e) return;
     e->mv_wide_min = n < 1 ? 1 : n;
-    if (e->has_graph) { hipGraphExecDestroy(e->gexec); e->has_graph = 0; }
+    graphs_drop(e);
 }
 
 int ssa_engine_set_rng(ssa_engine *e, int s, unsigned long long seed, unsigned long long call)
