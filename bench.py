@@ -31,6 +31,9 @@ ssa_amd = None   # loaded after the CPU baseline has forked its workers
 HBM_PEAK_GBS = 8000.0
 P_OVERSAMPLE = 10
 Q_POWER = 2
+# the reference's published decompose + reconstruct times (k = 50, p = 8,
+# L = N // 4, i9-14900KF, MKL 2025.0.3; its README.md:49-63), ms by N
+README_SHAPES_MS = {500: 2.0, 1000: 1.6, 2000: 2.5, 5000: 7.2, 10000: 12.2, 20000: 26.3}
 
 
 def algorithmic_bytes(N, L, k, b, s):
@@ -289,6 +292,15 @@ def _mkl_configs_child(threads):
         m5.decompose(15, P_OVERSAMPLE, 0, om5)
         m5.reconstruct(list(range(15)))
     out["ref_shape_N500_k15_q0_pipeline_p50_ms"], _ = timed(pipe, 200)
+    # the reference's published table: decompose (k = 50, p = 8, q = 0) +
+    # reconstruction of all 50, L = N // 4
+    for N in README_SHAPES_MS:
+        mN, omN = shape(N, N // 4, 50, 8, 0, 20 + N, 21)
+
+        def table_call(m_=mN, om_=omN):
+            m_.decompose(50, 8, 0, om_)
+            m_.reconstruct(list(range(50)))
+        out[f"published_table_N{N}_p50_ms"], _ = timed(table_call, 5)
     return out
 
 
@@ -595,6 +607,26 @@ def single_configs(reps=3):
             pipe.append(c - a)
         out[f"ref_shape_N500_k15_q{q}_decompose_p50_ms"] = round(1e3 * float(np.percentile(dec, 50)), 4)
         out[f"ref_shape_N500_k15_q{q}_pipeline_p50_ms"] = round(1e3 * float(np.percentile(pipe, 50)), 4)
+    # the reference's published single-series table (its README.md:49-63, method
+    # MKL/py/benchmark_vs_rssa.py:78-112): decompose_randomized(k = 50, p = 8) +
+    # reconstruct(all 50), L = N // 4, median of 5 calls on an i9-14900KF with
+    # MKL; here the same calls on one prepared plain FP64 context per N, q = 0
+    # (the reference's algorithm), median of 5 after 2 untimed calls
+    table = {}
+    for N in README_SHAPES_MS:
+        xs_ = workloads.single(N, seed=20 + N)
+        s = ssa_amd.SSA(xs_, N // 4, q=0, precision=0, seed=21)
+        g50 = list(range(50))
+        t = []
+        for r in range(7):
+            a = time.perf_counter()
+            s.decompose(50, oversampling=8)
+            s.reconstruct(g50)
+            if r >= 2:
+                t.append(time.perf_counter() - a)
+        table[str(N)] = {"ms": round(1e3 * float(np.median(t)), 3), "published_ms": README_SHAPES_MS[N]}
+        del s
+    out["published_table_q0_p8_k50"] = table
     # streaming ticks, the reference's low-latency pattern (MKL/ssa_opt_r2c.h:82-91):
     # one prepared context; per tick update_signal on the window moved by one
     # sample, decompose_randomized (k = 50, p = 10, q = 2) and the trend
