@@ -680,6 +680,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--count", type=int, default=4096)
+    ap.add_argument("--n-lo", type=int, default=500, help="shortest series of the batch (profiling other mixes)")
+    ap.add_argument("--n-hi", type=int, default=20000, help="longest series of the batch")
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -700,7 +702,8 @@ def main():
         wcorr_workload()
         return
 
-    xs, Ns, Ls, ks = workloads.ragged_batch(count=args.count, seed=4, dtype=np.float32)
+    xs, Ns, Ls, ks = workloads.ragged_batch(count=args.count, n_lo=args.n_lo, n_hi=args.n_hi, seed=4,
+                                            dtype=np.float32)
     print(f"[bench] batch: {len(xs)} series, sum N = {int(Ns.sum())}", flush=True)
 
     # CPU baseline first: its worker processes are forked before any GPU call
@@ -788,7 +791,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"ragged batch: {len(xs)} series, N log-uniform in [500, 20000], L = N // 4, "
+        "config": {"workload": f"ragged batch: {len(xs)} series, N log-uniform in [{args.n_lo}, {args.n_hi}], L = N // 4, "
                                f"k ~ U{{10..50}}, p = 10, q = 2, FP32 inputs; one step = spectra + "
                                f"decomposition + reconstruction of all k components of every series",
                    "series": len(xs), "sum_N": int(Ns.sum()), "form": "mixed (FP32 bulk, FP64 small solves)"},
