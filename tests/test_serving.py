@@ -128,12 +128,17 @@ def test_rebind_matches_fresh_batch():
 
 def test_series_bytes_match_the_batch():
     """ssa_amd.series_bytes (host arithmetic) is exactly what a batch's arenas
-    use, series by series."""
+    use, series by series, including the spectrum scratch of series whose
+    block matvec takes the staged form (chunked on both sides, N up to 50000)."""
     import ssa_amd as S
     import workloads
-    xs, Ns, Ls, ks = workloads.ragged_batch(count=12, n_lo=500, n_hi=20000, seed=61, dtype=np.float32)
+    xs, Ns, Ls, ks = workloads.ragged_batch(count=10, n_lo=500, n_hi=20000, seed=61, dtype=np.float32)
+    xl, Nl, Ll, kl = workloads.ragged_batch(count=4, n_lo=30000, n_hi=50000, seed=62, dtype=np.float32)
+    xs, Ls, ks = list(xs) + list(xl), np.concatenate([Ls, Ll]), np.concatenate([ks, kl])
     for prec in (0, 1):
         b = S.Batch(xs, Ls, ks, p=10, q=2, precision=prec, seed=1, h=50)
+        staged = [b.plan(i) for i in range(len(xs))]
+        assert any(pl["AL"] > 1 and pl["SK"] > 1 for pl in staged), prec
         est = sum(S.series_bytes(len(x), int(L), int(k), 10, 2, prec, 50) for x, L, k in zip(xs, Ls, ks))
         assert b.bytes() == est, prec
 
