@@ -89,6 +89,7 @@ struct ssa_engine {
     int cholqr2_all;           /* CholeskyQR2 for every orthonormalisation (default: last one only) */
     int final2;                /* CholeskyQR2 for the last orthonormalisation of Y and for B */
     int mv_wide_min;           /* block-matvec launches of at least this many items take the wide geometry */
+    int mv_stage;              /* the staged block matvec is used (SSA_AMD_MV_STAGED=0: the fused form only) */
     double dead_tol;           /* Cholesky pivot / original diagonal at or below which a column is dependent */
     /* component spectra cache (ssa_engine_cache_build): its own allocation,
      * addressed from arena_a like the panel arena */
@@ -311,7 +312,7 @@ static int build_staged(ssa_engine *e, int dir, int lgn)
     sp->off = e->hitems_n;
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
-        if (d->lgn != lgn || d->b <= 0 || !mv_staged(d, dir)) continue;
+        if (d->lgn != lgn || d->b <= 0 || !(e->mv_stage && mv_staged(d, dir))) continue;
         const int n_in = dir == 0 ? d->SK : d->M * d->AL;
         for (int p = 0; p < (d->b + 1) / 2; p++)
             for (int c = 0; c < n_in; c++)
@@ -321,7 +322,7 @@ static int build_staged(ssa_engine *e, int dir, int lgn)
     sa->off = e->hitems_n;
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
-        if (d->lgn != lgn || d->b <= 0 || !mv_staged(d, dir)) continue;
+        if (d->lgn != lgn || d->b <= 0 || !(e->mv_stage && mv_staged(d, dir))) continue;
         const int n_out = dir == 0 ? d->M * d->AL : d->SK;
         for (int p = 0; p < (d->b + 1) / 2; p++)
             for (int o = 0; o < n_out; o++)
@@ -336,13 +337,13 @@ static int build_matvec(ssa_engine *e, int dir, int lgn, span *sp)
     long long units = 0;   /* (pair, output chunk) units of the launch */
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
-        if (d->lgn != lgn || d->b <= 0 || mv_staged(d, dir)) continue;
+        if (d->lgn != lgn || d->b <= 0 || (e->mv_stage && mv_staged(d, dir))) continue;
         units += (long long)((d->b + 1) / 2) * (dir == 0 ? d->M * d->AL : d->SK);
     }
     sp->off = e->hitems_n;
     for (int s = 0; s < e->count; s++) {
         const ssa_desc *d = &e->hd[s];
-        if (d->lgn != lgn || d->b <= 0 || mv_staged(d, dir)) continue;
+        if (d->lgn != lgn || d->b <= 0 || (e->mv_stage && mv_staged(d, dir))) continue;
         int nLc = d->M * d->AL;
         int n_in = dir == 0 ? d->SK : nLc, n_out = dir == 0 ? nLc : d->SK;
         long long g = n_in == 1 ? units / MV_MIN_ITEMS : 1;
@@ -571,6 +572,10 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
         const char *w = getenv("SSA_AMD_MV_WIDE_MIN");
         e->mv_wide_min = (w && *w) ? atoi(w) : MV_WIDE_MIN_ITEMS;
         if (e->mv_wide_min < 1) e->mv_wide_min = 1;
+        /* the staged form sums the same products in the same order as the
+         * fused one; SSA_AMD_MV_STAGED=0 keeps the fused form (comparison) */
+        const char *ms = getenv("SSA_AMD_MV_STAGED");
+        e->mv_stage = !(ms && *ms == '0');
     }
     *err = 0;
     return e;

@@ -297,6 +297,10 @@ def test_staged_block_matvec(precision, N, L, k, tol):
     x, dev, orc = _pair(N, L, k, 10, 2, seed=N + precision, precision=precision)
     pl = dev.plan_info()
     assert pl["AL"] > 1 and pl["SK"] > 1, pl
+    # the fused form on the same series and test matrix gives the same bits
+    _, fused, _ = _fused(lambda: _pair(N, L, k, 10, 2, seed=N + precision, precision=precision))
+    assert np.array_equal(dev.singular_values, fused.singular_values)
+    assert np.array_equal(dev.reconstruct(list(range(k))), fused.reconstruct(list(range(k))))
     so = orc.sigma
     assert np.max(np.abs(dev.singular_values - so) / so) < (1e-10 if precision == 0 else 1e-5)
     a, b = dev.reconstruct(list(range(k))), orc.reconstruct(list(range(k)))
@@ -335,6 +339,53 @@ def test_nonfinite_input_leaves_context_usable(precision):
     fresh.decompose(k, oversampling=p)
     assert np.array_equal(used.singular_values, fresh.singular_values)
     assert np.array_equal(used.reconstruct(list(range(k))), fresh.reconstruct(list(range(k))))
+
+
+def _fused(make):
+    """make() with the staged block matvec switched off (the fused form)."""
+    import os
+    old = os.environ.get("SSA_AMD_MV_STAGED")
+    os.environ["SSA_AMD_MV_STAGED"] = "0"
+    try:
+        return make()
+    finally:
+        if old is None:
+            del os.environ["SSA_AMD_MV_STAGED"]
+        else:
+            os.environ["SSA_AMD_MV_STAGED"] = old
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_staged_block_matvec_mssa(precision):
+    """MSSA with the staged block matvec in both directions: M = 3 channels
+    of N = 12000, L = 3000 stack to 3 row chunks against 2 column chunks at
+    n = 8192.  The staged form sums the same products in the same order as
+    the fused form, so both give the same bits (one context, the same
+    transform geometry).  Against the FP64 oracle: the groups {0, 1},
+    {0, 1, 2, 3} and all k within 1e-9 of each channel's RMS (plain); in the
+    mixed form {0, 1, 2, 3} and all k within 1e-5 relative 2-norm.  The group
+    {0, 1} is about 25x more sensitive than the others here (its plain error
+    is 3.5e-13 relative against 1.3e-14) and reaches 1.4e-5 in the mixed form
+    with either matvec; a group splitting the near-equal sigma_2, sigma_3
+    (5e-5 apart) is worse still."""
+    S = _ssa()
+    X = workloads.mssa_channels(M=3, N=12000, seed=9)
+    L, k, p, q = 3000, 12, 10, 2
+    m = S.MSSA(X, L, q=q, precision=precision)
+    m.decompose(k, oversampling=p)
+    f = _fused(lambda: S.MSSA(X, L, q=q, precision=precision))
+    f.decompose(k, oversampling=p)
+    K = 12000 - L + 1
+    o = O.OracleMSSA(X, L).decompose(k, p, q, O.philox_gaussian(42, 0, K, k + p))
+    groups = ([0, 1], [0, 1, 2, 3], list(range(k))) if precision == 0 else ([0, 1, 2, 3], list(range(k)))
+    for ch in range(3):
+        for g in groups:
+            a, b = m.reconstruct(ch, g), o.reconstruct(ch, g)
+            assert np.array_equal(a, f.reconstruct(ch, g)), (ch, g)
+            if precision == 0:
+                assert np.abs(a - b).max() / rms(X[ch]) < 1e-9, (ch, g)
+            else:
+                assert rel2(a, b) < 1e-5, (ch, g)
 
 
 def _wcorr_np(rec, L):
