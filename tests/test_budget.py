@@ -46,3 +46,21 @@ def test_waves_respect_the_budget():
     big = max(reqs, key=tiny.footprint)
     wave, rest, refused = tiny.next_wave([big] + [r for r in reqs if r is not big])
     assert (refused and refused[0] is big) or (len(wave) == 1 and wave[0] is big)
+
+
+def test_wave_cap_shares_the_queue_over_slots():
+    """Without min_wave a wave takes up to max_wave requests; with it, an
+    equal share of the queue over the in-flight slots, between min_wave and
+    max_wave, and a backlog still fills waves of max_wave."""
+    import serve
+    plain = serve.Server(max_wave=512, inflight=4)
+    assert [plain.wave_cap(n) for n in (1, 100, 5000)] == [512, 512, 512]
+    shared = serve.Server(max_wave=512, inflight=4, min_wave=64)
+    assert [shared.wave_cap(n) for n in (1, 100, 256, 300, 1000, 5000)] == [64, 64, 64, 75, 250, 512]
+    reqs = serve.synthetic_requests(300, rate=np.inf, n_lo=500, n_hi=2000, seed=9)
+    wave, rest, refused = shared.next_wave(list(reqs))
+    assert len(wave) == 75 and len(rest) == 225 and not refused
+    assert [r.rid for r in wave + rest] == [r.rid for r in reqs]
+    # the warm-up grows every slot to a full wave
+    w, _, _ = shared.next_wave(list(reqs), cap=shared.max_wave)
+    assert len(w) == 300
