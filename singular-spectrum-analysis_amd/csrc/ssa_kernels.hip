@@ -1743,34 +1743,36 @@ __global__ __launch_bounds__(256) void k_olap(const ssa_desc *D, const ssa_item 
 // The group's components are staged EDGE_CG at a time: every load of a stage
 // is issued before its barrier (index-clamped and masked, not under
 // branches), so a group pays one memory latency per EDGE_CG components.  The
-// EDGE_P thread groups of 2 SSA_EDGE threads each sum EDGE_CG / EDGE_P of a
-// stage's components (two chains side by side, terms read eight at a time at
+// P thread groups of 2 SSA_EDGE threads each sum EDGE_CG / P of a stage's
+// components (two chains side by side, terms read eight at a time at
 // constant LDS offsets); the first group then adds the stage's sums in
 // component order.  Each sample is summed in component order, terms in index
-// order, whatever the staging.
+// order, whatever the staging and P: P = 4 for a launch of few items (one
+// series: its time is one item's latency), P = 1 for a batch (more items per
+// CU).
 // scale_sigma, weight, limit_gsize and the group layout as in k_recon/k_olap.
 // ---------------------------------------------------------------------------
-constexpr int EDGE_CG = 8, EDGE_P = 4, EDGE_THREADS = EDGE_P * 2 * SSA_EDGE;
-static_assert(EDGE_CG == 2 * EDGE_P, "each thread group sums two components of a stage");
+constexpr int EDGE_CG = 8;
 
-template <typename T>
-__global__ __launch_bounds__(EDGE_THREADS) void k_edge(const ssa_desc *__restrict__ D,
-                                                       const ssa_item *__restrict__ items,
-                                                       const int4 *__restrict__ gdesc,
-                                                       const int *__restrict__ glist, T *bulk,
-                                                       const double *__restrict__ small, int scale_sigma,
-                                                       int weight, int limit_gsize)
+template <typename T, int P>
+__global__ __launch_bounds__(P * 2 * SSA_EDGE) void k_edge(const ssa_desc *__restrict__ D,
+                                                           const ssa_item *__restrict__ items,
+                                                           const int4 *__restrict__ gdesc,
+                                                           const int *__restrict__ glist, T *bulk,
+                                                           const double *__restrict__ small, int scale_sigma,
+                                                           int weight, int limit_gsize)
 {
-    constexpr int E = SSA_EDGE, CG = EDGE_CG, P = EDGE_P;
+    constexpr int E = SSA_EDGE, CG = EDGE_CG, NTH = P * 2 * SSA_EDGE, CPG = CG / P;
+    static_assert(CG % (2 * P) == 0, "each thread group sums whole pairs of a stage's components");
     __shared__ double uh[CG][E], vh[CG][E], ut[CG][E], vt[CG][E];
     __shared__ double ss[CG][2 * E];
-    __shared__ int gl[EDGE_THREADS];
+    __shared__ int gl[NTH];
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
     const int4 gd = gdesc[it.x];
     const int L = d.L, K = d.K, N = d.N, m = gd.z;
-    // thread group p sums components p and p + P of a stage; within a group,
-    // thread j < E: head sample t = j; thread E + j: tail sample t = N - E + j
+    // thread group p sums components p + P c, c < CPG, of a stage; within a
+    // group, thread j < E: head sample t = j; thread E + j: tail sample t = N - E + j
     const int p = threadIdx.x / (2 * E), tt = threadIdx.x % (2 * E);
     const int j = tt & (E - 1);
     const bool tail = tt >= E;
@@ -1789,10 +1791,10 @@ __global__ __launch_bounds__(EDGE_THREADS) void k_edge(const ssa_desc *__restric
     const T *vbase = bulk + d.off_V + vrc;
     const int uo = tail ? L - E : 0, vo = tail ? K - E : 0, nt = hi - lo + 1;
     double acc = 0.0;
-    // the group's list in blocks of EDGE_THREADS entries: component indices
-    // in gl, -1 for one that is skipped
-    for (int b0 = 0; b0 < gd.y; b0 += EDGE_THREADS) {
-        const int ng = min(EDGE_THREADS, gd.y - b0);
+    // the group's list in blocks of NTH entries: component indices in gl, -1
+    // for one that is skipped
+    for (int b0 = 0; b0 < gd.y; b0 += NTH) {
+        const int ng = min(NTH, gd.y - b0);
         __syncthreads();   // the previous block's indices are no longer read
         if ((int)threadIdx.x < ng) {
             const int idx = glist[gd.x + b0 + threadIdx.x];
@@ -1800,77 +1802,90 @@ __global__ __launch_bounds__(EDGE_THREADS) void k_edge(const ssa_desc *__restric
         }
         for (int g0 = 0; g0 < ng; g0 += CG) {
             __syncthreads();   // gl is written; the previous stage's values are no longer read
-            // this thread's two components of the stage: c0 = p, c1 = p + P
-            const int c0 = p, c1 = p + P;
-            const int id0 = g0 + c0 < ng ? gl[g0 + c0] : -1, id1 = g0 + c1 < ng ? gl[g0 + c1] : -1;
+            // this thread's components of the stage: cs[c] = p + P c
+            int id[CPG];
             {
-                const int i0 = max(id0, 0), i1 = max(id1, 0);
-                const double a0 = (double)ubase[(long long)i0 * d.ML], b0v = (double)vbase[(long long)i0 * K];
-                const double a1 = (double)ubase[(long long)i1 * d.ML], b1v = (double)vbase[(long long)i1 * K];
-                const double ua0 = id0 >= 0 && uin ? a0 : 0.0, va0 = id0 >= 0 && vin ? b0v : 0.0;
-                const double ua1 = id1 >= 0 && uin ? a1 : 0.0, va1 = id1 >= 0 && vin ? b1v : 0.0;
-                if (!tail) {
-                    uh[c0][j] = ua0; vh[c0][j] = va0;
-                    uh[c1][j] = ua1; vh[c1][j] = va1;
-                } else {
-                    ut[c0][j] = ua0; vt[c0][j] = va0;
-                    ut[c1][j] = ua1; vt[c1][j] = va1;
+                double a[CPG], bv[CPG];
+#pragma unroll
+                for (int c = 0; c < CPG; c++) {
+                    const int cc = p + P * c;
+                    id[c] = g0 + cc < ng ? gl[g0 + cc] : -1;
+                    const int ci = max(id[c], 0);
+                    a[c] = (double)ubase[(long long)ci * d.ML];
+                    bv[c] = (double)vbase[(long long)ci * K];
+                }
+#pragma unroll
+                for (int c = 0; c < CPG; c++) {
+                    const int cc = p + P * c;
+                    const double ua = id[c] >= 0 && uin ? a[c] : 0.0, va = id[c] >= 0 && vin ? bv[c] : 0.0;
+                    if (!tail) {
+                        uh[cc][j] = ua;
+                        vh[cc][j] = va;
+                    } else {
+                        ut[cc][j] = ua;
+                        vt[cc][j] = va;
+                    }
                 }
             }
             __syncthreads();
-            if (mine && (id0 >= 0 || id1 >= 0)) {
+            if (mine) {
                 // head (t < E): both indices below E; tail (t >= N - E): i >= L - E
                 // and t - i >= K - E.  Terms i = lo..hi in order: term lo + ii
                 // reads up[ii] and vq[-ii] (constant offsets within a group of
                 // 8); the last, partial group reads clamped positions and adds
                 // only its terms.
-                const double *up0 = (tail ? ut[c0] : uh[c0]) + (lo - uo);
-                const double *vq0 = (tail ? vt[c0] : vh[c0]) + (t - lo - vo);
-                const double *up1 = (tail ? ut[c1] : uh[c1]) + (lo - uo);
-                const double *vq1 = (tail ? vt[c1] : vh[c1]) + (t - lo - vo);
-                double s0 = 0.0, s1 = 0.0;
-                int ii = 0;
-                for (; ii + 8 <= nt; ii += 8) {
-                    double x0[8], y0[8], x1[8], y1[8];
 #pragma unroll
-                    for (int r = 0; r < 8; r++) {
-                        x0[r] = up0[ii + r];
-                        y0[r] = (vq0 - ii - 7)[7 - r];
-                        x1[r] = up1[ii + r];
-                        y1[r] = (vq1 - ii - 7)[7 - r];
-                    }
+                for (int c = 0; c < CPG; c += 2) {
+                    if (id[c] < 0 && id[c + 1] < 0) continue;
+                    const int c0 = p + P * c, c1 = p + P * (c + 1);
+                    const double *up0 = (tail ? ut[c0] : uh[c0]) + (lo - uo);
+                    const double *vq0 = (tail ? vt[c0] : vh[c0]) + (t - lo - vo);
+                    const double *up1 = (tail ? ut[c1] : uh[c1]) + (lo - uo);
+                    const double *vq1 = (tail ? vt[c1] : vh[c1]) + (t - lo - vo);
+                    double s0 = 0.0, s1 = 0.0;
+                    int ii = 0;
+                    for (; ii + 8 <= nt; ii += 8) {
+                        double x0[8], y0[8], x1[8], y1[8];
 #pragma unroll
-                    for (int r = 0; r < 8; r++) {
-                        s0 = fma(x0[r], y0[r], s0);
-                        s1 = fma(x1[r], y1[r], s1);
-                    }
-                }
-                if (ii < nt) {
-                    double x0[8], y0[8], x1[8], y1[8];
+                        for (int r = 0; r < 8; r++) {
+                            x0[r] = up0[ii + r];
+                            y0[r] = (vq0 - ii - 7)[7 - r];
+                            x1[r] = up1[ii + r];
+                            y1[r] = (vq1 - ii - 7)[7 - r];
+                        }
 #pragma unroll
-                    for (int r = 0; r < 8; r++) {
-                        const int q = min(ii + r, nt - 1);
-                        x0[r] = up0[q];
-                        y0[r] = vq0[-q];
-                        x1[r] = up1[q];
-                        y1[r] = vq1[-q];
-                    }
-#pragma unroll
-                    for (int r = 0; r < 8; r++)
-                        if (ii + r < nt) {
+                        for (int r = 0; r < 8; r++) {
                             s0 = fma(x0[r], y0[r], s0);
                             s1 = fma(x1[r], y1[r], s1);
                         }
+                    }
+                    if (ii < nt) {
+                        double x0[8], y0[8], x1[8], y1[8];
+#pragma unroll
+                        for (int r = 0; r < 8; r++) {
+                            const int q = min(ii + r, nt - 1);
+                            x0[r] = up0[q];
+                            y0[r] = vq0[-q];
+                            x1[r] = up1[q];
+                            y1[r] = vq1[-q];
+                        }
+#pragma unroll
+                        for (int r = 0; r < 8; r++)
+                            if (ii + r < nt) {
+                                s0 = fma(x0[r], y0[r], s0);
+                                s1 = fma(x1[r], y1[r], s1);
+                            }
+                    }
+                    ss[c0][tt] = s0;
+                    ss[c1][tt] = s1;
                 }
-                ss[c0][tt] = s0;
-                ss[c1][tt] = s1;
             }
             __syncthreads();
             if (p == 0 && mine) {
 #pragma unroll
                 for (int c = 0; c < CG; c++) {
-                    const int id = g0 + c < ng ? gl[g0 + c] : -1;
-                    if (id >= 0) acc = fma(scale_sigma ? sig[id] : 1.0, ss[c][tt], acc);
+                    const int idc = g0 + c < ng ? gl[g0 + c] : -1;
+                    if (idc >= 0) acc = fma(scale_sigma ? sig[idc] : 1.0, ss[c][tt], acc);
                 }
             }
         }
@@ -2444,12 +2459,20 @@ int ssa_dev_edge(int prec, int scale_sigma, int weight, int limit_gsize, const s
                  int n_items, const int *gdesc, const int *glist, void *bulk, const double *small, void *stream)
 {
     if (n_items <= 0) return 0;
-    if (prec == SSA_PREC_F64)
-        hipLaunchKernelGGL(k_edge<double>, dim3(n_items), dim3(EDGE_THREADS), 0, (hipStream_t)stream, D, items,
-                           (const int4 *)gdesc, glist, (double *)bulk, small, scale_sigma, weight, limit_gsize);
-    else
-        hipLaunchKernelGGL(k_edge<float>, dim3(n_items), dim3(EDGE_THREADS), 0, (hipStream_t)stream, D, items,
-                           (const int4 *)gdesc, glist, (float *)bulk, small, scale_sigma, weight, limit_gsize);
+    const hipStream_t st = (hipStream_t)stream;
+    // a launch of few items spreads each item's components over four thread groups
+#define EDGE_LAUNCH(T, P)                                                                                     \
+    hipLaunchKernelGGL((k_edge<T, P>), dim3(n_items), dim3(P * 2 * SSA_EDGE), 0, st, D, items,                \
+                       (const int4 *)gdesc, glist, (T *)bulk, small, scale_sigma, weight, limit_gsize)
+    const bool few = n_items < 256;
+    if (prec == SSA_PREC_F64) {
+        if (few) EDGE_LAUNCH(double, 4);
+        else EDGE_LAUNCH(double, 1);
+    } else {
+        if (few) EDGE_LAUNCH(float, 4);
+        else EDGE_LAUNCH(float, 1);
+    }
+#undef EDGE_LAUNCH
     return (int)hipGetLastError();
 }
 
