@@ -591,10 +591,14 @@ def single_configs(reps=3):
     # k = 15, randomized decompose ~210 us, decompose + reconstruct ~250 us on an
     # i9-14900KF; its harness MKL/test/ssa_block_test.c:426-436 uses L = 375, p = 10):
     # the same calls on one prepared plain FP64 context, p50 over 200 calls, at
-    # q = 0 (the reference's algorithm) and q = 2 (this library's default)
+    # q = 0 (the reference's algorithm) and q = 2 (this library's default); and
+    # the mixed form (FP32 bulk) at q = 0, with its largest relative sigma
+    # difference from the plain form's
     x5 = workloads.single(500, seed=15)
-    for q in (0, 2):
-        s = ssa_amd.SSA(x5, 375, q=q, precision=0, seed=15)
+    sv_plain = None
+    for q, prec in ((0, 0), (2, 0), (0, 1)):
+        tag = f"q{q}" + ("_mixed" if prec else "")
+        s = ssa_amd.SSA(x5, 375, q=q, precision=prec, seed=15)
         s.decompose(15, oversampling=P_OVERSAMPLE)
         dec, pipe = [], []
         for r in range(200):
@@ -605,8 +609,12 @@ def single_configs(reps=3):
             c = time.perf_counter()
             dec.append(b_ - a)
             pipe.append(c - a)
-        out[f"ref_shape_N500_k15_q{q}_decompose_p50_ms"] = round(1e3 * float(np.percentile(dec, 50)), 4)
-        out[f"ref_shape_N500_k15_q{q}_pipeline_p50_ms"] = round(1e3 * float(np.percentile(pipe, 50)), 4)
+        out[f"ref_shape_N500_k15_{tag}_decompose_p50_ms"] = round(1e3 * float(np.percentile(dec, 50)), 4)
+        out[f"ref_shape_N500_k15_{tag}_pipeline_p50_ms"] = round(1e3 * float(np.percentile(pipe, 50)), 4)
+        if q == 0 and not prec:
+            sv_plain = s.singular_values.copy()
+        elif prec:
+            out["ref_shape_N500_k15_q0_mixed_sigma_rel_err"] = float(np.max(np.abs(s.singular_values - sv_plain) / sv_plain[0]))
     # the reference's published single-series table (its README.md:49-63, method
     # MKL/py/benchmark_vs_rssa.py:78-112): decompose_randomized(k = 50, p = 8) +
     # reconstruct(all 50), L = N // 4, median of 5 calls on an i9-14900KF with

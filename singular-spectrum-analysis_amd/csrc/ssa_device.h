@@ -118,8 +118,10 @@ int ssa_dev_mv_spec(int prec, int lgn, int dir, int in_panel, int wide, const ss
                     int n_items, void *bulk, const void *tw, void *stream);
 int ssa_dev_mv_acc(int prec, int lgn, int dir, int out_panel, int wide, const ssa_desc *D, const ssa_item *items,
                    int n_items, void *bulk, const void *tw, void *stream);
-/* Philox Gaussian test matrix into panel Z; items (s, row tile) of the K-dim panel */
-int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, void *stream);
+/* Philox Gaussian test matrix into panel Z; items (s, row tile) of the K-dim panel.
+ * call >= 0 replaces every series' descriptor counter (call) for this launch */
+int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, long long call,
+                  void *stream);
 /* Gram / apply: every panel of the launch has at most nb*16 columns (1 <= nb <= 5) */
 int ssa_dev_gram(int prec, int panel, int use_m, int nb, const ssa_desc *D, const ssa_item *items, int n_items,
                  void *bulk, double *small, void *stream);
@@ -161,6 +163,11 @@ int ssa_dev_cache_build(int prec, int lgn, const ssa_desc *D, const ssa_item *it
 int ssa_dev_recon_cached(int prec, int lgn, int scale_sigma, const ssa_desc *D, const ssa_item *items, int n_items,
                          int n_pieces, const int *gdesc, const int *glist, void *bulk, const double *small,
                          const void *tw, void *stream);
+/* misc_n FP64 values from misc, then un panel elements from U and vn from V,
+ * packed in that order into out (8 misc_n + (un + vn) element bytes; out may
+ * be the device address of page-locked host memory) */
+int ssa_dev_pack_results(int prec, const double *misc, int misc_n, const void *U, long long un, const void *V,
+                         long long vn, void *out, void *stream);
 /* copy every series' step output (N + h values at off_out) to its off_gather slot; items (s) */
 int ssa_dev_gather(int prec, int h, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, void *stream);
 /* the first and last SSA_EDGE samples of every group's reconstruction by direct

@@ -19,6 +19,7 @@
 #include "ssa_engine.h"
 
 #include <hip/hip_runtime_api.h>
+#include <limits.h>
 #include <math.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -29,6 +30,7 @@
 #define NLG 14                 /* lgn buckets 0..13 (6..13 used) */
 #define NBW 6                  /* Gram / apply lists by column blocks nb = 1..5 */
 #define OLAP_TILE_STEP 1024   /* overlap-add samples per work item in the batched step */
+#define RESULTS_DIRECT_MAX (256 << 10)   /* result bytes up to which the device writes them to the host */
 
 typedef struct { long long off; int n; } span;
 
@@ -58,6 +60,12 @@ struct ssa_engine {
     span stgS[2][NLG], stgA[2][NLG];   /* staged block matvec (by direction): spectra, sums */
     span rec_step[NLG], olap_step;
     int lists_dirty, desc_dirty;
+    /* >= 0: the test-matrix counter of a one-series engine whose descriptor
+     * is otherwise unchanged since its last upload.  An eager decomposition
+     * passes it to the test-matrix kernel instead of uploading the
+     * descriptor again (a host-to-device copy per call, whose latency varies
+     * with the copy engine's state); a batched step uploads it. */
+    long long omega_call;
     int dyn_base;              /* first item of the per-call region */
     /* groups (int4 descriptors followed by component lists) */
     int *dgroups;
@@ -73,6 +81,10 @@ struct ssa_engine {
     /* pinned host staging */
     void *hpin;
     size_t hpin_bytes;
+    /* page-locked, coherent host buffer that the device writes a small
+     * decomposition's results into (ssa_engine_results), and its device address */
+    void *hres, *hres_dev;
+    size_t hres_bytes;
     /* pinned staging of the descriptor upload, which is not waited for: the
      * event marks when the copy has read it, before the next upload refills it */
     void *dpin;
@@ -198,6 +210,20 @@ static long long align64(long long v) { return (v + 63) & ~63LL; }
  * segments then never straddle two lines */
 static int panel_ld(int rows) { return (rows + SSA_PANEL_ALIGN - 1) / SSA_PANEL_ALIGN * SSA_PANEL_ALIGN; }
 
+
+/* the coherent page-locked buffer the device writes small results into */
+static int hres_reserve(ssa_engine *e, size_t bytes)
+{
+    if (e->hres_bytes >= bytes) return 0;
+    if (e->hres) HIPCHK(hipHostFree(e->hres));
+    e->hres = e->hres_dev = NULL;
+    e->hres_bytes = 0;
+    const size_t nb = bytes < (64 << 10) ? (64 << 10) : bytes;
+    HIPCHK(hipHostMalloc(&e->hres, nb, hipHostMallocCoherent));
+    e->hres_bytes = nb;
+    HIPCHK(hipHostGetDevicePointer(&e->hres_dev, e->hres, 0));
+    return 0;
+}
 
 static int pin_reserve(ssa_engine *e, size_t bytes)
 {
@@ -523,6 +549,7 @@ static int engine_layout(ssa_engine *e, int count, const int *N, const int *L, c
     e->step_h = 0;
     e->lists_dirty = 1;
     e->desc_dirty = 1;
+    e->omega_call = -1;
     e->step_groups_ready = 0;
     graphs_drop(e);
     return 0;
@@ -541,6 +568,7 @@ ssa_engine *ssa_engine_create(int prec, int count, const int *N, const int *L, c
     e->esz = prec == SSA_PREC_F64 ? 8 : 4;
     e->q_hint = q_hint;
     e->dyn_last_n = e->grp_last_n = -1;
+    e->omega_call = -1;
     if (hipStreamCreate(&e->st) != hipSuccess) goto fail;
     if (ssa_dev_twiddles(prec, &e->tw, e->st) != 0) goto fail;
     if (engine_layout(e, count, N, L, M, bmax, kmax)) goto fail;
@@ -607,6 +635,7 @@ void ssa_engine_destroy(ssa_engine *e)
     free(e->dyn_last);
     free(e->grp_last);
     if (e->hpin) hipHostFree(e->hpin);
+    if (e->hres) hipHostFree(e->hres);
     if (e->dpin) hipHostFree(e->dpin);
     if (e->dpin_ev) hipEventDestroy(e->dpin_ev);
     for (int i = 0; i < 2 * e->prof_cap; i++) hipEventDestroy(e->prof_ev[i]);
@@ -893,9 +922,14 @@ void ssa_engine_set_wide_min(ssa_engine *e, int n)
 int ssa_engine_set_rng(ssa_engine *e, int s, unsigned long long seed, unsigned long long call)
 {
     if (s < 0 || s >= e->count) return -1;
-    e->hd[s].seed = seed;
     e->hd[s].call = call;
+    if (e->count == 1 && !e->desc_dirty && e->hd[s].seed == seed && call <= (unsigned long long)LLONG_MAX) {
+        e->omega_call = (long long)call;
+        return 0;
+    }
+    e->hd[s].seed = seed;
     e->desc_dirty = 1;
+    e->omega_call = -1;
     return 0;
 }
 
@@ -960,7 +994,7 @@ static int omega_all(ssa_engine *e)
     for (int nb = 1; nb < NBW; nb++) {
         if (!e->tZ[nb].n) continue;
         prof_begin(e, PH_OMEGA, nb);
-        DEVCHK(ssa_dev_omega(e->prec, e->dd, it_ptr(e, e->tZ[nb]), e->tZ[nb].n, e->arena_a, e->st));
+        DEVCHK(ssa_dev_omega(e->prec, e->dd, it_ptr(e, e->tZ[nb]), e->tZ[nb].n, e->arena_a, e->omega_call, e->st));
         prof_end(e);
     }
     return 0;
@@ -1120,19 +1154,35 @@ int ssa_engine_results(ssa_engine *e, int s, double *sigma, double *lambda, doub
     size_t misc_n = 4 * SSA_BMAX + 3;
     size_t u_n = U ? (size_t)d->ML * d->k : 0, v_n = V ? (size_t)d->K * d->k : 0;
     const int f64 = e->prec == SSA_PREC_F64;
-    /* U and V are page-locked (ssa_engine_host_alloc): FP64 results are copied into them directly */
-    const int du = f64 && u_n, dv = f64 && v_n;
+    const double *dmisc = e->small + d->off_small + (long long)SSA_SLOT_MISC * SSA_BMAX * SSA_BMAX;
+    const char *dU = (const char *)e->arena_a + e->esz * d->off_U, *dV = (const char *)e->arena_a + e->esz * d->off_V;
+    const size_t all = misc_n * 8 + (u_n + v_n) * e->esz;
+    /* Small results are written by one kernel into coherent page-locked
+     * memory: no copy command.  Copies of U and V cost about 10 us each on
+     * an idle copy path, and about 50 us each once the process has freed
+     * large device buffers (measured at N = 500, k = 15); the copy of the
+     * small FP64 block alone did not slow down. */
+    const int direct = all <= RESULTS_DIRECT_MAX;
+    /* large U and V are page-locked (ssa_engine_host_alloc): FP64 results are copied into them directly */
+    const int du = !direct && f64 && u_n, dv = !direct && f64 && v_n;
     const size_t su = du ? 0 : u_n, sv = dv ? 0 : v_n;   /* elements staged */
-    DEVCHK(pin_reserve(e, misc_n * 8 + (su + sv) * e->esz));
-    double *misc = (double *)e->hpin;
-    char *pu = (char *)e->hpin + misc_n * 8;
+    double *misc;
+    if (direct) {
+        DEVCHK(hres_reserve(e, all));
+        DEVCHK(ssa_dev_pack_results(e->prec, dmisc, (int)misc_n, dU, (long long)u_n, dV, (long long)v_n,
+                                    e->hres_dev, e->st));
+        misc = (double *)e->hres;
+    } else {
+        DEVCHK(pin_reserve(e, misc_n * 8 + (su + sv) * e->esz));
+        misc = (double *)e->hpin;
+        char *su_p = (char *)e->hpin + misc_n * 8;
+        HIPCHK(hipMemcpyAsync(misc, dmisc, misc_n * 8, hipMemcpyDeviceToHost, e->st));
+        if (u_n) HIPCHK(hipMemcpyAsync(du ? (void *)U : (void *)su_p, dU, u_n * e->esz, hipMemcpyDeviceToHost, e->st));
+        if (v_n) HIPCHK(hipMemcpyAsync(dv ? (void *)V : (void *)(su_p + su * e->esz), dV, v_n * e->esz,
+                                       hipMemcpyDeviceToHost, e->st));
+    }
+    char *pu = (char *)misc + misc_n * 8;
     char *pv = pu + su * e->esz;
-    HIPCHK(hipMemcpyAsync(misc, e->small + d->off_small + (long long)SSA_SLOT_MISC * SSA_BMAX * SSA_BMAX,
-                          misc_n * 8, hipMemcpyDeviceToHost, e->st));
-    if (u_n) HIPCHK(hipMemcpyAsync(du ? (void *)U : (void *)pu, (char *)e->arena_a + e->esz * d->off_U, u_n * e->esz,
-                                   hipMemcpyDeviceToHost, e->st));
-    if (v_n) HIPCHK(hipMemcpyAsync(dv ? (void *)V : (void *)pv, (char *)e->arena_a + e->esz * d->off_V, v_n * e->esz,
-                                   hipMemcpyDeviceToHost, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
     if (sigma) memcpy(sigma, misc + SSA_MISC_SIGMA, sizeof(double) * d->k);
     if (lambda) memcpy(lambda, misc + SSA_MISC_LAMBDA, sizeof(double) * d->k);
@@ -1236,12 +1286,21 @@ static int launch_recon(ssa_engine *e, int s, int ng, int scale_sigma, int weigh
 static int fetch_out(ssa_engine *e, int s, long long count, double *out)
 {
     const ssa_desc *d = &e->hd[s];
-    DEVCHK(pin_reserve(e, count * e->esz));
-    HIPCHK(hipMemcpyAsync(e->hpin, (char *)e->arena_a + e->esz * d->off_out, count * e->esz, hipMemcpyDeviceToHost,
-                          e->st));
+    const char *src = (const char *)e->arena_a + e->esz * d->off_out;
+    const void *h;
+    if ((size_t)count * e->esz <= RESULTS_DIRECT_MAX) {
+        /* written into host memory by the device, as small decomposition results are */
+        DEVCHK(hres_reserve(e, (size_t)count * e->esz));
+        DEVCHK(ssa_dev_pack_results(e->prec, NULL, 0, src, count, NULL, 0, e->hres_dev, e->st));
+        h = e->hres;
+    } else {
+        DEVCHK(pin_reserve(e, count * e->esz));
+        HIPCHK(hipMemcpyAsync(e->hpin, src, count * e->esz, hipMemcpyDeviceToHost, e->st));
+        h = e->hpin;
+    }
     HIPCHK(hipStreamSynchronize(e->st));
-    if (e->prec == SSA_PREC_F64) memcpy(out, e->hpin, count * 8);
-    else for (long long i = 0; i < count; i++) out[i] = ((float *)e->hpin)[i];
+    if (e->prec == SSA_PREC_F64) memcpy(out, h, count * 8);
+    else for (long long i = 0; i < count; i++) out[i] = ((const float *)h)[i];
     return 0;
 }
 
@@ -1530,6 +1589,11 @@ static int launch_step(ssa_engine *e, int q)
 int ssa_engine_step(ssa_engine *e, int q, int use_graph)
 {
     if (!e->prepared || q < 0) return -1;
+    /* the step (and its graph) reads the counter from the descriptor */
+    if (e->omega_call >= 0) {
+        e->omega_call = -1;
+        e->desc_dirty = 1;
+    }
     DEVCHK(sync_desc(e));
     DEVCHK(prepare_step_groups(e));
     if (!use_graph) return launch_step(e, q);

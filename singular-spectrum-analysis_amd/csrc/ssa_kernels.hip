@@ -185,16 +185,17 @@ __global__ __launch_bounds__((FftGeom<T, LOGN>::NT), 4) void k_spectra(const ssa
 
 // ---------------------------------------------------------------------------
 // Gaussian test matrix into panel Z (K x b): columns 4j .. 4j+3 of row r are
-// the Philox draw (row r, quad j) of the series' key and call counter; FP32
-// panels use the FP32 Box-Muller of the same uniforms.  Item (s, tile of
-// SSA_GRAM_ROWS rows).
+// the Philox draw (row r, quad j) of the series' key and call counter (the
+// descriptor's, or call_arg when it is >= 0); FP32 panels use the FP32
+// Box-Muller of the same uniforms.  Item (s, tile of SSA_GRAM_ROWS rows).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_omega(const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items,
-                                               T *bulk)
+                                               T *bulk, long long call_arg)
 {
     const ssa_item it = items[blockIdx.x];
     const ssa_desc d = D[it.s];
+    const unsigned long long call = call_arg >= 0 ? (unsigned long long)call_arg : d.call;
     const Panel<T> P = panel_of<T>(bulk, d, SSA_PANEL_Z);
     const int r0 = it.x * SSA_GRAM_ROWS, r1 = min(P.rows, r0 + SSA_GRAM_ROWS);
     const int quads = (d.b + 3) / 4;
@@ -205,8 +206,8 @@ __global__ __launch_bounds__(256) void k_omega(const ssa_desc *__restrict__ D, c
         const int nc = min(4, d.b - 4 * j);
         for (int r = r0 + (int)threadIdx.x; r < r1; r += 256) {
             T g[4];
-            if constexpr (sizeof(T) == 4) gauss_quad_f(d.seed, d.call, r, j, g);
-            else gauss_quad(d.seed, d.call, r, j, g);
+            if constexpr (sizeof(T) == 4) gauss_quad_f(d.seed, call, r, j, g);
+            else gauss_quad(d.seed, call, r, j, g);
 #pragma unroll
             for (int t = 0; t < 4; t++)
                 if (t < nc) c0[(long long)t * P.ld + r] = g[t];
@@ -1907,6 +1908,25 @@ __global__ __launch_bounds__(256) void k_gather(const ssa_desc *__restrict__ D, 
     for (int t = threadIdx.x; t < d.N + h; t += 256) dst[t] = src[t];
 }
 
+// The results one decomposition hands to the host: the FP64 block of singular
+// values, eigenvalues and totals (misc_n values), then un values of U and vn
+// of V in the panel type, packed into one buffer (page-locked host memory,
+// written through its device address).
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack_results(const double *__restrict__ misc, int misc_n,
+                                                      const T *__restrict__ U, long long un,
+                                                      const T *__restrict__ V, long long vn, char *out)
+{
+    double *om = reinterpret_cast<double *>(out);
+    T *ouv = reinterpret_cast<T *>(out + 8LL * misc_n);
+    const long long n = misc_n + un + vn, step = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) {
+        if (i < misc_n) om[i] = misc[i];
+        else if (i < misc_n + un) ouv[i - misc_n] = U[i - misc_n];
+        else ouv[i - misc_n] = V[i - misc_n - un];
+    }
+}
+
 // W-correlation normalisation: W_ij = G_ij / sqrt(G_ii G_jj), rows whose
 // sigma * ||h||_w <= 1e-12 are zero (MKL/ssa_opt.h:2466-2471); row-major k x k.
 __global__ __launch_bounds__(256) void k_wnorm(const ssa_desc *D, const ssa_item *items, double *small)
@@ -2189,7 +2209,8 @@ int ssa_dev_mv_acc(int prec, int lgn, int dir, int out_panel, int wide, const ss
     return (int)hipGetLastError();
 }
 
-int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, void *stream)
+int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_items, void *bulk, long long call,
+                  void *stream)
 {
     if (n_items <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
@@ -2198,8 +2219,9 @@ int ssa_dev_omega(int prec, const ssa_desc *D, const ssa_item *items, int n_item
     int py = 1024 / n_items;
     py = py < 1 ? 1 : py > SSA_BMAX / 4 ? SSA_BMAX / 4 : py;
     const dim3 grid(n_items, py);
-    if (prec == SSA_PREC_F64) hipLaunchKernelGGL(k_omega<double>, grid, dim3(256), 0, st, D, items, (double *)bulk);
-    else hipLaunchKernelGGL(k_omega<float>, grid, dim3(256), 0, st, D, items, (float *)bulk);
+    if (prec == SSA_PREC_F64)
+        hipLaunchKernelGGL(k_omega<double>, grid, dim3(256), 0, st, D, items, (double *)bulk, call);
+    else hipLaunchKernelGGL(k_omega<float>, grid, dim3(256), 0, st, D, items, (float *)bulk, call);
     return (int)hipGetLastError();
 }
 
@@ -2485,6 +2507,22 @@ int ssa_dev_gather(int prec, int h, const ssa_desc *D, const ssa_item *items, in
     else
         hipLaunchKernelGGL(k_gather<float>, dim3(n_items), dim3(256), 0, (hipStream_t)stream, D, items,
                            (float *)bulk, h);
+    return (int)hipGetLastError();
+}
+
+int ssa_dev_pack_results(int prec, const double *misc, int misc_n, const void *U, long long un, const void *V,
+                         long long vn, void *out, void *stream)
+{
+    const long long n = (long long)misc_n + un + vn;
+    if (misc_n < 0 || un < 0 || vn < 0 || n <= 0) return -1;
+    const long long blocks = (n + 255) / 256;
+    const dim3 grid((unsigned)(blocks < 256 ? blocks : 256));
+    if (prec == SSA_PREC_F64)
+        hipLaunchKernelGGL(k_pack_results<double>, grid, dim3(256), 0, (hipStream_t)stream, misc, misc_n,
+                           (const double *)U, un, (const double *)V, vn, (char *)out);
+    else
+        hipLaunchKernelGGL(k_pack_results<float>, grid, dim3(256), 0, (hipStream_t)stream, misc, misc_n,
+                           (const float *)U, un, (const float *)V, vn, (char *)out);
     return (int)hipGetLastError();
 }
 
