@@ -30,7 +30,7 @@
 #define NLG 14                 /* lgn buckets 0..13 (6..13 used) */
 #define NBW 6                  /* Gram / apply lists by column blocks nb = 1..5 */
 #define OLAP_TILE_STEP 1024   /* overlap-add samples per work item in the batched step */
-#define RESULTS_DIRECT_MAX (256 << 10)   /* result bytes up to which the device writes them to the host */
+#define RESULTS_DIRECT_MAX (1 << 20)   /* result bytes up to which the device writes them to the host */
 
 typedef struct { long long off; int n; } span;
 
@@ -1161,7 +1161,10 @@ int ssa_engine_results(ssa_engine *e, int s, double *sigma, double *lambda, doub
      * memory: no copy command.  Copies of U and V cost about 10 us each on
      * an idle copy path, and about 50 us each once the process has freed
      * large device buffers (measured at N = 500, k = 15); the copy of the
-     * small FP64 block alone did not slow down. */
+     * small FP64 block alone did not slow down.  Up to 1 MiB the kernel's
+     * writes over the host link were as fast as the copies or faster (N =
+     * 1000, k = 50: 1.05 -> 1.04 ms plain); 8 MiB of FP64 U and V (N =
+     * 20000) took 0.28 ms longer than copying them. */
     const int direct = all <= RESULTS_DIRECT_MAX;
     /* large U and V are page-locked (ssa_engine_host_alloc): FP64 results are copied into them directly */
     const int du = !direct && f64 && u_n, dv = !direct && f64 && v_n;
