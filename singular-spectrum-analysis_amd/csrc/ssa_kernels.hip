@@ -553,12 +553,20 @@ template <int NB> constexpr int n_upper_blocks() { return NB * (NB + 1) / 2; }
 // relative error (about eps32 * sqrt(rows)) is far below what that output
 // needs, unlike the orthonormalisation Grams, whose smallest eigenvalues the
 // Cholesky needs to eps * kappa^2.
+// PAD (Y and Z panels, USE_M = 0): the columns are padded to a multiple of 32
+// rows, so a lane's 4 rows r .. r + 3 exist whenever r is inside the panel;
+// every lane loads unconditionally (lanes outside the panel or its blocks read
+// row r_begin of column 0 and discard it).  Without branches around them the
+// next slab's loads stay in flight while the current slab's products run;
+// with them the compiler waits for all loads before the first product.  The
+// values, and so the sums, are the same either way.
 // ---------------------------------------------------------------------------
-template <typename T, int NB, int USE_M, bool SPLIT, bool ACC32 = false>
+template <typename T, int NB, int USE_M, bool SPLIT, bool ACC32 = false, bool PAD = false>
 __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, const ssa_item *__restrict__ items,
                                               const T *__restrict__ bulk, double *small, int which)
 {
     static_assert(!ACC32 || (sizeof(T) == 4 && !USE_M), "FP32 accumulation is for FP32 panels without M");
+    static_assert(!PAD || !USE_M, "padded loads are for the P^T P form");
     extern __shared__ double sm[];
     constexpr int NBLK = n_upper_blocks<NB>(), MP = small_ld<NB>();
     const ssa_item it = items[blockIdx.x];
@@ -596,8 +604,15 @@ __global__ __launch_bounds__(256) void k_gram(const ssa_desc *__restrict__ D, co
 #pragma unroll
             for (int I = 0; I < NB; I++) {
                 const int c = I * 16 + cl, r = r0 + 4 * kq;
-                if (((need >> I) & 1u) && c < b && r0 < r_end) load4(P.p + (long long)c * P.ld + r, r_end - r, x[I]);
-                else {
+                if constexpr (PAD) {
+                    const bool okc = ((need >> I) & 1u) && c < b, okr = r < r_end;
+                    T v4[4];
+                    load4(P.p + (long long)(okc ? c : 0) * P.ld + (okr ? r : r_begin), 4, v4);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) x[I][v] = (okc && r + v < r_end) ? v4[v] : (T)0;
+                } else if (((need >> I) & 1u) && c < b && r0 < r_end) {
+                    load4(P.p + (long long)c * P.ld + r, r_end - r, x[I]);
+                } else {
 #pragma unroll
                     for (int v = 0; v < 4; v++) x[I][v] = (T)0;
                 }
@@ -2248,15 +2263,23 @@ static void launch_gram(int panel, int use_m, const ssa_desc *D, const ssa_item 
             return;
         }
     }
+    // Y and Z columns are padded (ld a multiple of SSA_PANEL_ALIGN): branch-free loads
+    const bool pad = panel == SSA_PANEL_Y || panel == SSA_PANEL_Z;
     if (py > 1) {
         if (use_m)
             hipLaunchKernelGGL((k_gram<T, NB, 1, true>), grid, dim3(256), red > ms ? red : ms, st, D, items,
+                               (const T *)bulk, small, panel);
+        else if (pad)
+            hipLaunchKernelGGL((k_gram<T, NB, 0, true, false, true>), grid, dim3(256), red, st, D, items,
                                (const T *)bulk, small, panel);
         else
             hipLaunchKernelGGL((k_gram<T, NB, 0, true>), grid, dim3(256), red, st, D, items, (const T *)bulk,
                                small, panel);
     } else if (use_m)
         hipLaunchKernelGGL((k_gram<T, NB, 1, false>), grid, dim3(256), red > ms ? red : ms, st, D, items,
+                           (const T *)bulk, small, panel);
+    else if (pad)
+        hipLaunchKernelGGL((k_gram<T, NB, 0, false, false, true>), grid, dim3(256), red, st, D, items,
                            (const T *)bulk, small, panel);
     else
         hipLaunchKernelGGL((k_gram<T, NB, 0, false>), grid, dim3(256), red, st, D, items, (const T *)bulk, small,
